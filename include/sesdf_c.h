@@ -1,0 +1,223 @@
+/* C ABI of the MI355X solvent-excluded-surface builder (libsesdf_hip.so).
+ *
+ * Plain C types only; every call returns a sesdf_status and never prints. Device pointers in the
+ * returned structs are owned by the handle and stay valid until the next build or destroy, as in
+ * the reference (bcs::Sesdf::getData(), sesdf/src/sesdf/sesdf.cu:302-344).
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference root).
+ */
+#ifndef SESDF_C_H
+#define SESDF_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum
+{
+    SESDF_OK              = 0,
+    SESDF_ERR_INVALID     = 1, /* bad argument */
+    SESDF_ERR_HIP         = 2, /* HIP runtime error (message in sesdf_last_error) */
+    SESDF_ERR_UNSUPPORTED = 3, /* e.g. graphics = 1 (OpenGL interop is not available) */
+    SESDF_ERR_CAPACITY    = 4, /* a capacity could not be grown (out of device memory) */
+    SESDF_ERR_NOT_BUILT   = 5  /* data requested before the first build */
+} sesdf_status;
+
+typedef enum
+{
+    SESDF_MODE_COMPLETE = 0, /* bcs::Sesdf: complete surface (convex sectors, segments, concave) */
+    SESDF_MODE_EXTERIOR = 1  /* bcs::Ssesdf: exterior path (torus records, concave) */
+} sesdf_mode;
+
+typedef struct sesdf_handle sesdf_handle;
+
+/* Constructor arguments of bcs::Sesdf / bcs::Ssesdf (sesdf/include/bcs/sesdf/sesdf.hpp:18,
+ * sesdf/include/bcs/ssesdf/ssesdf.hpp:17-21) plus the capacities the reference fixes at compile
+ * time (sesdf/src/sesdf/sesdf.cu:15-18) and the launch environment. */
+typedef struct
+{
+    const float * xyzr;                 /* host atoms, float4 xyz + vdW radius (non-owning, read at build) */
+    uint32_t      atom_nb;
+    float         probe_radius;         /* reference default 1.4 */
+    float         aabb[ 6 ];            /* min xyz, max xyz of the vdW spheres; all zero: computed */
+    int32_t       mode;                 /* sesdf_mode */
+    uint32_t      max_neighbors;        /* per-atom neighbour capacity K (128 / 288); multiple of 32, <= 320 */
+    uint32_t      max_intersection_neighbors; /* per-probe neighbour capacity (reference 32) */
+    float         cell_size;            /* grid cell edge in A; 0: default */
+    int32_t       device;               /* HIP device ordinal */
+    void *        stream;               /* hipStream_t; NULL: the handle creates its own */
+    int32_t       graphics;             /* must be 0: OpenGL interop is unsupported */
+    int32_t       use_graph;            /* 1: replay the stage sequence as one hipGraph */
+    int32_t       build_surface;        /* 1: build inside sesdf_create (reference buildSurface) */
+} sesdf_create_info;
+
+/* Mirror of bcs::sesdf::SesdfData (sesdf/include/bcs/sesdf/graphics.hpp:39-66). */
+typedef struct
+{
+    uint32_t  atom_nb;
+    float *   atoms;                     /* float4, grid-sorted */
+    uint32_t  convex_patch_nb;
+    uint32_t * convex_patches;           /* uint2 [start, end) per atom */
+    uint32_t  sector_nb;
+    float *   sectors;                   /* float4 (axis, half angle) */
+    uint32_t  segment_patch_nb;
+    uint32_t * segment_patches;          /* uint4 (i, j, intersection A, intersection B) */
+    uint32_t  circle_patch_nb;
+    uint32_t * circle_patches;           /* uint2 (i, j) full circles */
+    uint32_t  concave_patch_nb;
+    float *   concave_patches_position;  /* float4 (probe xyz, +-1 / +-2) */
+    int32_t * concave_patches_id;        /* int4 (i, j, k, neighbour count) */
+    float *   concave_patches_neighbors; /* float4 [max_concave_neighbors * with-neighbour count] */
+    float     probe_radius;
+    uint16_t  max_concave_neighbors;
+} sesdf_data;
+
+/* Mirror of bcs::ssesdf::SsesdfData (sesdf/include/bcs/ssesdf/graphics.hpp:100-119). */
+typedef struct
+{
+    uint32_t  atom_nb;
+    float *   atoms;
+    uint32_t  circle_patch_nb;
+    float *   circle_position;           /* float4 (torus centre, R) */
+    float *   circle_axis;               /* float4 (axis, probe radius) */
+    float *   circle_vs;                 /* float4 visibility sphere */
+    uint32_t  concave_patch_nb;
+    float *   concave_patches_position;
+    int32_t * concave_patches_id;
+    float *   concave_patches_neighbors;
+    float     probe_radius;
+    uint16_t  max_concave_neighbors;
+} ssesdf_data;
+
+/* Counts of the last build (the host counters of bcs::Sesdf, sesdf/include/bcs/sesdf/sesdf.hpp:55-80). */
+typedef struct
+{
+    uint32_t atom_nb;
+    uint32_t neighbor_nb;              /* sum of neighbour counts (ordered circles) */
+    uint32_t sector_nb;                /* V */
+    uint32_t intersected_circle_nb;    /* Ci */
+    uint32_t full_circle_nb;           /* Cf */
+    uint32_t visible_circle_nb;        /* exterior: status != 0 among i < j */
+    uint32_t concave_patch_nb;         /* I */
+    uint32_t concave_with_neighbors_nb;/* Iw */
+    uint32_t segment_nb;               /* G */
+    uint32_t torus_nb;                 /* T */
+    uint32_t neighbor_overflow_atoms;  /* atoms whose neighbour list was truncated at K */
+    uint32_t probe_neighbor_overflow;  /* probes with max_intersection_neighbors neighbours (truncated) */
+    uint32_t circle_intersection_overflow; /* circles with more than 15 intersections */
+    uint32_t jk_anomaly_nb;            /* concave records whose third circle is not intersected */
+    uint32_t fp64_resolve_nb;          /* triplets re-solved in FP64 */
+    uint32_t rebuild_nb;               /* capacity-growth rebuilds performed by the last build */
+    uint32_t max_atom_records;
+    uint64_t device_bytes;             /* device memory held by the handle */
+} sesdf_counts;
+
+/* Device pointers of the internal arrays exposed by the reference's analysis getters
+ * (getDAtomIndices / getDAtomNeighborsCount / getDAtomNeighborsIndices / getDCircleVisibilityStatus /
+ * getDGlobalToTrimmedId, sesdf/include/bcs/sesdf/sesdf.hpp:37-42). */
+typedef struct
+{
+    uint32_t * sorted_to_initial;      /* atom_nb */
+    uint32_t * neighbor_count;         /* atom_nb */
+    uint32_t * neighbor_ids;           /* atom_nb * max_neighbors, sorted indices */
+    uint8_t *  circle_status;          /* atom_nb * max_neighbors */
+    uint32_t * trimmed_to_global;      /* intersected_circle_nb (complete surface) */
+    uint32_t * circle_intersection_nb; /* intersected_circle_nb */
+    uint32_t   max_neighbors;
+    uint16_t   max_intersections_per_circle;
+    uint16_t   max_intersection_neighbors;
+} sesdf_internal;
+
+/* bcs::Sesdf::Sesdf / bcs::Ssesdf::Ssesdf (sesdf/src/sesdf/sesdf.cu:20-75, sesdf/src/ssesdf/ssesdf.cu:19-66). */
+sesdf_status sesdf_create( const sesdf_create_info * info, sesdf_handle ** out );
+/* bcs::Sesdf::~Sesdf (sesdf.cu:159-170). */
+sesdf_status sesdf_destroy( sesdf_handle * h );
+
+/* bcs::Sesdf::build / bcs::Ssesdf::build (sesdf.cu:172-264, ssesdf.cu:133-187): uploads the host
+ * atoms given at create (or by sesdf_set_atoms) and runs the stage sequence. Blocks until counts
+ * are on the host (one readback per build). */
+sesdf_status sesdf_build( sesdf_handle * h );
+/* Same stage sequence on atoms already resident in device memory (float4, atom_nb of them). When
+ * d_xyzr is the handle's own input buffer (sesdf_input_buffer) nothing is copied. */
+sesdf_status sesdf_build_device( sesdf_handle * h, const float * d_xyzr );
+/* Replace the host atom span read by sesdf_build (same atom count). */
+sesdf_status sesdf_set_atoms( sesdf_handle * h, const float * xyzr );
+/* Device input buffer of the handle (float4 * atom_nb). */
+sesdf_status sesdf_input_buffer( sesdf_handle * h, float ** d_xyzr );
+/* Enqueue the stage sequence only (no readback, no synchronisation); counts are read later with
+ * sesdf_sync. Used for timing. */
+sesdf_status sesdf_enqueue( sesdf_handle * h );
+sesdf_status sesdf_sync( sesdf_handle * h );
+
+/* bcs::Sesdf::getData (sesdf.cu:302-344). */
+sesdf_status sesdf_get_data( sesdf_handle * h, sesdf_data * out );
+/* bcs::Ssesdf::getData (ssesdf.cu:212-232). */
+sesdf_status sesdf_get_exterior_data( sesdf_handle * h, ssesdf_data * out );
+sesdf_status sesdf_get_counts( sesdf_handle * h, sesdf_counts * out );
+/* getMaxNeighborPerAtom ... getDGlobalToTrimmedId (sesdf.cu:346-356). */
+sesdf_status sesdf_get_internal( sesdf_handle * h, sesdf_internal * out );
+/* bcs::Sesdf::getAtomIndices (sesdf.cu:350): sorted -> input permutation copied to the host. */
+sesdf_status sesdf_get_atom_indices( sesdf_handle * h, uint32_t * host_out );
+/* Stream of the handle (hipStream_t). */
+sesdf_status sesdf_get_stream( sesdf_handle * h, void ** stream );
+
+/* Per-stage device times of the last timed build, milliseconds (app/detailed/sesdf_benchmark.cu:50-232:
+ * Circle / Intersection / Segment / Neighborhood). stage_ms receives up to 8 values:
+ * grid, circles, intersections, circle scan, fill, segments, probe neighbours, torus. */
+sesdf_status sesdf_build_timed( sesdf_handle * h, float * stage_ms, int stage_count );
+
+/* -------- SDF sampling and its gradient (no reference counterpart: the reference ships no SDF
+ * evaluator; the field is defined as rp + the signed distance to the SAS boundary of the built
+ * surface, clamped to [-band, band]) -------- */
+typedef struct
+{
+    float    origin[ 3 ];
+    float    spacing[ 3 ];
+    uint32_t dims[ 3 ];
+} sesdf_grid_desc;
+
+/* SDF at npoints device points (float4 xyz_), written to d_out (float). */
+sesdf_status sesdf_sample_points( sesdf_handle * h, const float * d_points, uint64_t npoints, float band, float * d_out );
+/* SDF on a regular vertex grid (x fastest), written to d_out (float, dims product). */
+sesdf_status sesdf_sample_grid( sesdf_handle * h, const sesdf_grid_desc * grid, float band, float * d_out );
+/* d(sum_i w_i SDF(q_i)) / d(atom xyz, radius) with the patch topology of the last build held
+ * fixed; d_grad receives float4 per atom in INPUT order (dx, dy, dz, dr). recompute = 1 re-derives
+ * the per-query winning element instead of reading the one stored by the last sample call. */
+sesdf_status sesdf_sdf_backward( sesdf_handle * h, const float * d_points, const float * d_weights, uint64_t npoints, float band,
+                                 int recompute, float * d_grad );
+
+/* -------- Batched build (no reference counterpart: the reference builds one molecule per
+ * handle, app/benchmarks/main.cu:174-186). Molecules are concatenated molecule-major; outputs are
+ * the concatenation with per-molecule offset tables. -------- */
+typedef struct
+{
+    const float *    xyzr;          /* host atoms of all molecules, molecule-major */
+    const uint64_t * atom_offsets;  /* molecule_nb + 1 */
+    uint32_t         molecule_nb;
+    float            probe_radius;
+    int32_t          mode;
+    uint32_t         max_neighbors;
+    uint32_t         max_intersection_neighbors;
+    float            cell_size;     /* 0: default */
+    int32_t          device;
+    void *           stream;
+    int32_t          use_graph;
+} sesdf_batch_info;
+
+sesdf_status sesdf_batch_create( const sesdf_batch_info * info, sesdf_handle ** out );
+/* Per-molecule ranges into the concatenated outputs: out_offsets[(molecule_nb + 1) * 7] holds,
+ * for each of atoms, sectors, full circles, concave with neighbours, concave without neighbours,
+ * segments, torus records, the molecule_nb + 1 prefix offsets (host memory). */
+sesdf_status sesdf_batch_offsets( sesdf_handle * h, uint64_t * out_offsets );
+
+const char * sesdf_status_string( sesdf_status s );
+const char * sesdf_last_error( void );
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SESDF_C_H */

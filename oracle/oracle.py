@@ -1,0 +1,87 @@
+"""ORACLE front-end (test infrastructure only): ctypes over oracle/liboracle.so.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module.
+It runs the CPU restatement of the reference's stage sequence (oracle/sesdf_oracle.cpp) and the
+FP64 SDF / gradient oracle (oracle/sdf_oracle.cpp) and returns numpy arrays.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+ARRAYS = {
+    "atoms": np.float32, "sorted_to_initial": np.uint32, "nb_count": np.uint32, "nb_ids": np.uint32,
+    "status": np.uint8, "trimmed_to_global": np.uint32, "convex_patches": np.uint32, "sectors": np.float32,
+    "full_circles": np.uint32, "concave_ids": np.int32, "concave_pos": np.float32, "concave_pos64": np.float64,
+    "concave_nb": np.float32, "circle_int_nb": np.uint32, "segments": np.uint32, "torus": np.float32,
+    "torus_slots": np.uint32, "amb_pairs": np.uint32, "amb_circle": np.uint8, "amb_concave": np.uint8,
+    "amb_hasnb": np.uint8, "amb_rejected": np.uint32, "amb_seg_circle": np.uint8, "amb_probe_nb": np.uint8,
+    "stats": np.uint64, "grid": np.float32, "counters": np.uint64,
+}
+SHAPES = {"atoms": 4, "convex_patches": 2, "sectors": 4, "full_circles": 2, "concave_ids": 4, "concave_pos": 4,
+          "concave_pos64": 3, "concave_nb": 4, "segments": 4, "amb_pairs": 2, "amb_rejected": 4}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.oracle_build.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_uint32, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+        L.oracle_build.restype = ctypes.c_void_p
+        L.oracle_free.argtypes = [ctypes.c_void_p]
+        L.oracle_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_get.restype = ctypes.c_uint64
+        if hasattr(L, "oracle_sdf"):
+            L.oracle_sdf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+            L.oracle_sdf.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+class OracleResult:
+    """Arrays of one CPU build (reference grid-sorted index space, see sorted_to_initial)."""
+
+    def __init__(self, handle, mode, n, K, rp):
+        self._h = handle
+        self.mode, self.n, self.K, self.rp = mode, n, K, rp
+        self._cache = {}
+
+    def __getitem__(self, name):
+        if name not in self._cache:
+            L = lib()
+            es = ctypes.c_uint32(0)
+            cnt = L.oracle_get(self._h, name.encode(), None, ctypes.byref(es))
+            arr = np.zeros(cnt, ARRAYS[name])
+            assert cnt == 0 or arr.itemsize == es.value, (name, arr.itemsize, es.value)
+            if cnt:
+                L.oracle_get(self._h, name.encode(), ctypes.c_void_p(arr.ctypes.data), None)
+            if name in SHAPES:
+                arr = arr.reshape(-1, SHAPES[name])
+            self._cache[name] = arr
+        return self._cache[name]
+
+    @property
+    def iw(self):
+        return int(self["counters"][0])
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().oracle_free(self._h)
+            self._h = None
+
+
+def build(atoms, probe_radius=1.4, mode=0, precision=64, max_neighbors=128, track=True, delta=1e-3,
+          delta_theta=1e-4, threads=1):
+    """Run the oracle build. mode 0 = complete surface (Sesdf), 1 = exterior path (Ssesdf)."""
+    a = np.ascontiguousarray(atoms, np.float32)
+    h = lib().oracle_build(ctypes.c_void_p(a.ctypes.data), a.shape[0], probe_radius, mode, precision,
+                           max_neighbors, int(track), delta, delta_theta, threads)
+    return OracleResult(h, mode, a.shape[0], max_neighbors, probe_radius)

@@ -1,0 +1,1062 @@
+// ORACLE — CPU restatement of PlathC/SESDF's solvent-excluded-surface construction.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker for the HIP product path: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it. The product library never
+// links it and never falls back to it.
+//
+// It restates, stage by stage and in the reference's order and predicates, the host classes
+// bcs::Sesdf::build() (sesdf/src/sesdf/sesdf.cu:172-264, complete surface) and
+// bcs::Ssesdf::build() (sesdf/src/ssesdf/ssesdf.cu:133-187, exterior path), templated on the
+// scalar type:
+//   * T = float  : plain IEEE FP32 (no fast-math emulation) -- the host C++ baseline;
+//   * T = double : the accuracy oracle. With tracking on, every threshold predicate is also
+//                  classified robust / ambiguous against an epsilon band (|value - threshold| <= delta)
+//                  so that GPU-vs-oracle topology differences can be attributed to near-degenerate
+//                  cases and counted.
+// All indices are in the reference's grid-sorted atom order; sortedToInitial maps them back.
+//
+// Parity pins (tests/test_oracle_*.py): brute-force O(N^2)/O(N^3) enumeration on small inputs,
+// closed forms for 1/2/3/4-atom configurations, and the invariants sum(per-circle counts) = 3I,
+// even per-circle counts, G = 3I/2, status(ij) = status(ji), i<j<k in every concave id.
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <thread>
+#include <type_traits>
+#include <vector>
+
+namespace orc
+{
+    template<class T>
+    struct V3
+    {
+        T x, y, z;
+    };
+    template<class T> inline V3<T> operator+( V3<T> a, V3<T> b ) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
+    template<class T> inline V3<T> operator-( V3<T> a, V3<T> b ) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+    template<class T> inline V3<T> operator*( V3<T> a, T s ) { return { a.x * s, a.y * s, a.z * s }; }
+    template<class T> inline V3<T> operator*( T s, V3<T> a ) { return { s * a.x, s * a.y, s * a.z }; }
+    template<class T> inline V3<T> operator/( V3<T> a, T s ) { return { a.x / s, a.y / s, a.z / s }; }
+    template<class T> inline T     dot( V3<T> a, V3<T> b ) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+    template<class T> inline T     len2( V3<T> a ) { return dot( a, a ); }
+    template<class T> inline T     len( V3<T> a ) { return std::sqrt( dot( a, a ) ); }
+    template<class T> inline V3<T> cross( V3<T> a, V3<T> b )
+    {
+        return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x };
+    }
+    // helper_math normalize: v * rsqrt(dot(v, v))
+    template<class T> inline V3<T> normalize( V3<T> v ) { return v * ( T( 1 ) / std::sqrt( dot( v, v ) ) ); }
+    template<class T> inline T     sgn( T f ) { return std::copysign( T( 1 ), f ); } // math.cuh:13-16
+    template<class T> inline T     clampv( T v, T lo, T hi ) { return std::min( std::max( v, lo ), hi ); }
+    template<class T> inline T     det3( V3<T> a, V3<T> b, V3<T> c ) { return dot( a, cross( b, c ) ); } // math.cuh:27-30
+
+    struct AtomF
+    {
+        float x, y, z, r;
+    };
+
+    // bcs::GridInfo (sesdf/include/bcs/cuda/grid.cuh:8-17, grid.inl:8-28)
+    struct GridInfo
+    {
+        float ox, oy, oz;
+        float cx, cy, cz;
+        int   size;
+
+        void pos( float px, float py, float pz, int & gx, int & gy, int & gz ) const
+        {
+            gx = static_cast<int>( std::floor( ( px - ox ) / cx ) );
+            gy = static_cast<int>( std::floor( ( py - oy ) / cy ) );
+            gz = static_cast<int>( std::floor( ( pz - oz ) / cz ) );
+        }
+        uint32_t hash( int gx, int gy, int gz ) const
+        {
+            gx &= ( size - 1 );
+            gy &= ( size - 1 );
+            gz &= ( size - 1 );
+            return uint32_t( gz * size * size + gy * size + gx );
+        }
+    };
+
+    constexpr float    MaxVdwRadius = 3.48f;            // sesdf.cu:23, operations.cu:80
+    constexpr float    Eps          = 1e-4f;            // predicate slack used throughout the reference
+    constexpr uint32_t Empty        = 0xffffffffu;      // AccelerationGrid::EmptyGridCellValue
+    constexpr float    TwoPif       = 6.283185307f;     // core/math.hpp
+
+    // nextPowerOfTwoValue(nextPowerOfTwoExponent(n)) (core/math.inl:5-19)
+    inline uint32_t referenceGridSize( uint32_t n )
+    {
+        uint32_t e = 0, b = n;
+        while ( b >>= 1 )
+            e++;
+        uint32_t v = 1;
+        while ( e > v )
+            v <<= 1;
+        return v;
+    }
+
+    // Constructor grid configuration (sesdf.cu:23-33) from the vdW AABB (molecule.cpp:132-143).
+    inline GridInfo referenceGrid( const AtomF * atoms, uint32_t n, float rp )
+    {
+        float mn[ 3 ] = { 3.4e38f, 3.4e38f, 3.4e38f }, mx[ 3 ] = { -3.4e38f, -3.4e38f, -3.4e38f };
+        for ( uint32_t i = 0; i < n; i++ )
+        {
+            const float p[ 3 ] = { atoms[ i ].x, atoms[ i ].y, atoms[ i ].z };
+            for ( int d = 0; d < 3; d++ )
+            {
+                mn[ d ] = std::min( p[ d ] - atoms[ i ].r, mn[ d ] );
+                mx[ d ] = std::max( p[ d ] + atoms[ i ].r, mx[ d ] );
+            }
+        }
+        GridInfo       g;
+        const uint32_t gs = referenceGridSize( n );
+        float          o[ 3 ], c[ 3 ];
+        for ( int d = 0; d < 3; d++ )
+        {
+            o[ d ]           = mn[ d ] - MaxVdwRadius - rp;
+            const float size = std::fabs( mx[ d ] + MaxVdwRadius + rp - o[ d ] );
+            c[ d ]           = size / float( gs );
+        }
+        g.ox = o[ 0 ], g.oy = o[ 1 ], g.oz = o[ 2 ];
+        g.cx = c[ 0 ], g.cy = c[ 1 ], g.cz = c[ 2 ];
+        g.size = int( gs );
+        return g;
+    }
+
+    // AccelerationGrid::build (grid.cu:99-132): hash, stable sort by hash, cell start/end.
+    struct SortedGrid
+    {
+        std::vector<uint32_t> sortedIdx;  // sorted -> initial
+        std::vector<uint32_t> cellStart, cellEnd;
+    };
+
+    template<class P>
+    SortedGrid buildGrid( const GridInfo & g, uint32_t n, P posOf )
+    {
+        SortedGrid            sg;
+        std::vector<uint32_t> hashes( n );
+        for ( uint32_t i = 0; i < n; i++ )
+        {
+            float x, y, z;
+            posOf( i, x, y, z );
+            int gx, gy, gz;
+            g.pos( x, y, z, gx, gy, gz );
+            hashes[ i ] = g.hash( gx, gy, gz );
+        }
+        sg.sortedIdx.resize( n );
+        for ( uint32_t i = 0; i < n; i++ )
+            sg.sortedIdx[ i ] = i;
+        std::stable_sort( sg.sortedIdx.begin(), sg.sortedIdx.end(), [ & ]( uint32_t a, uint32_t b ) { return hashes[ a ] < hashes[ b ]; } );
+        const size_t cells = size_t( g.size ) * g.size * g.size;
+        sg.cellStart.assign( cells, Empty );
+        sg.cellEnd.assign( cells, 0 );
+        for ( uint32_t s = 0; s < n; s++ )
+        {
+            const uint32_t h = hashes[ sg.sortedIdx[ s ] ];
+            if ( s == 0 || h != hashes[ sg.sortedIdx[ s - 1 ] ] )
+                sg.cellStart[ h ] = s;
+            sg.cellEnd[ h ] = s + 1;
+        }
+        return sg;
+    }
+
+    template<class Fn>
+    void parallelFor( uint32_t n, int threads, Fn fn )
+    {
+        if ( threads <= 1 || n < 256 )
+        {
+            for ( uint32_t i = 0; i < n; i++ )
+                fn( i );
+            return;
+        }
+        std::atomic<uint32_t>    next { 0 };
+        std::vector<std::thread> pool;
+        for ( int t = 0; t < threads; t++ )
+            pool.emplace_back(
+                [ & ]
+                {
+                    for ( ;; )
+                    {
+                        const uint32_t b = next.fetch_add( 64 );
+                        if ( b >= n )
+                            break;
+                        const uint32_t e = std::min( n, b + 64 );
+                        for ( uint32_t i = b; i < e; i++ )
+                            fn( i );
+                    }
+                } );
+        for ( auto & th : pool )
+            th.join();
+    }
+
+    // Output of one build, mirroring SesdfData / SsesdfData (sesdf/include/bcs/sesdf/graphics.hpp:39-66,
+    // ssesdf/graphics.hpp:100-119) plus the internal arrays the reference exposes through getters.
+    struct Result
+    {
+        int      mode = 0; // 0 complete (Sesdf), 1 exterior path (Ssesdf)
+        uint32_t n = 0, K = 0;
+        float    rp = 0;
+        GridInfo grid {};
+
+        std::vector<float>    atoms;            // sorted float4
+        std::vector<uint32_t> sortedToInitial;  // n
+        std::vector<uint32_t> nbCount;          // n
+        std::vector<uint32_t> nbIds;            // n*K
+        std::vector<uint8_t>  status;           // n*K (final statuses, after exterior promotion)
+        std::vector<uint32_t> trimmedToGlobal;  // Ci (complete)
+        std::vector<uint32_t> convexPatches;    // uint2 n (complete)
+        std::vector<float>    sectors;          // float4 V (complete)
+        std::vector<uint32_t> fullCircles;      // uint2 Cf (complete)
+        std::vector<int32_t>  concaveIds;       // int4 I
+        std::vector<float>    concavePos;       // float4 I (FP32-rounded)
+        std::vector<double>   concavePos64;     // xyz I at T precision
+        std::vector<float>    concaveNb;        // float4 32*Iw
+        std::vector<uint32_t> circleIntNb;      // per trimmed circle intersection count (complete)
+        std::vector<uint32_t> segments;         // uint4 G (complete)
+        std::vector<float>    torus;            // 3 x float4 T (exterior): pos[T], axis[T], vs[T]
+        std::vector<uint32_t> torusSlots;       // global slot id per torus record
+        uint32_t              Iw = 0;
+
+        // epsilon-band tracking (double precision with track = true)
+        std::vector<uint32_t> ambPairs;       // (i, j) sorted idx, i < j: neighbour predicate within delta
+        std::vector<uint8_t>  ambCircle;      // n*K: status decision within delta
+        std::vector<uint8_t>  ambConcave;     // I: existence/side ambiguous
+        std::vector<uint8_t>  ambHasNb;       // I: hasNeighbors predicate within delta
+        std::vector<uint32_t> ambRejected;    // (i, j, k, side) candidates rejected within delta
+        std::vector<uint8_t>  ambSegCircle;   // per trimmed circle: its segments are ambiguous
+        std::vector<uint8_t>  ambProbeNb;     // Iw: probe neighbour set ambiguous
+        uint64_t              anomalyJk = 0;  // records whose jk circle is not status 1
+        uint64_t              overflowNb = 0, overflowProbeNb = 0, overflowCircleInt = 0;
+        std::vector<uint64_t> stats;          // predicate evaluation counts (visibility tests, occluder tests, ...)
+    };
+
+    template<class T>
+    struct Geometry
+    {
+        V3<T> center, normal;
+        T     radius;
+    };
+
+    // SesdfContext::getCircleGeometry (sesdf/include/bcs/sesdf/data.cuh:98-114) with
+    // getCircleCenter (cuda/circles.cuh:9-14).
+    template<class T>
+    inline Geometry<T> circleGeometry( V3<T> ci, T Ri, V3<T> cj, T Rj )
+    {
+        const T     d = len2( ci - cj );
+        const T     t = ( Ri * Ri - Rj * Rj + d ) / ( T( 2 ) * d );
+        Geometry<T> g;
+        g.center                = ci + ( cj - ci ) * t;
+        const V3<T> circleToI   = ci - g.center;
+        const T     sqDist      = dot( circleToI, circleToI );
+        g.radius                = std::sqrt( Ri * Ri - sqDist );
+        g.normal                = normalize( cj - ci );
+        return g;
+    }
+
+    // bcs::orthogonalVector (cuda/math.cuh:32-38)
+    template<class T>
+    inline V3<T> orthogonalVector( V3<T> n )
+    {
+        V3<T> ref { 1, 0, 0 };
+        if ( std::fabs( dot( n, ref ) ) > T( 1 ) - T( 1e-4f ) )
+            ref = { 0, 1, 0 };
+        return normalize( cross( n, ref ) );
+    }
+
+    // bcs::angleBetweenEdges (cuda/circles.cuh:27-31)
+    template<class T>
+    inline T angleBetweenEdges( V3<T> u, V3<T> v, V3<T> n )
+    {
+        const T theta = std::acos( clampv( dot( u, v ) / ( len( u ) * len( v ) ), T( -1 ), T( 1 ) ) );
+        return sgn( det3( u, v, n ) ) > T( 0 ) ? theta : T( TwoPif ) - theta;
+    }
+
+    struct Record
+    {
+        uint32_t i, a, j, b, k, jkIdx;
+        int      side;      // +1: x1 = xc - n, -1: x2 = xc + n
+        int      hasNb;
+        uint8_t  amb, ambHas;
+    };
+
+    template<class T>
+    struct Builder
+    {
+        const AtomF *      input;
+        uint32_t           n;
+        T                  rp;
+        float              rpf;
+        uint32_t           K;
+        int                mode;
+        bool               track;
+        double             delta, deltaTheta;
+        int                threads;
+        Result &           res;
+
+        std::vector<V3<T>> c;   // sorted centres
+        std::vector<T>     r;   // sorted vdW radii
+        std::vector<T>     R;   // r + rp
+
+        Builder( const AtomF * in, uint32_t n_, float rp_, uint32_t K_, int mode_, bool track_, double d_, double dt_, int th, Result & out ) :
+            input( in ), n( n_ ), rp( T( rp_ ) ), rpf( rp_ ), K( K_ ), mode( mode_ ), track( track_ ), delta( d_ ), deltaTheta( dt_ ),
+            threads( th ), res( out )
+        {
+        }
+
+        bool inBand( T v ) const { return track && std::fabs( double( v ) ) <= delta; }
+
+        // -- #2 findCircles: grid, sort, neighbours (operations.inl:14-30, operations.cu:34-124) ----------
+        void neighbours()
+        {
+            res.grid       = referenceGrid( input, n, rpf );
+            SortedGrid sg  = buildGrid( res.grid, n, [ & ]( uint32_t i, float & x, float & y, float & z ) { x = input[ i ].x, y = input[ i ].y, z = input[ i ].z; } );
+            res.sortedToInitial = sg.sortedIdx;
+            cellStart_          = sg.cellStart;
+            cellEnd_            = sg.cellEnd;
+            res.atoms.resize( size_t( n ) * 4 );
+            c.resize( n ), r.resize( n ), R.resize( n );
+            for ( uint32_t s = 0; s < n; s++ )
+            {
+                const AtomF & a = input[ sg.sortedIdx[ s ] ];
+                res.atoms[ 4 * s + 0 ] = a.x, res.atoms[ 4 * s + 1 ] = a.y, res.atoms[ 4 * s + 2 ] = a.z, res.atoms[ 4 * s + 3 ] = a.r;
+                c[ s ] = { T( a.x ), T( a.y ), T( a.z ) };
+                r[ s ] = T( a.r );
+                R[ s ] = T( a.r ) + rp;
+            }
+            res.nbCount.assign( n, 0 );
+            res.nbIds.assign( size_t( n ) * K, 0 );
+            std::vector<std::vector<uint32_t>> amb( threads > 1 ? n : 0 );
+            std::vector<uint32_t>              ambSeq;
+            std::atomic<uint64_t>              overflow { 0 };
+            const GridInfo &                   g = res.grid;
+            parallelFor( n, threads, [ & ]( uint32_t i ) {
+                const AtomF a { res.atoms[ 4 * i ], res.atoms[ 4 * i + 1 ], res.atoms[ 4 * i + 2 ], res.atoms[ 4 * i + 3 ] };
+                int         gx, gy, gz;
+                g.pos( a.x, a.y, a.z, gx, gy, gz );
+                const float range = a.r + MaxVdwRadius + 2.f * rpf;
+                const int   rx = std::max( int( std::ceil( range / g.cx ) ), 1 );
+                const int   ry = std::max( int( std::ceil( range / g.cy ) ), 1 );
+                const int   rz = std::max( int( std::ceil( range / g.cz ) ), 1 );
+                const int   sx = std::max( gx - rx, 0 ), sy = std::max( gy - ry, 0 ), sz = std::max( gz - rz, 0 );
+                const int   ex = std::min( gx + rx, g.size - 1 ), ey = std::min( gy + ry, g.size - 1 ), ez = std::min( gz + rz, g.size - 1 );
+                uint32_t    count = 0;
+                bool        full  = false;
+                for ( int z = sz; z <= ez && !full; z++ )
+                    for ( int y = sy; y <= ey && !full; y++ )
+                        for ( int x = sx; x <= ex && !full; x++ )
+                        {
+                            const uint32_t h  = g.hash( x, y, z );
+                            const uint32_t st = sgCellStart( h );
+                            if ( st == Empty )
+                                continue;
+                            const uint32_t en = sgCellEnd( h );
+                            for ( uint32_t j = st; j < en; j++ )
+                            {
+                                if ( j == i )
+                                    continue;
+                                const T d = len2( c[ i ] - c[ j ] );
+                                const T e = R[ i ] + R[ j ];
+                                const T m = d - e * e;
+                                if ( track && j > i && inBand( m - T( Eps ) ) )
+                                {
+                                    if ( threads > 1 )
+                                        amb[ i ].push_back( j );
+                                    else
+                                        ambSeq.push_back( i ), ambSeq.push_back( j );
+                                }
+                                if ( m <= T( Eps ) )
+                                {
+                                    res.nbIds[ size_t( i ) * K + count ] = j;
+                                    count++;
+                                    if ( count == K )
+                                    {
+                                        overflow++;
+                                        full = true;
+                                        break;
+                                    }
+                                }
+                            }
+                        }
+                res.nbCount[ i ] = count;
+            } );
+            res.overflowNb = overflow;
+            if ( threads > 1 )
+            {
+                for ( uint32_t i = 0; i < n; i++ )
+                    for ( uint32_t j : amb[ i ] )
+                        res.ambPairs.push_back( i ), res.ambPairs.push_back( j );
+            }
+            else
+                res.ambPairs = ambSeq;
+        }
+        // grid lookups bound after neighbours() builds the grid
+        std::vector<uint32_t> cellStart_, cellEnd_;
+        uint32_t              sgCellStart( uint32_t h ) const { return cellStart_[ h ]; }
+        uint32_t              sgCellEnd( uint32_t h ) const { return cellEnd_[ h ]; }
+
+        // -- computeCirclesVisibilityStatus (sesdf operations.cu:126-204; ssesdf operations.cu:169-244) ------
+        void visibility()
+        {
+            res.status.assign( size_t( n ) * K, 0 );
+            res.ambCircle.assign( track ? size_t( n ) * K : 0, 0 );
+            std::atomic<uint64_t> tests { 0 };
+            parallelFor( n, threads, [ & ]( uint32_t i ) {
+                const uint32_t ni    = res.nbCount[ i ];
+                uint64_t       local = 0;
+                for ( uint32_t a = 0; a < ni; a++ )
+                {
+                    const uint32_t j = res.nbIds[ size_t( i ) * K + a ];
+                    if ( mode == 1 && !( i < j ) )
+                        continue; // exterior path: only i < j is evaluated (ssesdf operations.cu:200-204)
+                    const Geometry<T> g       = circleGeometry( c[ i ], R[ i ], c[ j ], R[ j ] );
+                    uint32_t          visible = 1, complete = 1;
+                    bool              hiddenRobust = false, hiddenAmb = false, nearRobust = false, nearAmb = false;
+                    for ( uint32_t kk = 0; kk < ni; kk++ )
+                    {
+                        if ( kk == a )
+                            continue;
+                        const uint32_t k  = res.nbIds[ size_t( i ) * K + kk ];
+                        const T        Rk = R[ k ];
+                        local++;
+                        const V3<T> kToCircle      = g.center - c[ k ];
+                        const T     distToCircle   = len( kToCircle );
+                        const T     secondDistance = dot( kToCircle, g.normal );
+                        const T     cosTheta       = secondDistance / distToCircle;
+                        const T     sinTheta       = std::sqrt( T( 1 ) - cosTheta * cosTheta );
+                        const T     firstFar       = sinTheta * distToCircle + g.radius;
+                        const T     far            = firstFar * firstFar + secondDistance * secondDistance;
+                        const T     mFar           = far - Rk * Rk - T( Eps );
+                        const T     firstNear      = ( -sinTheta ) * distToCircle + g.radius;
+                        const T     nearv          = firstNear * firstNear + secondDistance * secondDistance;
+                        const T     mNear          = nearv - Rk * Rk - T( Eps );
+                        if ( track )
+                        {
+                            // evaluate every k so that the decision can be classified robust / ambiguous
+                            if ( inBand( mFar ) )
+                                hiddenAmb = true;
+                            else if ( mFar < 0 )
+                                hiddenRobust = true;
+                            if ( inBand( mNear ) )
+                                nearAmb = true;
+                            else if ( mNear < 0 )
+                                nearRobust = true;
+                        }
+                        if ( mFar < 0 )
+                        {
+                            visible = 0;
+                            if ( !track )
+                                break;
+                        }
+                        if ( complete && mNear < 0 )
+                            complete = 0;
+                    }
+                    res.status[ size_t( i ) * K + a ] = uint8_t( visible * ( 1 + complete ) );
+                    if ( track )
+                    {
+                        // hidden decided robustly if some k hides robustly; otherwise ambiguous if any k is in band.
+                        bool amb = false;
+                        if ( !hiddenRobust )
+                        {
+                            if ( hiddenAmb )
+                                amb = true;
+                            else if ( !nearRobust && nearAmb )
+                                amb = true;
+                        }
+                        res.ambCircle[ size_t( i ) * K + a ] = amb;
+                    }
+                }
+                tests += local;
+            } );
+            res.stats.push_back( tests );
+        }
+
+        // -- trimmed ids, sectors, full circles (operations.inl:32-93, 175-228) ---------------------------
+        std::vector<uint32_t> globalToTrimmedScan;  // exclusive scan of the intersected mask, n*K+1
+        void compaction()
+        {
+            if ( mode != 0 )
+                return;
+            const size_t total = size_t( n ) * K;
+            globalToTrimmedScan.assign( total + 1, 0 );
+            uint32_t acc = 0;
+            for ( size_t g = 0; g < total; g++ )
+            {
+                globalToTrimmedScan[ g ] = acc;
+                const uint32_t i = uint32_t( g / K ), a = uint32_t( g % K );
+                if ( a < res.nbCount[ i ] && res.status[ g ] == 1 && res.nbIds[ g ] > i )
+                {
+                    res.trimmedToGlobal.push_back( uint32_t( g ) );
+                    acc++;
+                }
+            }
+            globalToTrimmedScan[ total ] = acc;
+            res.convexPatches.resize( size_t( n ) * 2 );
+            uint32_t written = 0;
+            for ( uint32_t i = 0; i < n; i++ )
+            {
+                res.convexPatches[ 2 * i ] = written;
+                for ( uint32_t a = 0; a < res.nbCount[ i ]; a++ )
+                {
+                    const size_t   g = size_t( i ) * K + a;
+                    const uint32_t j = res.nbIds[ g ];
+                    if ( res.status[ g ] == 0 )
+                        continue;
+                    const T           rirp   = R[ i ];
+                    const V3<T>       cij    = circleGeometry( c[ i ], R[ i ], c[ j ], R[ j ] ).center;
+                    const V3<T>       cijci  = c[ i ] - cij;
+                    const T           radius = std::sqrt( rirp * rirp - dot( cijci, cijci ) );
+                    const V3<T>       cicj   = normalize( c[ j ] - c[ i ] );
+                    const V3<T>       nn     = orthogonalVector( cicj );
+                    const V3<T>       x      = cij + nn * radius;
+                    const T           ang    = std::acos( clampv( dot( cicj, ( x - c[ i ] ) / rirp ), T( -1 ), T( 1 ) ) );
+                    res.sectors.push_back( float( cicj.x ) );
+                    res.sectors.push_back( float( cicj.y ) );
+                    res.sectors.push_back( float( cicj.z ) );
+                    res.sectors.push_back( float( ang ) );
+                    written++;
+                    if ( res.status[ g ] == 2 && j > i )
+                        res.fullCircles.push_back( i ), res.fullCircles.push_back( j );
+                }
+                res.convexPatches[ 2 * i + 1 ] = written;
+            }
+        }
+
+        int findInNeighbors( uint32_t j, uint32_t k ) const
+        {
+            const uint32_t nj = res.nbCount[ j ];
+            for ( uint32_t q = 0; q < nj; q++ )
+                if ( res.nbIds[ size_t( j ) * K + q ] == k )
+                    return int( q );
+            return -1;
+        }
+        bool pairAmbiguous( uint32_t a, uint32_t b ) const
+        {
+            if ( !track )
+                return false;
+            const uint32_t lo = std::min( a, b ), hi = std::max( a, b );
+            return std::binary_search( ambPairSorted.begin(), ambPairSorted.end(), ( uint64_t( lo ) << 32 ) | hi );
+        }
+        std::vector<uint64_t> ambPairSorted;
+
+        // -- findIntersectionsBetweenCircles (sesdf operations.inl:230-486; ssesdf operation.inl:100-351) ------
+        std::vector<Record> records;
+        void intersections()
+        {
+            if ( track )
+            {
+                for ( size_t q = 0; q + 1 < res.ambPairs.size(); q += 2 )
+                    ambPairSorted.push_back( ( uint64_t( res.ambPairs[ q ] ) << 32 ) | res.ambPairs[ q + 1 ] );
+                std::sort( ambPairSorted.begin(), ambPairSorted.end() );
+            }
+            std::vector<std::vector<Record>>   perAtom( n );
+            std::vector<std::vector<uint32_t>> rejected( track ? n : 0 );
+            std::atomic<uint64_t>              candTests { 0 }, occTests { 0 };
+            parallelFor( n, threads, [ & ]( uint32_t i ) {
+                const uint32_t ni = res.nbCount[ i ];
+                auto           isCand = [ & ]( uint32_t a, bool & amb ) {
+                    const size_t   g = size_t( i ) * K + a;
+                    const uint32_t j = res.nbIds[ g ];
+                    amb              = track && res.ambCircle[ g ];
+                    return j > i && ( res.status[ g ] == 1 || amb );
+                };
+                uint64_t ct = 0, ot = 0;
+                for ( uint32_t a = 0; a < ni; a++ )
+                {
+                    bool ambA;
+                    if ( !isCand( a, ambA ) )
+                        continue;
+                    const uint32_t j     = res.nbIds[ size_t( i ) * K + a ];
+                    const bool     realA = res.status[ size_t( i ) * K + a ] == 1;
+                    for ( uint32_t b = 0; b < ni; b++ )
+                    {
+                        bool ambB;
+                        if ( !isCand( b, ambB ) )
+                            continue;
+                        const uint32_t k = res.nbIds[ size_t( i ) * K + b ];
+                        if ( !( j < k ) )
+                            continue;
+                        const bool realB  = res.status[ size_t( i ) * K + b ] == 1;
+                        const int  jkIdx  = findInNeighbors( j, k );
+                        const bool ambJk  = pairAmbiguous( j, k );
+                        const bool real   = realA && realB && jkIdx >= 0;
+                        const bool ambAny = ambA || ambB || ambJk || pairAmbiguous( i, j ) || pairAmbiguous( i, k );
+                        if ( !real && !ambAny )
+                            continue;
+                        ct++;
+                        const Geometry<T> ij = circleGeometry( c[ i ], R[ i ], c[ j ], R[ j ] );
+                        const Geometry<T> ik = circleGeometry( c[ i ], R[ i ], c[ k ], R[ k ] );
+                        const V3<T>       u  = ik.normal - dot( ik.normal, ij.normal ) * ij.normal;
+                        const V3<T>       xc = ij.center + ( u * dot( ik.center - ij.center, ik.normal ) ) / dot( u, ik.normal );
+                        const T           t  = -len2( xc - c[ i ] ) + R[ i ] * R[ i ];
+                        const bool        tAmb = inBand( t );
+                        if ( !( t >= T( 0 ) ) )
+                        {
+                            if ( tAmb )
+                                for ( int side : { 1, -1 } )
+                                    rejected[ i ].insert( rejected[ i ].end(), { i, j, k, uint32_t( side ) } );
+                            continue;
+                        }
+                        const V3<T> nn = std::sqrt( t ) * normalize( cross( ik.normal, ij.normal ) );
+                        const V3<T> xs[ 2 ] = { xc - nn, xc + nn };
+                        bool        vis[ 2 ] = { true, true }, amb[ 2 ] = { false, false }, robustHidden[ 2 ] = { false, false };
+                        for ( uint32_t l = 0; l < ni; l++ )
+                        {
+                            if ( l == a || l == b )
+                                continue;
+                            const uint32_t li = res.nbIds[ size_t( i ) * K + l ];
+                            const T        Rl2 = R[ li ] * R[ li ];
+                            for ( int s = 0; s < 2; s++ )
+                            {
+                                ot++;
+                                const T m = len2( c[ li ] - xs[ s ] ) - Rl2;
+                                if ( inBand( m ) )
+                                    amb[ s ] = true;
+                                else if ( m < 0 )
+                                    robustHidden[ s ] = true;
+                                vis[ s ] = vis[ s ] && m > T( 0 );
+                            }
+                            if ( !track && !vis[ 0 ] && !vis[ 1 ] )
+                                break;
+                        }
+                        const V3<T> p2p3 = normalize( c[ j ] - c[ i ] );
+                        const V3<T> p4p3 = normalize( c[ j ] - c[ k ] );
+                        const V3<T> pn   = normalize( cross( p2p3, p4p3 ) );
+                        for ( int s = 0; s < 2; s++ )
+                        {
+                            const bool sideAmb = ambAny || tAmb || ( amb[ s ] && !robustHidden[ s ] );
+                            if ( real && vis[ s ] )
+                            {
+                                const T d      = dot( c[ i ] - xs[ s ], pn );
+                                const T hm     = d * d - rp * rp + T( Eps );
+                                Record  rec;
+                                rec.i = i, rec.a = a, rec.j = j, rec.b = b, rec.k = k, rec.jkIdx = uint32_t( jkIdx );
+                                rec.side   = s == 0 ? 1 : -1;
+                                rec.hasNb  = hm < T( 0 );
+                                rec.amb    = sideAmb;
+                                rec.ambHas = inBand( hm );
+                                perAtom[ i ].push_back( rec );
+                            }
+                            else if ( track && sideAmb && !robustHidden[ s ] )
+                                rejected[ i ].insert( rejected[ i ].end(), { i, j, k, uint32_t( s == 0 ? 1 : -1 ) } );
+                        }
+                    }
+                }
+                candTests += ct;
+                occTests += ot;
+            } );
+            res.stats.push_back( candTests );
+            res.stats.push_back( occTests );
+            // thrust::sort with HasNeighborsComparator (|w| descending) -- made deterministic here:
+            // with-neighbour probes first, each bucket in (i, ij slot, ik slot, side) order.
+            for ( int pass = 0; pass < 2; pass++ )
+                for ( uint32_t i = 0; i < n; i++ )
+                    for ( const Record & rec : perAtom[ i ] )
+                        if ( ( pass == 0 ) == ( rec.hasNb == 1 ) )
+                            records.push_back( rec );
+            res.Iw = 0;
+            for ( const Record & rec : records )
+                res.Iw += rec.hasNb;
+            if ( track )
+                for ( uint32_t i = 0; i < n; i++ )
+                    res.ambRejected.insert( res.ambRejected.end(), rejected[ i ].begin(), rejected[ i ].end() );
+        }
+
+        // -- fillIntersections (sesdf operations.cu:206-269; ssesdf operations.cu:287-329) --------------
+        V3<T> probePosition( uint32_t i, uint32_t j, uint32_t k, int side ) const
+        {
+            const Geometry<T> ij = circleGeometry( c[ i ], R[ i ], c[ j ], R[ j ] );
+            const Geometry<T> ik = circleGeometry( c[ i ], R[ i ], c[ k ], R[ k ] );
+            const V3<T>       u  = ik.normal - dot( ik.normal, ij.normal ) * ij.normal;
+            const T           t  = dot( ik.center - ij.center, ik.normal ) / dot( u, ik.normal );
+            const V3<T>       x1 = ij.center + t * u;
+            const T           sq = len2( x1 - c[ i ] );
+            const T           cc = std::max( -sq + R[ i ] * R[ i ], T( 0 ) );
+            const V3<T>       nn = normalize( cross( ij.normal, ik.normal ) );
+            return x1 + ( T( side ) * std::sqrt( cc ) ) * nn;
+        }
+
+        std::vector<uint32_t> circleLists, circleStart; // complete: per trimmed circle intersection lists
+        std::vector<uint32_t> recCircles;                // 3 circle ids per record (trimmed or global)
+        void fill()
+        {
+            const uint32_t I = uint32_t( records.size() );
+            res.concaveIds.resize( size_t( I ) * 4 );
+            res.concavePos.resize( size_t( I ) * 4 );
+            res.concavePos64.resize( size_t( I ) * 3 );
+            res.ambConcave.resize( I );
+            res.ambHasNb.resize( I );
+            recCircles.resize( size_t( I ) * 3 );
+            const size_t total = size_t( n ) * K;
+            for ( uint32_t q = 0; q < I; q++ )
+            {
+                const Record & rec = records[ q ];
+                const int      w   = rec.side * ( 1 + rec.hasNb );
+                const V3<T>    x   = probePosition( rec.i, rec.j, rec.k, rec.side );
+                res.concavePos[ 4 * q + 0 ] = float( x.x );
+                res.concavePos[ 4 * q + 1 ] = float( x.y );
+                res.concavePos[ 4 * q + 2 ] = float( x.z );
+                res.concavePos[ 4 * q + 3 ] = float( w );
+                res.concavePos64[ 3 * q + 0 ] = double( x.x );
+                res.concavePos64[ 3 * q + 1 ] = double( x.y );
+                res.concavePos64[ 3 * q + 2 ] = double( x.z );
+                res.concaveIds[ 4 * q + 0 ] = int32_t( rec.i );
+                res.concaveIds[ 4 * q + 1 ] = int32_t( rec.j );
+                res.concaveIds[ 4 * q + 2 ] = int32_t( rec.k );
+                res.concaveIds[ 4 * q + 3 ] = 0;
+                res.ambConcave[ q ] = rec.amb;
+                res.ambHasNb[ q ]   = rec.ambHas;
+                const size_t gij = size_t( rec.i ) * K + rec.a, gik = size_t( rec.i ) * K + rec.b, gjk = size_t( rec.j ) * K + rec.jkIdx;
+                if ( mode == 0 )
+                {
+                    // ids = (trimmed ij, trimmed jk, trimmed ik) -- operations.inl:437-440
+                    recCircles[ 3 * q + 0 ] = globalToTrimmedScan[ gij ];
+                    recCircles[ 3 * q + 1 ] = globalToTrimmedScan[ gjk ];
+                    recCircles[ 3 * q + 2 ] = globalToTrimmedScan[ gik ];
+                    if ( res.status[ gjk ] != 1 )
+                        res.anomalyJk++;
+                }
+                else
+                {
+                    // ids = (global ij, global ik, global jk) -- ssesdf operation.inl:308-310
+                    recCircles[ 3 * q + 0 ] = uint32_t( gij );
+                    recCircles[ 3 * q + 1 ] = uint32_t( gik );
+                    recCircles[ 3 * q + 2 ] = uint32_t( gjk );
+                    (void)total;
+                }
+            }
+            if ( mode == 0 )
+            {
+                const uint32_t Ci = uint32_t( res.trimmedToGlobal.size() );
+                res.circleIntNb.assign( Ci, 0 );
+                for ( uint32_t q = 0; q < I; q++ )
+                    for ( int e = 0; e < 3; e++ )
+                        if ( recCircles[ 3 * q + e ] < Ci )
+                            res.circleIntNb[ recCircles[ 3 * q + e ] ]++;
+                circleStart.assign( Ci + 1, 0 );
+                for ( uint32_t t = 0; t < Ci; t++ )
+                    circleStart[ t + 1 ] = circleStart[ t ] + res.circleIntNb[ t ];
+                circleLists.assign( circleStart[ Ci ], 0 );
+                std::vector<uint32_t> fillc( Ci, 0 );
+                for ( uint32_t q = 0; q < I; q++ )
+                    for ( int e = 0; e < 3; e++ )
+                    {
+                        const uint32_t t = recCircles[ 3 * q + e ];
+                        if ( t < Ci )
+                            circleLists[ circleStart[ t ] + fillc[ t ]++ ] = q;
+                    }
+            }
+            else
+            {
+                // atomicCAS status 1 -> 2 on the three circles (ssesdf operations.cu:301-303)
+                for ( uint32_t q = 0; q < I; q++ )
+                    for ( int e = 0; e < 3; e++ )
+                    {
+                        uint8_t & s = res.status[ recCircles[ 3 * q + e ] ];
+                        if ( s == 1 )
+                            s = 2;
+                    }
+            }
+        }
+
+        // -- buildConicPatches / buildSegments (operations.inl:488-668) ----------------------------------
+        void segmentsStage()
+        {
+            if ( mode != 0 )
+                return;
+            const uint32_t Ci = uint32_t( res.trimmedToGlobal.size() );
+            res.ambSegCircle.assign( Ci, 0 );
+            // circles touched by an ambiguous candidate are ambiguous
+            if ( track )
+            {
+                for ( uint32_t q = 0; q < records.size(); q++ )
+                    if ( res.ambConcave[ q ] )
+                        for ( int e = 0; e < 3; e++ )
+                            if ( recCircles[ 3 * q + e ] < Ci )
+                                res.ambSegCircle[ recCircles[ 3 * q + e ] ] = 1;
+                markRejectedCircles();
+            }
+            for ( uint32_t t = 0; t < Ci; t++ )
+            {
+                uint32_t cnt = res.circleIntNb[ t ];
+                if ( cnt / 2 == 0 )
+                    continue;
+                if ( cnt % 2 )
+                    res.ambSegCircle[ t ] = 1;
+                if ( cnt > 15 )
+                {
+                    res.overflowCircleInt++;
+                    cnt = 15;
+                }
+                const uint32_t g  = res.trimmedToGlobal[ t ];
+                const uint32_t ci = g / K, cj = res.nbIds[ g ];
+                const Geometry<T> geo = circleGeometry( c[ ci ], R[ ci ], c[ cj ], R[ cj ] );
+                const uint32_t *  ids = circleLists.data() + circleStart[ t ];
+                const uint32_t    st  = ids[ 0 ];
+                const bool        isSecondary = ci == uint32_t( res.concaveIds[ 4 * st ] ) && cj == uint32_t( res.concaveIds[ 4 * st + 2 ] );
+                const T           dir = sgn( T( res.concavePos[ 4 * st + 3 ] ) ) * ( isSecondary ? T( -1 ) : T( 1 ) );
+                const V3<T>       startPos { T( res.concavePos64[ 3 * st ] ), T( res.concavePos64[ 3 * st + 1 ] ), T( res.concavePos64[ 3 * st + 2 ] ) };
+                const V3<T>       ref = startPos - geo.center;
+                std::vector<std::pair<T, uint32_t>> ang;
+                ang.push_back( { T( 0 ), 0u } );
+                for ( uint32_t q = 1; q < cnt; q++ )
+                {
+                    const uint32_t id = ids[ q ];
+                    const V3<T>    p { T( res.concavePos64[ 3 * id ] ), T( res.concavePos64[ 3 * id + 1 ] ), T( res.concavePos64[ 3 * id + 2 ] ) };
+                    V3<T>          v1 = ref, v2 = normalize( p - geo.center );
+                    if ( dir < T( 0 ) )
+                        std::swap( v1, v2 );
+                    ang.push_back( { angleBetweenEdges( v1, v2, geo.normal ), q } );
+                }
+                std::stable_sort( ang.begin(), ang.end(), []( auto & x, auto & y ) { return x.first < y.first; } );
+                if ( track )
+                {
+                    for ( size_t q = 0; q < ang.size(); q++ )
+                    {
+                        const double a0 = double( ang[ q ].first );
+                        const double a1 = q + 1 < ang.size() ? double( ang[ q + 1 ].first ) : 2 * M_PI;
+                        if ( a1 - a0 <= deltaTheta )
+                            res.ambSegCircle[ t ] = 1;
+                    }
+                }
+                for ( uint32_t q = 0; q < cnt / 2; q++ )
+                {
+                    uint32_t A = ids[ ang[ 2 * q ].second ], B = ids[ ang[ 2 * q + 1 ].second ];
+                    if ( dir < T( 0 ) )
+                        std::swap( A, B );
+                    res.segments.insert( res.segments.end(), { ci, cj, A, B } );
+                }
+            }
+        }
+        void markRejectedCircles()
+        {
+            // a rejected-but-ambiguous candidate (i, j, k) may exist on the GPU: its circles' segments are ambiguous
+            std::map<uint64_t, uint32_t> slotToTrimmed;
+            for ( uint32_t t = 0; t < res.trimmedToGlobal.size(); t++ )
+            {
+                const uint32_t g = res.trimmedToGlobal[ t ];
+                const uint32_t i = g / K, j = res.nbIds[ g ];
+                slotToTrimmed[ ( uint64_t( i ) << 32 ) | j ] = t;
+            }
+            auto mark = [ & ]( uint32_t a, uint32_t b ) {
+                auto it = slotToTrimmed.find( ( uint64_t( std::min( a, b ) ) << 32 ) | std::max( a, b ) );
+                if ( it != slotToTrimmed.end() )
+                    res.ambSegCircle[ it->second ] = 1;
+            };
+            for ( size_t q = 0; q + 3 < res.ambRejected.size(); q += 4 )
+            {
+                const uint32_t i = res.ambRejected[ q ], j = res.ambRejected[ q + 1 ], k = res.ambRejected[ q + 2 ];
+                mark( i, j ), mark( i, k ), mark( j, k );
+            }
+            // a circle whose status is ambiguous may gain or lose segments
+            for ( uint32_t t = 0; t < res.trimmedToGlobal.size(); t++ )
+                if ( res.ambCircle[ res.trimmedToGlobal[ t ] ] )
+                    res.ambSegCircle[ t ] = 1;
+        }
+
+        // -- handleIntersectionSingularities + findIntersectionsNeighbors (sesdf operations.cu:14-32, 271-338;
+        //    ssesdf operations.cu:57-75, 331-399) ------------------------------------------------------------
+        void probeNeighbours()
+        {
+            const uint32_t Iw = res.Iw;
+            const uint32_t M  = 32;
+            res.concaveNb.assign( size_t( Iw ) * M * 4, 0.f );
+            res.ambProbeNb.assign( Iw, 0 );
+            if ( Iw == 0 )
+                return;
+            const GridInfo & g  = res.grid;
+            SortedGrid       pg = buildGrid( g, Iw, [ & ]( uint32_t q, float & x, float & y, float & z ) {
+                x = res.concavePos[ 4 * q ], y = res.concavePos[ 4 * q + 1 ], z = res.concavePos[ 4 * q + 2 ];
+            } );
+            const float range = 2.f * rpf;
+            for ( uint32_t s = 0; s < Iw; s++ )
+            {
+                const uint32_t orig = pg.sortedIdx[ s ];
+                const float    px = res.concavePos[ 4 * orig ], py = res.concavePos[ 4 * orig + 1 ], pz = res.concavePos[ 4 * orig + 2 ];
+                const V3<T>    p64 { T( res.concavePos64[ 3 * orig ] ), T( res.concavePos64[ 3 * orig + 1 ] ), T( res.concavePos64[ 3 * orig + 2 ] ) };
+                int            gx, gy, gz;
+                g.pos( px, py, pz, gx, gy, gz );
+                const int rx = std::max( int( std::ceil( range / g.cx ) ), 1 );
+                const int ry = std::max( int( std::ceil( range / g.cy ) ), 1 );
+                const int rz = std::max( int( std::ceil( range / g.cz ) ), 1 );
+                const int sx = std::max( gx - rx, 0 ), sy = std::max( gy - ry, 0 ), sz = std::max( gz - rz, 0 );
+                const int ex = std::min( gx + rx, g.size - 1 ), ey = std::min( gy + ry, g.size - 1 ), ez = std::min( gz + rz, g.size - 1 );
+                uint32_t  count = 0;
+                bool      full  = false;
+                for ( int z = sz; z <= ez && !full; z++ )
+                    for ( int y = sy; y <= ey && !full; y++ )
+                        for ( int x = sx; x <= ex && !full; x++ )
+                        {
+                            const uint32_t h  = g.hash( x, y, z );
+                            const uint32_t st = pg.cellStart[ h ];
+                            if ( st == Empty )
+                                continue;
+                            for ( uint32_t t = st; t < pg.cellEnd[ h ]; t++ )
+                            {
+                                if ( t == s )
+                                    continue;
+                                const uint32_t o2 = pg.sortedIdx[ t ];
+                                const V3<T>    q64 { T( res.concavePos64[ 3 * o2 ] ), T( res.concavePos64[ 3 * o2 + 1 ] ), T( res.concavePos64[ 3 * o2 + 2 ] ) };
+                                const T        dist = len( p64 - q64 );
+                                if ( inBand( dist - T( range ) ) )
+                                    res.ambProbeNb[ orig ] = 1;
+                                if ( dist < T( range ) )
+                                {
+                                    float * o = &res.concaveNb[ ( size_t( orig ) * M + count ) * 4 ];
+                                    o[ 0 ] = res.concavePos[ 4 * o2 ], o[ 1 ] = res.concavePos[ 4 * o2 + 1 ], o[ 2 ] = res.concavePos[ 4 * o2 + 2 ];
+                                    o[ 3 ] = mode == 0 ? px * py * pz : 0.f;
+                                    if ( res.ambConcave[ o2 ] || res.ambHasNb[ o2 ] )
+                                        res.ambProbeNb[ orig ] = 1;
+                                    count++;
+                                    if ( count == M )
+                                    {
+                                        res.overflowProbeNb++;
+                                        full = true;
+                                        break;
+                                    }
+                                }
+                            }
+                        }
+                res.concaveIds[ 4 * orig + 3 ] = int32_t( count );
+                if ( mode == 1 )
+                    res.concavePos[ 4 * orig + 3 ] = float( orig * M );
+            }
+        }
+
+        // -- fillCircles (ssesdf operations.cu:246-285; operation.inl:73-95) --------------------------------
+        void torusStage()
+        {
+            if ( mode != 1 )
+                return;
+            std::vector<float> pos, axis, vs;
+            for ( uint32_t i = 0; i < n; i++ )
+                for ( uint32_t a = 0; a < K; a++ )
+                {
+                    const size_t g = size_t( i ) * K + a;
+                    if ( a >= res.nbCount[ i ] || res.status[ g ] != 2 )
+                        continue;
+                    const uint32_t    j   = res.nbIds[ g ];
+                    const Geometry<T> geo = circleGeometry( c[ i ], R[ i ], c[ j ], R[ j ] );
+                    const V3<T>       ta  = normalize( geo.center - c[ i ] );
+                    const V3<T>       tc  = geo.center;
+                    const V3<T>       ortho = normalize( cross( ta, V3<T> { 0, 0, 1 } ) );
+                    const V3<T>       P   = tc + ( ortho * geo.radius );
+                    const V3<T>       X   = normalize( P - c[ i ] ) * r[ i ];
+                    V3<T>             C   = ( len( P - c[ i ] ) / ( len( P - c[ j ] ) + len( P - c[ i ] ) ) ) * ( c[ j ] - c[ i ] );
+                    const T           distance = len( X - C );
+                    C                      = ( C + c[ i ] ) - tc;
+                    pos.insert( pos.end(), { float( tc.x ), float( tc.y ), float( tc.z ), float( geo.radius ) } );
+                    axis.insert( axis.end(), { float( ta.x ), float( ta.y ), float( ta.z ), rpf } );
+                    vs.insert( vs.end(), { float( C.x ), float( C.y ), float( C.z ), float( distance ) } );
+                    res.torusSlots.push_back( uint32_t( g ) );
+                }
+            res.torus = pos;
+            res.torus.insert( res.torus.end(), axis.begin(), axis.end() );
+            res.torus.insert( res.torus.end(), vs.begin(), vs.end() );
+        }
+
+        void run()
+        {
+            neighbours();
+            visibility();
+            compaction();
+            intersections();
+            fill();
+            segmentsStage();
+            probeNeighbours();
+            torusStage();
+        }
+    };
+} // namespace orc
+
+using namespace orc;
+
+extern "C"
+{
+    // Build the surface of n atoms (float4 xyzr, input order) on the CPU.
+    //   mode      0 = complete surface (bcs::Sesdf), 1 = exterior path (bcs::Ssesdf)
+    //   precision 32 or 64
+    //   track     classify predicates against the epsilon band delta (A^2 / A) and deltaTheta (rad)
+    //   threads   host threads for the per-atom stages
+    void * oracle_build( const float * xyzr, uint32_t n, float probeRadius, int mode, int precision, uint32_t maxNeighbors,
+                         int track, double delta, double deltaTheta, int threads )
+    {
+        auto * res = new Result();
+        res->mode  = mode;
+        res->n     = n;
+        res->K     = maxNeighbors;
+        res->rp    = probeRadius;
+        if ( n == 0 )
+            return res;
+        const AtomF * atoms = reinterpret_cast<const AtomF *>( xyzr );
+        if ( precision == 64 )
+        {
+            Builder<double> b( atoms, n, probeRadius, maxNeighbors, mode, track != 0, delta, deltaTheta, threads, *res );
+            b.run();
+        }
+        else
+        {
+            Builder<float> b( atoms, n, probeRadius, maxNeighbors, mode, false, delta, deltaTheta, threads, *res );
+            b.run();
+        }
+        return res;
+    }
+
+    void oracle_free( void * h ) { delete static_cast<Result *>( h ); }
+
+    // Named array access: returns the element count and element size (bytes) of array `name`;
+    // copies it into out when out is not null.
+    uint64_t oracle_get( void * h, const char * name, void * out, uint32_t * elemSize )
+    {
+        Result *          r = static_cast<Result *>( h );
+        const std::string s( name );
+        auto              give = [ & ]( const auto & v ) -> uint64_t {
+            using E = typename std::decay_t<decltype( v )>::value_type;
+            if ( elemSize )
+                *elemSize = sizeof( E );
+            if ( out && !v.empty() )
+                std::memcpy( out, v.data(), v.size() * sizeof( E ) );
+            return v.size();
+        };
+        if ( s == "atoms" ) return give( r->atoms );
+        if ( s == "sorted_to_initial" ) return give( r->sortedToInitial );
+        if ( s == "nb_count" ) return give( r->nbCount );
+        if ( s == "nb_ids" ) return give( r->nbIds );
+        if ( s == "status" ) return give( r->status );
+        if ( s == "trimmed_to_global" ) return give( r->trimmedToGlobal );
+        if ( s == "convex_patches" ) return give( r->convexPatches );
+        if ( s == "sectors" ) return give( r->sectors );
+        if ( s == "full_circles" ) return give( r->fullCircles );
+        if ( s == "concave_ids" ) return give( r->concaveIds );
+        if ( s == "concave_pos" ) return give( r->concavePos );
+        if ( s == "concave_pos64" ) return give( r->concavePos64 );
+        if ( s == "concave_nb" ) return give( r->concaveNb );
+        if ( s == "circle_int_nb" ) return give( r->circleIntNb );
+        if ( s == "segments" ) return give( r->segments );
+        if ( s == "torus" ) return give( r->torus );
+        if ( s == "torus_slots" ) return give( r->torusSlots );
+        if ( s == "amb_pairs" ) return give( r->ambPairs );
+        if ( s == "amb_circle" ) return give( r->ambCircle );
+        if ( s == "amb_concave" ) return give( r->ambConcave );
+        if ( s == "amb_hasnb" ) return give( r->ambHasNb );
+        if ( s == "amb_rejected" ) return give( r->ambRejected );
+        if ( s == "amb_seg_circle" ) return give( r->ambSegCircle );
+        if ( s == "amb_probe_nb" ) return give( r->ambProbeNb );
+        if ( s == "stats" ) return give( r->stats );
+        if ( s == "grid" )
+        {
+            std::vector<float> g { r->grid.ox, r->grid.oy, r->grid.oz, r->grid.cx, r->grid.cy, r->grid.cz, float( r->grid.size ) };
+            return give( g );
+        }
+        if ( s == "counters" )
+        {
+            std::vector<uint64_t> v { r->Iw, r->anomalyJk, r->overflowNb, r->overflowProbeNb, r->overflowCircleInt };
+            return give( v );
+        }
+        if ( elemSize )
+            *elemSize = 0;
+        return 0;
+    }
+}

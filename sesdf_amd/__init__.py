@@ -1,0 +1,7 @@
+"""MI355X-native solvent-excluded-surface builder (hand-written gfx950 HIP kernels behind a C ABI).
+
+Public API mirrors the reference host classes bcs::Sesdf / bcs::Ssesdf; see ``surface.py``.
+"""
+from .surface import Sesdf, Ssesdf, get_aabb  # noqa: F401
+
+__all__ = ["Sesdf", "Ssesdf", "get_aabb"]

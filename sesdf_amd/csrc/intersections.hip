@@ -1,0 +1,791 @@
+// Concave patches, toroidal segments, probe neighbourhoods and exterior torus records.
+//
+//   intersections_kernel  <- findIntersectionsBetweenCircles (sesdf/include/bcs/sesdf/operations.inl:230-486;
+//                            ssesdf/include/bcs/ssesdf/operation.inl:100-351) + the |w| sort of
+//                            findIntersections (operations.inl:128-141): one wave per atom i,
+//                            candidate pairs (ij, ik) flattened over the lanes, N(j) membership by
+//                            binary search in j's ascending list, occluders N(i) from LDS.
+//   circle_scan_kernel    <- the per-circle offset scans (operations.inl:149-156, 500-517)
+//   fill_kernel           <- fillIntersections (sesdf/src/sesdf/operations.cu:206-269;
+//                            ssesdf/src/ssesdf/operations.cu:287-329)
+//   segments_kernel       <- buildSegments (operations.inl:552-668)
+//   probe_neighbors_kernel<- handleIntersectionSingularities + findIntersectionsNeighbors
+//                            (sesdf operations.cu:14-32, 271-338; ssesdf operations.cu:57-75, 331-399)
+//   torus_kernel          <- fillCircles + its count/copy_if (ssesdf operations.cu:246-285,
+//                            operation.inl:73-95)
+#include "kernels.h"
+
+namespace sesdf
+{
+    constexpr int      InterWaves  = 4;
+    constexpr int      MaxAtomRecs = 256;   // concave records held in LDS per atom
+    constexpr uint32_t RecMinus    = 1u << 31;
+    constexpr uint32_t RecHasNb    = 1u << 30;
+
+    struct CircleGeo
+    {
+        f3    centre;
+        f3    normal;
+        float radius2;
+    };
+    __device__ __forceinline__ CircleGeo circle_geo( f3 dj, float Ri2, float Rj2 )
+    {
+        CircleGeo   c;
+        const float d2 = len2( dj );
+        const float t  = ( Ri2 - Rj2 + d2 ) / ( 2.f * d2 );
+        c.centre       = dj * t;
+        c.radius2      = Ri2 - len2( c.centre );
+        c.normal       = dj * ( 1.0f / sqrtf( d2 ) );
+        return c;
+    }
+
+    // Ill-conditioned triplets (near-collinear centres or discriminant near zero) are re-solved in
+    // FP64: the circle-circle intersection point xc and discriminant t of the probe trilateration.
+    __device__ __forceinline__ bool ill_conditioned( float den, float t, float Ri2 ) { return den < 1e-3f || fabsf( t ) < 1e-3f * Ri2; }
+
+    // Probe position of triplet (i, j, k) in i's local frame, mirroring the arithmetic of
+    // fillIntersections: t = (c_ik - c_ij).n_ik / (u.n_ik), x1 = c_ij + t u,
+    // x = x1 + side sqrt(max(R_i^2 - |x1|^2, 0)) n(n_ij x n_ik).
+    __device__ d3 probe_fp64( f3 dj, f3 dk, float ri, float rj, float rk, float rp, int side )
+    {
+        const double Ri = double( ri ) + double( rp ), Rj = double( rj ) + double( rp ), Rk = double( rk ) + double( rp );
+        const d3     Dj = mkd( dj ), Dk = mkd( dk );
+        const double dj2 = dot( Dj, Dj ), dk2 = dot( Dk, Dk );
+        const d3     cij = Dj * ( ( Ri * Ri - Rj * Rj + dj2 ) / ( 2.0 * dj2 ) );
+        const d3     cik = Dk * ( ( Ri * Ri - Rk * Rk + dk2 ) / ( 2.0 * dk2 ) );
+        const d3     nij = Dj * ( 1.0 / sqrt( dj2 ) ), nik = Dk * ( 1.0 / sqrt( dk2 ) );
+        const d3     u   = nik - nij * dot( nik, nij );
+        const double t   = dot( cik - cij, nik ) / dot( u, nik );
+        const d3     x1  = cij + u * t;
+        const double c   = fmax( Ri * Ri - dot( x1, x1 ), 0.0 );
+        const d3     n   = unit( cross( nij, nik ) );
+        return x1 + n * ( double( side ) * sqrt( c ) );
+    }
+
+    __device__ __forceinline__ f3 probe_fp32( f3 dj, f3 dk, float Ri2, float Rj2, float Rk2, int side, bool & ill )
+    {
+        const CircleGeo ij = circle_geo( dj, Ri2, Rj2 );
+        const CircleGeo ik = circle_geo( dk, Ri2, Rk2 );
+        const f3        u  = ik.normal - ij.normal * dot( ik.normal, ij.normal );
+        const float     den = dot( u, ik.normal );
+        const float     t   = dot( ik.centre - ij.centre, ik.normal ) / den;
+        const f3        x1  = ij.centre + u * t;
+        const float     c   = Ri2 - len2( x1 );
+        ill                 = ill_conditioned( den, c, Ri2 );
+        const f3 n          = unit( cross( ij.normal, ik.normal ) );
+        return x1 + n * ( float( side ) * sqrtf( fmaxf( c, 0.f ) ) );
+    }
+
+    __device__ __forceinline__ uint32_t trimmed_id( const Buffers & b, uint32_t atom, uint32_t slot )
+    {
+        const uint32_t * m   = b.intMask + size_t( atom ) * ( b.K / 32 );
+        uint32_t         acc = b.trimBase[ atom ];
+        for ( uint32_t w = 0; w < slot / 32; w++ )
+            acc += __popc( m[ w ] );
+        return acc + __popc( m[ slot / 32 ] & ( ( 1u << ( slot % 32 ) ) - 1u ) );
+    }
+
+    __device__ __forceinline__ int find_in_list( const uint32_t * list, uint32_t n, uint32_t key )
+    {
+        int lo = 0, hi = int( n ) - 1;
+        while ( lo <= hi )
+        {
+            const int      mid = ( lo + hi ) >> 1;
+            const uint32_t v   = list[ mid ];
+            if ( v == key )
+                return mid;
+            if ( v < key )
+                lo = mid + 1;
+            else
+                hi = mid - 1;
+        }
+        return -1;
+    }
+
+    template<int KMAX, int MODE>
+    __global__ void __launch_bounds__( InterWaves * WaveSize ) intersections_kernel( Buffers b )
+    {
+        __shared__ uint32_t sIdx[ InterWaves ][ KMAX ];
+        __shared__ float4   sAtom[ InterWaves ][ KMAX ];   // (c_l - c_i, R_l^2)
+        __shared__ float    sRad[ InterWaves ][ KMAX ];    // vdW radius r_l
+        __shared__ uint16_t sCand[ InterWaves ][ KMAX ];
+        __shared__ uint4    sRec[ InterWaves ][ MaxAtomRecs ];
+
+        const int      w    = threadIdx.x / WaveSize;
+        const int      lane = lane_id();
+        const float    rp   = b.probeRadius;
+        const uint32_t K    = b.K;
+        const uint32_t tag  = stage_tag( b.ctl, 3 );
+
+        for ( ;; )
+        {
+            const uint32_t i = wave_ticket( &b.ctl->tickets[ TicketIntersections ] );
+            if ( i >= b.n )
+                break;
+            const float4   ai  = b.atoms[ i ];
+            const f3       ci  = xyz( ai );
+            const float    Ri  = ai.w + rp;
+            const float    Ri2 = Ri * Ri;
+            const uint32_t ni  = b.nbCount[ i ];
+
+            uint32_t m = 0;
+            for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+            {
+                const uint32_t l    = lb + lane;
+                bool           cand = false;
+                if ( l < ni )
+                {
+                    const uint32_t j  = b.nbIds[ size_t( i ) * K + l ];
+                    const float4   aj = b.atoms[ j ];
+                    const float    Rj = aj.w + rp;
+                    sIdx[ w ][ l ]    = j;
+                    sAtom[ w ][ l ]   = make_float4( aj.x - ci.x, aj.y - ci.y, aj.z - ci.z, Rj * Rj );
+                    sRad[ w ][ l ]    = aj.w;
+                    cand              = j > i && b.status[ size_t( i ) * K + l ] == 1;
+                }
+                const uint64_t mask = ballot( cand );
+                if ( cand )
+                    sCand[ w ][ m + rank_below( mask ) ] = uint16_t( l );
+                m += uint32_t( __popcll( mask ) );
+            }
+            wave_sync();
+
+            // ---- candidate pairs (a < b over the intersected circles of i), flattened ------------
+            const uint32_t P    = m * ( m - 1 ) / 2;
+            uint32_t       nRec = 0;
+            for ( uint32_t pb = 0; pb < P; pb += WaveSize )
+            {
+                const uint32_t p    = pb + lane;
+                int            nOut = 0;
+                uint4          out0 = make_uint4( 0, 0, 0, 0 ), out1 = out0;
+                if ( p < P )
+                {
+                    // row-major triangular index -> (a, c) with a < c
+                    const float fm = float( 2 * m - 1 );
+                    int         a  = int( ( fm - sqrtf( fm * fm - 8.f * float( p ) ) ) * 0.5f );
+                    a              = max( 0, min( a, int( m ) - 2 ) );
+                    while ( a > 0 && uint32_t( a ) * ( 2 * m - a - 1 ) / 2 > p )
+                        a--;
+                    while ( uint32_t( a + 1 ) * ( 2 * m - a - 2 ) / 2 <= p )
+                        a++;
+                    const uint32_t cc  = p - uint32_t( a ) * ( 2 * m - a - 1 ) / 2 + a + 1;
+                    const uint32_t la  = sCand[ w ][ a ];
+                    const uint32_t lk  = sCand[ w ][ cc ];
+                    const uint32_t j   = sIdx[ w ][ la ];
+                    const uint32_t k   = sIdx[ w ][ lk ];
+                    const int      jk  = find_in_list( b.nbIds + size_t( j ) * K, b.nbCount[ j ], k );
+                    if ( jk >= 0 )
+                    {
+                        const float4    aj  = sAtom[ w ][ la ];
+                        const float4    ak  = sAtom[ w ][ lk ];
+                        const f3        dj  = xyz( aj ), dk = xyz( ak );
+                        const CircleGeo ij  = circle_geo( dj, Ri2, aj.w );
+                        const CircleGeo ik  = circle_geo( dk, Ri2, ak.w );
+                        const f3        u   = ik.normal - ij.normal * dot( ik.normal, ij.normal );
+                        const float     den = dot( u, ik.normal );
+                        const float     q   = dot( ik.centre - ij.centre, ik.normal );
+                        f3              xc  = mk( ( u.x * q ) / den, ( u.y * q ) / den, ( u.z * q ) / den ) + ij.centre;
+                        float           t   = Ri2 - len2( xc );
+                        f3              x[ 2 ];
+                        bool            ok  = false;
+                        if ( ill_conditioned( den, t, Ri2 ) )
+                        {
+                            // FP64 re-solve of xc, t and the two probe positions
+                            atomicAdd( &b.ctl->fp64ResolveNb, 1u );
+                            const double RiD = double( ai.w ) + double( rp );
+                            const double RjD = double( sRad[ w ][ la ] ) + double( rp );
+                            const double RkD = double( sRad[ w ][ lk ] ) + double( rp );
+                            const d3     Dj = mkd( dj ), Dk = mkd( dk );
+                            const double dj2 = dot( Dj, Dj ), dk2 = dot( Dk, Dk );
+                            const d3     cij = Dj * ( ( RiD * RiD - RjD * RjD + dj2 ) / ( 2.0 * dj2 ) );
+                            const d3     cik = Dk * ( ( RiD * RiD - RkD * RkD + dk2 ) / ( 2.0 * dk2 ) );
+                            const d3     nij = Dj * ( 1.0 / sqrt( dj2 ) ), nik = Dk * ( 1.0 / sqrt( dk2 ) );
+                            const d3     uu  = nik - nij * dot( nik, nij );
+                            const d3     xcD = cij + uu * ( dot( cik - cij, nik ) / dot( uu, nik ) );
+                            const double tD  = RiD * RiD - dot( xcD, xcD );
+                            if ( tD >= 0.0 )
+                            {
+                                const d3 nn = unit( cross( nik, nij ) ) * sqrt( tD );
+                                const d3 x1 = xcD - nn, x2 = xcD + nn;
+                                x[ 0 ]      = mk( float( x1.x ), float( x1.y ), float( x1.z ) );
+                                x[ 1 ]      = mk( float( x2.x ), float( x2.y ), float( x2.z ) );
+                                ok          = true;
+                            }
+                        }
+                        else if ( t >= 0.f )
+                        {
+                            const f3 nn = unit( cross( ik.normal, ij.normal ) ) * sqrtf( t );
+                            x[ 0 ]      = xc - nn;
+                            x[ 1 ]      = xc + nn;
+                            ok          = true;
+                        }
+                        if ( ok )
+                        {
+                            bool vis0 = true, vis1 = true;
+                            for ( uint32_t l = 0; l < ni && ( vis0 || vis1 ); l++ )
+                            {
+                                if ( l == la || l == lk )
+                                    continue;
+                                const float4 al = sAtom[ w ][ l ];
+                                const f3     cl = xyz( al );
+                                vis0            = vis0 && len2( cl - x[ 0 ] ) - al.w > 0.f;
+                                vis1            = vis1 && len2( cl - x[ 1 ] ) - al.w > 0.f;
+                            }
+                            const f3       pn     = unit( cross( ij.normal, unit( dj - dk ) ) );
+                            const uint32_t packed = la | ( lk << 10 ) | ( uint32_t( jk ) << 20 );
+                            for ( int s = 0; s < 2; s++ )
+                            {
+                                if ( !( s == 0 ? vis0 : vis1 ) )
+                                    continue;
+                                const float    d     = dot( -x[ s ], pn );
+                                const bool     hasNb = d * d - rp * rp < -PredicateEps;
+                                const uint4    rec   = make_uint4( i, j, k, packed | ( s ? RecMinus : 0u ) | ( hasNb ? RecHasNb : 0u ) );
+                                if ( nOut == 0 )
+                                    out0 = rec;
+                                else
+                                    out1 = rec;
+                                nOut++;
+                            }
+                        }
+                    }
+                }
+                const uint64_t has1 = ballot( nOut >= 1 ), has2 = ballot( nOut == 2 );
+                const uint32_t off  = nRec + rank_below( has1 ) + rank_below( has2 );
+                if ( nOut >= 1 && off < MaxAtomRecs )
+                    sRec[ w ][ off ] = out0;
+                if ( nOut == 2 && off + 1 < MaxAtomRecs )
+                    sRec[ w ][ off + 1 ] = out1;
+                nRec += uint32_t( __popcll( has1 ) + __popcll( has2 ) );
+            }
+            if ( nRec > MaxAtomRecs )
+            {
+                if ( lane == 0 )
+                    set_overflow( b.ctl, OverflowAtomRecords );
+                nRec = MaxAtomRecs;
+            }
+            if ( lane == 0 && nRec )
+                atomicMax( &b.ctl->maxAtomRecords, nRec );
+            wave_sync();
+
+            // ---- bucket counts, ordered offsets, emission ----------------------------------------
+            uint32_t nW = 0;
+            for ( uint32_t rb = 0; rb < nRec; rb += WaveSize )
+            {
+                const uint32_t r = rb + lane;
+                nW += uint32_t( __popcll( ballot( r < nRec && ( sRec[ w ][ r ].w & RecHasNb ) ) ) );
+            }
+            const uint32_t local[ 2 ] = { nW, nRec - nW };
+            uint32_t       excl[ 2 ];
+            wave_lookback<2>( b.flags, b.flagStride, i, tag, local, excl );
+            if ( lane == 0 )
+            {
+                b.wStart[ i ] = excl[ 0 ];
+                b.nStart[ i ] = excl[ 1 ];
+                if ( i == b.n - 1 )
+                {
+                    b.wStart[ b.n ]              = excl[ 0 ] + nW;
+                    b.nStart[ b.n ]              = excl[ 1 ] + nRec - nW;
+                    b.ctl->intersectionWithNbNb = excl[ 0 ] + nW;
+                    b.ctl->intersectionNoNbNb   = excl[ 1 ] + nRec - nW;
+                }
+            }
+            uint32_t runW = excl[ 0 ], runN = excl[ 1 ];
+            for ( uint32_t rb = 0; rb < nRec; rb += WaveSize )
+            {
+                const uint32_t r    = rb + lane;
+                const bool     live = r < nRec;
+                const uint4    rec  = live ? sRec[ w ][ r ] : make_uint4( 0, 0, 0, 0 );
+                const bool     isW  = live && ( rec.w & RecHasNb );
+                const uint64_t mW = ballot( isW ), mN = ballot( live && !isW );
+                if ( live )
+                {
+                    const uint32_t idx = isW ? runW + rank_below( mW ) : runN + rank_below( mN );
+                    if ( idx < b.capRecords )
+                        b.records[ isW ? idx : b.capRecords - 1 - idx ] = rec;
+                    else
+                        set_overflow( b.ctl, OverflowIntersections );
+                    if ( MODE == 0 )
+                    {
+                        // per-circle intersection counts on (ij, jk, ik)
+                        const uint32_t la = rec.w & 1023u, lk = ( rec.w >> 10 ) & 1023u, jk = ( rec.w >> 20 ) & 1023u;
+                        const uint32_t ci = min( b.ctl->intersectedCircleNb, b.capTrimmed );
+                        const uint32_t t0 = trimmed_id( b, i, la ), t1 = trimmed_id( b, rec.y, jk ), t2 = trimmed_id( b, i, lk );
+                        if ( !( b.intMask[ size_t( rec.y ) * ( K / 32 ) + jk / 32 ] & ( 1u << ( jk % 32 ) ) ) )
+                            atomicAdd( &b.ctl->jkAnomalyNb, 1u );
+                        if ( t0 < ci )
+                            atomicAdd( b.circleIntNb + t0, 1u );
+                        if ( t1 < ci )
+                            atomicAdd( b.circleIntNb + t1, 1u );
+                        if ( t2 < ci )
+                            atomicAdd( b.circleIntNb + t2, 1u );
+                    }
+                }
+                runW += uint32_t( __popcll( mW ) );
+                runN += uint32_t( __popcll( mN ) );
+            }
+        }
+    }
+
+    // Exclusive scans of the per-circle intersection counts (list offsets) and of count/2
+    // (segment offsets) over the Ci intersected circles; clears the list cursors.
+    constexpr int CScanThreads = 256;
+    constexpr int CScanItems   = 4;
+    constexpr int CScanTile    = CScanThreads * CScanItems;
+
+    __global__ void __launch_bounds__( CScanThreads ) circle_scan_kernel( Buffers b )
+    {
+        __shared__ uint32_t waveSums[ 2 ][ CScanThreads / WaveSize ];
+        __shared__ uint32_t tileBase[ 2 ];
+        __shared__ uint32_t ticket;
+        const uint32_t      Ci    = min( b.ctl->intersectedCircleNb, b.capTrimmed );
+        const uint32_t      tiles = ( Ci + CScanTile - 1 ) / CScanTile;
+        const uint32_t      tag   = stage_tag( b.ctl, 4 );
+        if ( Ci == 0 )
+        {
+            if ( blockIdx.x == 0 && threadIdx.x == 0 )
+            {
+                b.circleStart[ 0 ] = 0;
+                b.segStart[ 0 ]    = 0;
+            }
+            return;
+        }
+        for ( ;; )
+        {
+            if ( threadIdx.x == 0 )
+                ticket = atomicAdd( &b.ctl->tickets[ TicketCircleScan ], 1u );
+            __syncthreads();
+            const uint32_t t = ticket;
+            if ( t >= tiles )
+                break;
+            const uint32_t base = t * CScanTile + threadIdx.x * CScanItems;
+            uint32_t       v[ CScanItems ];
+            uint32_t       s0 = 0, s1 = 0;
+#pragma unroll
+            for ( int q = 0; q < CScanItems; q++ )
+            {
+                v[ q ] = base + q < Ci ? b.circleIntNb[ base + q ] : 0u;
+                s0 += v[ q ];
+                s1 += v[ q ] / 2;
+            }
+            const uint32_t i0 = wave_incl_scan( s0 ), i1 = wave_incl_scan( s1 );
+            const int      w  = threadIdx.x / WaveSize;
+            if ( lane_id() == WaveSize - 1 )
+            {
+                waveSums[ 0 ][ w ] = i0;
+                waveSums[ 1 ][ w ] = i1;
+            }
+            __syncthreads();
+            if ( w == 0 )
+            {
+                uint32_t t0 = 0, t1 = 0;
+                for ( int q = 0; q < CScanThreads / WaveSize; q++ )
+                    t0 += waveSums[ 0 ][ q ], t1 += waveSums[ 1 ][ q ];
+                const uint32_t local[ 2 ] = { t0, t1 };
+                uint32_t       excl[ 2 ];
+                wave_lookback<2>( b.flags, b.flagStride, t, tag, local, excl );
+                if ( lane_id() == 0 )
+                {
+                    tileBase[ 0 ] = excl[ 0 ];
+                    tileBase[ 1 ] = excl[ 1 ];
+                    if ( t == tiles - 1 )
+                    {
+                        b.circleStart[ Ci ]   = excl[ 0 ] + t0;
+                        b.segStart[ Ci ]      = excl[ 1 ] + t1;
+                        b.ctl->circleListNb   = excl[ 0 ] + t0;
+                        b.ctl->segmentNb      = excl[ 1 ] + t1;
+                    }
+                }
+            }
+            __syncthreads();
+            uint32_t r0 = tileBase[ 0 ] + i0 - s0, r1 = tileBase[ 1 ] + i1 - s1;
+            for ( int q = 0; q < w; q++ )
+                r0 += waveSums[ 0 ][ q ], r1 += waveSums[ 1 ][ q ];
+#pragma unroll
+            for ( int q = 0; q < CScanItems; q++ )
+                if ( base + q < Ci )
+                {
+                    b.circleStart[ base + q ]  = r0;
+                    b.segStart[ base + q ]     = r1;
+                    b.circleCursor[ base + q ] = 0u;
+                    r0 += v[ q ];
+                    r1 += v[ q ] / 2;
+                }
+            __syncthreads();
+        }
+    }
+
+    // 8-bit compare-and-swap through the enclosing aligned 32-bit word.
+    __device__ __forceinline__ void promote_status( uint8_t * status, size_t slot )
+    {
+        uint32_t *     word  = reinterpret_cast<uint32_t *>( status + ( slot & ~size_t( 3 ) ) );
+        const uint32_t shift = uint32_t( slot & 3 ) * 8;
+        uint32_t       old   = __hip_atomic_load( word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+        for ( ;; )
+        {
+            if ( ( ( old >> shift ) & 0xffu ) != 1u )
+                return;
+            const uint32_t nv   = ( old & ~( 0xffu << shift ) ) | ( 2u << shift );
+            const uint32_t prev = atomicCAS( word, old, nv );
+            if ( prev == old )
+                return;
+            old = prev;
+        }
+    }
+
+    template<int MODE>
+    __global__ void __launch_bounds__( 256 ) fill_kernel( Buffers b )
+    {
+        const uint32_t Iw = b.ctl->intersectionWithNbNb, In = b.ctl->intersectionNoNbNb;
+        if ( Iw + In > b.capRecords || ( b.ctl->overflow & ( OverflowTrimmed | OverflowIntersections ) ) )
+            return; // overflow: host grows and rebuilds
+        const uint32_t I  = Iw + In;
+        const float    rp = b.probeRadius;
+        const uint32_t K  = b.K;
+        for ( uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < I; q += gridDim.x * blockDim.x )
+        {
+            const uint4    rec  = q < Iw ? b.records[ q ] : b.records[ b.capRecords - 1 - ( q - Iw ) ];
+            const uint32_t i = rec.x, j = rec.y, k = rec.z;
+            const int      side = ( rec.w & RecMinus ) ? -1 : 1;
+            const int      hasNb = ( rec.w & RecHasNb ) ? 1 : 0;
+            const uint32_t la = rec.w & 1023u, lk = ( rec.w >> 10 ) & 1023u, jk = ( rec.w >> 20 ) & 1023u;
+            const float4   ai = b.atoms[ i ], aj = b.atoms[ j ], ak = b.atoms[ k ];
+            const f3       ci = xyz( ai );
+            const f3       dj = xyz( aj ) - ci, dk = xyz( ak ) - ci;
+            const float    Ri = ai.w + rp, Rj = aj.w + rp, Rk = ak.w + rp;
+            bool           ill;
+            f3             x = probe_fp32( dj, dk, Ri * Ri, Rj * Rj, Rk * Rk, side, ill );
+            if ( ill )
+            {
+                const d3 xd = probe_fp64( dj, dk, ai.w, aj.w, ak.w, rp, side );
+                x           = mk( float( xd.x ), float( xd.y ), float( xd.z ) );
+            }
+            b.concavePos[ q ] = make_float4( ci.x + x.x, ci.y + x.y, ci.z + x.z, float( side * ( 1 + hasNb ) ) );
+            b.concaveIds[ q ] = make_int4( int( i ), int( j ), int( k ), 0 );
+            if ( MODE == 0 )
+            {
+                const uint32_t Ci     = min( b.ctl->intersectedCircleNb, b.capTrimmed );
+                const uint32_t ids[ 3 ] = { trimmed_id( b, i, la ), trimmed_id( b, j, jk ), trimmed_id( b, i, lk ) };
+                for ( int e = 0; e < 3; e++ )
+                {
+                    if ( ids[ e ] >= Ci )
+                        continue;
+                    const uint32_t slot = b.circleStart[ ids[ e ] ] + atomicAdd( b.circleCursor + ids[ e ], 1u );
+                    if ( slot < b.capLists )
+                        b.circleLists[ slot ] = q;
+                    else
+                        set_overflow( b.ctl, OverflowCircleLists );
+                }
+            }
+            else
+            {
+                promote_status( b.status, size_t( i ) * K + la );
+                promote_status( b.status, size_t( i ) * K + lk );
+                promote_status( b.status, size_t( j ) * K + jk );
+            }
+        }
+    }
+
+    // angleBetweenEdges (sesdf/include/bcs/cuda/circles.cuh:27-31)
+    __device__ __forceinline__ float angle_between_edges( f3 u, f3 v, f3 n )
+    {
+        const float theta = acosf( fminf( fmaxf( dot( u, v ) / ( sqrtf( len2( u ) ) * sqrtf( len2( v ) ) ), -1.f ), 1.f ) );
+        const float det   = dot( u, cross( v, n ) );
+        return copysignf( 1.f, det ) > 0.f ? theta : TwoPi - theta;
+    }
+
+    constexpr int SegThreads = 128;
+    constexpr int SegMax     = 16; // the reference's MaxIntersectionsPerCircles (sesdf.cu:17)
+
+    __global__ void __launch_bounds__( SegThreads ) segments_kernel( Buffers b )
+    {
+        __shared__ uint32_t sList[ SegMax ][ SegThreads ];
+        __shared__ float    sAng[ SegMax ][ SegThreads ];
+        __shared__ uint8_t  sOrd[ SegMax ][ SegThreads ];
+        const int      tx = threadIdx.x;
+        const uint32_t Ci = min( b.ctl->intersectedCircleNb, b.capTrimmed );
+        const float    rp = b.probeRadius;
+        // any overflow leaves lists incomplete: skip, the host grows the buffers and rebuilds
+        if ( b.ctl->overflow || b.ctl->intersectionWithNbNb + b.ctl->intersectionNoNbNb > b.capRecords ||
+             b.ctl->circleListNb > b.capLists )
+            return;
+        for ( uint32_t c = blockIdx.x * blockDim.x + tx; c < Ci; c += gridDim.x * blockDim.x )
+        {
+            uint32_t cnt = b.circleIntNb[ c ];
+            if ( cnt / 2 == 0 )
+                continue;
+            if ( cnt > SegMax - 1 )
+            {
+                atomicAdd( &b.ctl->circleIntOverflowNb, 1u );
+                cnt = SegMax - 1;
+            }
+            const uint32_t g  = b.trimmedToGlobal[ c ];
+            const uint32_t i  = g / b.K;
+            const uint32_t j  = b.nbIds[ g ];
+            const uint32_t st = b.circleStart[ c ];
+            // deterministic start: the lowest record index
+            for ( uint32_t q = 0; q < cnt; q++ )
+            {
+                const uint32_t v = b.circleLists[ st + q ];
+                int            p = int( q ) - 1;
+                while ( p >= 0 && sList[ p ][ tx ] > v )
+                {
+                    sList[ p + 1 ][ tx ] = sList[ p ][ tx ];
+                    p--;
+                }
+                sList[ p + 1 ][ tx ] = v;
+            }
+            const float4    ai   = b.atoms[ i ], aj = b.atoms[ j ];
+            const f3        ci   = xyz( ai );
+            const float     Ri   = ai.w + rp, Rj = aj.w + rp;
+            const CircleGeo geo  = circle_geo( xyz( aj ) - ci, Ri * Ri, Rj * Rj );
+            const uint32_t  s0   = sList[ 0 ][ tx ];
+            const int4      sIds = b.concaveIds[ s0 ];
+            const float4    sPos = b.concavePos[ s0 ];
+            const bool      secondary = i == uint32_t( sIds.x ) && j == uint32_t( sIds.z );
+            const float     dir       = copysignf( 1.f, sPos.w ) * ( secondary ? -1.f : 1.f );
+            const f3        ref       = ( xyz( sPos ) - ci ) - geo.centre;
+            sAng[ 0 ][ tx ]           = 0.f;
+            sOrd[ 0 ][ tx ]           = 0;
+            for ( uint32_t q = 1; q < cnt; q++ )
+            {
+                const float4 p  = b.concavePos[ sList[ q ][ tx ] ];
+                f3           v1 = ref, v2 = unit( ( xyz( p ) - ci ) - geo.centre );
+                if ( dir < 0.f )
+                {
+                    const f3 tmp = v1;
+                    v1           = v2;
+                    v2           = tmp;
+                }
+                const float a = angle_between_edges( v1, v2, geo.normal );
+                // stable insertion by angle
+                int pidx = int( q ) - 1;
+                while ( pidx >= 0 && sAng[ pidx ][ tx ] > a )
+                {
+                    sAng[ pidx + 1 ][ tx ] = sAng[ pidx ][ tx ];
+                    sOrd[ pidx + 1 ][ tx ] = sOrd[ pidx ][ tx ];
+                    pidx--;
+                }
+                sAng[ pidx + 1 ][ tx ] = a;
+                sOrd[ pidx + 1 ][ tx ] = uint8_t( q );
+            }
+            const uint32_t gs = b.segStart[ c ];
+            for ( uint32_t q = 0; q < cnt / 2; q++ )
+            {
+                uint32_t A = sList[ sOrd[ 2 * q ][ tx ] ][ tx ], B = sList[ sOrd[ 2 * q + 1 ][ tx ] ][ tx ];
+                if ( dir < 0.f )
+                {
+                    const uint32_t tmp = A;
+                    A                  = B;
+                    B                  = tmp;
+                }
+                if ( gs + q < b.capSegments )
+                    b.segments[ gs + q ] = make_uint4( i, j, A, B );
+                else
+                    set_overflow( b.ctl, OverflowSegments );
+            }
+        }
+    }
+
+    template<int MODE>
+    __global__ void __launch_bounds__( 256 ) probe_neighbors_kernel( Buffers b )
+    {
+        const uint32_t Iw = b.ctl->intersectionWithNbNb;
+        if ( Iw + b.ctl->intersectionNoNbNb > b.capRecords || Iw > b.capProbeNb || ( b.ctl->overflow & OverflowIntersections ) )
+            return;
+        const float    rp    = b.probeRadius;
+        const float    rmax  = __uint_as_float( b.ctl->radiusMaxBits );
+        const float    range = 2.f * rp;
+        const uint32_t maxNb = b.maxProbeNb;
+        for ( uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Iw; q += gridDim.x * blockDim.x )
+        {
+            const uint32_t owner = b.records[ q ].x;
+            const float4   pq    = b.concavePos[ q ];
+            const Molecule M     = b.mols[ find_molecule( b.mols, b.molNb, owner ) ];
+            // a probe within 2 rp of pq lies on the SAS sphere of its owner atom, so that atom is
+            // within 2 rp + (r_max + rp) of pq
+            const float search = range + rmax + rp + 1e-3f;
+            const int   reach  = max( int( ceilf( search / M.cell ) ), 1 );
+            int         gx, gy, gz;
+            cell_of( M, pq.x, pq.y, pq.z, gx, gy, gz );
+            const int x0 = max( gx - reach, 0 ), x1 = min( gx + reach, M.dx - 1 );
+            const int y0 = max( gy - reach, 0 ), y1 = min( gy + reach, M.dy - 1 );
+            const int z0 = max( gz - reach, 0 ), z1 = min( gz + reach, M.dz - 1 );
+            const float wv = MODE == 0 ? pq.x * pq.y * pq.z : 0.f;
+            uint32_t    count = 0;
+            for ( int zz = z0; zz <= z1 && count < maxNb; zz++ )
+                for ( int yy = y0; yy <= y1 && count < maxNb; yy++ )
+                {
+                    const uint32_t h0 = cell_index( M, x0, yy, zz );
+                    const uint32_t a0 = b.cellStart[ h0 ], a1 = b.cellStart[ h0 + uint32_t( x1 - x0 ) + 1 ];
+                    if ( a0 == a1 )
+                        continue;
+                    const uint32_t r0 = b.wStart[ a0 ], r1 = b.wStart[ a1 ];
+                    for ( uint32_t r = r0; r < r1 && count < maxNb; r++ )
+                    {
+                        if ( r == q )
+                            continue;
+                        const float4 pr = b.concavePos[ r ];
+                        const f3     d  = xyz( pq ) - xyz( pr );
+                        if ( sqrtf( len2( d ) ) < range )
+                        {
+                            b.concaveNb[ size_t( q ) * maxNb + count ] = make_float4( pr.x, pr.y, pr.z, wv );
+                            count++;
+                        }
+                    }
+                }
+            if ( count >= maxNb )
+                atomicAdd( &b.ctl->probeNbOverflowNb, 1u );
+            b.concaveIds[ q ].w = int( count );
+            if ( MODE == 1 )
+                b.concavePos[ q ].w = float( q * maxNb );
+        }
+    }
+
+    template<int KMAX>
+    __global__ void __launch_bounds__( 256 ) torus_kernel( Buffers b )
+    {
+        const int      lane = lane_id();
+        const uint32_t K    = b.K;
+        const float    rp   = b.probeRadius;
+        const uint32_t tag  = stage_tag( b.ctl, 5 );
+        for ( ;; )
+        {
+            const uint32_t i = wave_ticket( &b.ctl->tickets[ TicketTorus ] );
+            if ( i >= b.n )
+                break;
+            const uint32_t ni  = b.nbCount[ i ];
+            uint32_t       cnt = 0;
+            for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+            {
+                const uint32_t l = lb + lane;
+                cnt += uint32_t( __popcll( ballot( l < ni && b.status[ size_t( i ) * K + l ] == 2 ) ) );
+            }
+            const uint32_t local[ 1 ] = { cnt };
+            uint32_t       excl[ 1 ];
+            wave_lookback<1>( b.flags, b.flagStride, i, tag, local, excl );
+            if ( lane == 0 )
+            {
+                b.torusBase[ i ] = excl[ 0 ];
+                if ( i == b.n - 1 )
+                {
+                    b.ctl->torusNb       = excl[ 0 ] + cnt;
+                    b.torusBase[ b.n ]   = excl[ 0 ] + cnt;
+                }
+            }
+            if ( cnt == 0 )
+                continue;
+            const float4 ai  = b.atoms[ i ];
+            const f3     ci  = xyz( ai );
+            const float  Ri  = ai.w + rp;
+            uint32_t     run = excl[ 0 ];
+            for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+            {
+                const uint32_t l    = lb + lane;
+                const bool     is2  = l < ni && b.status[ size_t( i ) * K + l ] == 2;
+                const uint64_t mask = ballot( is2 );
+                if ( is2 )
+                {
+                    const uint32_t dst = run + rank_below( mask );
+                    if ( dst < b.capTorus )
+                    {
+                        const uint32_t  j     = b.nbIds[ size_t( i ) * K + l ];
+                        const float4    aj    = b.atoms[ j ];
+                        const f3        dj    = xyz( aj ) - ci;
+                        const float     Rj    = aj.w + rp;
+                        const CircleGeo geo   = circle_geo( dj, Ri * Ri, Rj * Rj );
+                        const float     rad   = sqrtf( geo.radius2 );
+                        const f3        ta    = unit( geo.centre );
+                        const f3        ortho = unit( cross( ta, mk( 0.f, 0.f, 1.f ) ) );
+                        const f3        P     = geo.centre + ortho * rad;
+                        const f3        X     = unit( P ) * ai.w;
+                        const float     lpi   = sqrtf( len2( P ) );
+                        const f3        C     = dj * ( lpi / ( sqrtf( len2( P - dj ) ) + lpi ) );
+                        const float     dist  = sqrtf( len2( X - C ) );
+                        const f3        Cout  = C - geo.centre;
+                        b.torusPos[ dst ]     = make_float4( ci.x + geo.centre.x, ci.y + geo.centre.y, ci.z + geo.centre.z, rad );
+                        b.torusAxis[ dst ]    = make_float4( ta.x, ta.y, ta.z, rp );
+                        b.torusVs[ dst ]      = make_float4( Cout.x, Cout.y, Cout.z, dist );
+                    }
+                    else
+                        set_overflow( b.ctl, OverflowTorus );
+                }
+                run += uint32_t( __popcll( mask ) );
+            }
+        }
+    }
+
+    static uint32_t persistent_blocks( uint64_t work, uint32_t threads, uint32_t perCu )
+    {
+        const uint64_t wanted = ( work + threads - 1 ) / threads;
+        const uint64_t cap    = uint64_t( device_cu_count() ) * perCu;
+        const uint64_t g      = wanted < cap ? wanted : cap;
+        return uint32_t( g > 0 ? g : 1 );
+    }
+
+    void launch_intersections( const Buffers & b, hipStream_t s )
+    {
+        if ( b.n == 0 )
+            return;
+        const uint32_t blocks = persistent_blocks( uint64_t( b.n ) * WaveSize, InterWaves * WaveSize, 16 );
+        if ( b.K <= 128 )
+        {
+            if ( b.mode == 0 )
+                intersections_kernel<128, 0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+            else
+                intersections_kernel<128, 1><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+        }
+        else
+        {
+            if ( b.mode == 0 )
+                intersections_kernel<320, 0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+            else
+                intersections_kernel<320, 1><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+        }
+    }
+
+    void launch_circle_scan( const Buffers & b, hipStream_t s )
+    {
+        if ( b.n == 0 || b.mode != 0 )
+            return;
+        const uint32_t blocks = persistent_blocks( b.capTrimmed, CScanTile, 4 );
+        circle_scan_kernel<<<blocks, CScanThreads, 0, s>>>( b );
+    }
+
+    void launch_fill( const Buffers & b, hipStream_t s )
+    {
+        if ( b.n == 0 )
+            return;
+        const uint32_t blocks = persistent_blocks( b.capRecords, 256, 8 );
+        if ( b.mode == 0 )
+            fill_kernel<0><<<blocks, 256, 0, s>>>( b );
+        else
+            fill_kernel<1><<<blocks, 256, 0, s>>>( b );
+    }
+
+    void launch_segments( const Buffers & b, hipStream_t s )
+    {
+        if ( b.n == 0 || b.mode != 0 )
+            return;
+        const uint32_t blocks = persistent_blocks( b.capTrimmed, SegThreads, 8 );
+        segments_kernel<<<blocks, SegThreads, 0, s>>>( b );
+    }
+
+    void launch_probe_neighbors( const Buffers & b, hipStream_t s )
+    {
+        if ( b.n == 0 )
+            return;
+        const uint32_t blocks = persistent_blocks( b.capProbeNb, 256, 8 );
+        if ( b.mode == 0 )
+            probe_neighbors_kernel<0><<<blocks, 256, 0, s>>>( b );
+        else
+            probe_neighbors_kernel<1><<<blocks, 256, 0, s>>>( b );
+    }
+
+    void launch_torus( const Buffers & b, hipStream_t s )
+    {
+        if ( b.n == 0 || b.mode != 1 )
+            return;
+        const uint32_t blocks = persistent_blocks( uint64_t( b.n ) * WaveSize, 256, 16 );
+        torus_kernel<128><<<blocks, 256, 0, s>>>( b );
+    }
+} // namespace sesdf
