@@ -38,10 +38,11 @@ def lib():
         L.oracle_free.argtypes = [ctypes.c_void_p]
         L.oracle_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
         L.oracle_get.restype = ctypes.c_uint64
-        if hasattr(L, "oracle_sdf"):
-            L.oracle_sdf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
-                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
-            L.oracle_sdf.restype = ctypes.c_int
+        L.oracle_sdf.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.oracle_sdf.restype = ctypes.c_int
+        L.oracle_components.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_components.restype = ctypes.c_uint32
         _LIB = L
     return _LIB
 
@@ -67,6 +68,22 @@ class OracleResult:
                 arr = arr.reshape(-1, SHAPES[name])
             self._cache[name] = arr
         return self._cache[name]
+
+    def sdf(self, points, band=4.0, exterior=False, threads=1, codes=False):
+        """FP64 SDF at points [n, 3] (complete-surface builds only); optionally the winner codes."""
+        p = np.ascontiguousarray(np.asarray(points, np.float64).reshape(-1, 3))
+        out = np.zeros(p.shape[0], np.float64)
+        cd = np.zeros(p.shape[0], np.int64) if codes else None
+        rc = lib().oracle_sdf(self._h, ctypes.c_void_p(p.ctypes.data), p.shape[0], 1 if exterior else 0, band,
+                              ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(cd.ctypes.data) if codes else None, threads)
+        if rc:
+            raise ValueError("oracle SDF needs a complete-surface build")
+        return (out, cd) if codes else out
+
+    def components(self):
+        comp = np.zeros(self.n, np.uint32)
+        ext = lib().oracle_components(self._h, ctypes.c_void_p(comp.ctypes.data))
+        return comp, int(ext)
 
     @property
     def iw(self):

@@ -969,12 +969,279 @@ namespace orc
             torusStage();
         }
     };
+    // ------------------------------------------------------------------------------------------------
+    // SDF oracle (FP64). The reference ships no SDF evaluator; this library defines
+    //   SDF(q) = clamp(rp + f(q), -B, B),  f = signed distance to the boundary of the SAS union A
+    // (negative inside), evaluated over the trimmed boundary elements of the complete surface:
+    //   faces    : sphere a, when the radial projection of q onto it is not inside another sphere;
+    //   arcs     : full circles and segments (A -> B counter-clockwise about n(i -> j));
+    //   vertices : concave patch (probe) positions.
+    // Outside A, f = min_a |q - c_a| - R_a. The zero level set is the SES; the field is 1-Lipschitz.
+    // Exterior variant: only elements of the surface component that contains the atom with the
+    // largest x + r count, and points outside A whose nearest sphere belongs to another component
+    // (probe-accessible cavities) are inside.
+    // Winner codes: type << 40 | index; types 0 outside(atom), 1 face(atom), 2 full circle,
+    // 3 segment, 4 vertex, 5 clamped.
+    // ------------------------------------------------------------------------------------------------
+    struct SdfOracle
+    {
+        const Result &        r;
+        double                rp;
+        std::vector<V3<double>> c;
+        std::vector<double>   R;
+        std::vector<uint32_t> fullStart, segStart, vtxStart, vtxList;
+        std::vector<uint32_t> comp;
+        uint32_t              extComp = 0;
+        // uniform grid over atoms
+        double                cell = 1, ox = 0, oy = 0, oz = 0;
+        int                   dx = 1, dy = 1, dz = 1;
+        std::vector<uint32_t> cellStart, cellAtoms;
+        double                rmax = 0;
+
+        explicit SdfOracle( const Result & res ) : r( res ), rp( double( res.rp ) )
+        {
+            const uint32_t n = r.n;
+            c.resize( n ), R.resize( n );
+            for ( uint32_t a = 0; a < n; a++ )
+            {
+                c[ a ] = { double( r.atoms[ 4 * a ] ), double( r.atoms[ 4 * a + 1 ] ), double( r.atoms[ 4 * a + 2 ] ) };
+                R[ a ] = double( r.atoms[ 4 * a + 3 ] ) + rp;
+                rmax   = std::max( rmax, double( r.atoms[ 4 * a + 3 ] ) );
+            }
+            // element ranges per owner atom (lowest sorted index)
+            auto ranges = [ & ]( std::vector<uint32_t> & start, const std::vector<uint32_t> & owner ) {
+                start.assign( n + 1, 0 );
+                for ( uint32_t o : owner )
+                    start[ o + 1 ]++;
+                for ( uint32_t a = 0; a < n; a++ )
+                    start[ a + 1 ] += start[ a ];
+            };
+            std::vector<uint32_t> fo, so;
+            for ( size_t q = 0; q + 1 < r.fullCircles.size(); q += 2 )
+                fo.push_back( r.fullCircles[ q ] );
+            for ( size_t q = 0; q + 3 < r.segments.size(); q += 4 )
+                so.push_back( r.segments[ q ] );
+            ranges( fullStart, fo );
+            ranges( segStart, so );
+            // vertices by owner i, in record order
+            const size_t I = r.concaveIds.size() / 4;
+            std::vector<uint32_t> vo( I );
+            for ( size_t q = 0; q < I; q++ )
+                vo[ q ] = uint32_t( r.concaveIds[ 4 * q ] );
+            ranges( vtxStart, vo );
+            vtxList.resize( I );
+            std::vector<uint32_t> fillv( vtxStart.begin(), vtxStart.end() - 1 );
+            for ( size_t q = 0; q < I; q++ )
+                vtxList[ fillv[ vo[ q ] ]++ ] = uint32_t( q );
+            // surface components: union-find over arcs
+            comp.resize( n );
+            for ( uint32_t a = 0; a < n; a++ )
+                comp[ a ] = a;
+            auto find = [ & ]( uint32_t a ) {
+                while ( comp[ a ] != a )
+                    a = comp[ a ] = comp[ comp[ a ] ];
+                return a;
+            };
+            auto unite = [ & ]( uint32_t a, uint32_t b ) {
+                a = find( a ), b = find( b );
+                if ( a != b )
+                    comp[ std::max( a, b ) ] = std::min( a, b );
+            };
+            for ( size_t q = 0; q + 1 < r.fullCircles.size(); q += 2 )
+                unite( r.fullCircles[ q ], r.fullCircles[ q + 1 ] );
+            for ( size_t q = 0; q + 3 < r.segments.size(); q += 4 )
+                unite( r.segments[ q ], r.segments[ q + 1 ] );
+            for ( uint32_t a = 0; a < n; a++ )
+                comp[ a ] = find( a );
+            uint32_t best = 0;
+            for ( uint32_t a = 1; a < n; a++ )
+                if ( double( r.atoms[ 4 * a ] ) + double( r.atoms[ 4 * a + 3 ] ) >
+                     double( r.atoms[ 4 * best ] ) + double( r.atoms[ 4 * best + 3 ] ) )
+                    best = a;
+            extComp = n ? comp[ best ] : 0;
+        }
+
+        void buildGrid( double band )
+        {
+            const uint32_t n = r.n;
+            cell             = std::max( rp + band + rmax + rp, 1.0 );
+            double mn[ 3 ] = { 1e300, 1e300, 1e300 }, mx[ 3 ] = { -1e300, -1e300, -1e300 };
+            for ( uint32_t a = 0; a < n; a++ )
+            {
+                mn[ 0 ] = std::min( mn[ 0 ], c[ a ].x ), mx[ 0 ] = std::max( mx[ 0 ], c[ a ].x );
+                mn[ 1 ] = std::min( mn[ 1 ], c[ a ].y ), mx[ 1 ] = std::max( mx[ 1 ], c[ a ].y );
+                mn[ 2 ] = std::min( mn[ 2 ], c[ a ].z ), mx[ 2 ] = std::max( mx[ 2 ], c[ a ].z );
+            }
+            ox = mn[ 0 ], oy = mn[ 1 ], oz = mn[ 2 ];
+            dx = int( ( mx[ 0 ] - mn[ 0 ] ) / cell ) + 1, dy = int( ( mx[ 1 ] - mn[ 1 ] ) / cell ) + 1, dz = int( ( mx[ 2 ] - mn[ 2 ] ) / cell ) + 1;
+            cellStart.assign( size_t( dx ) * dy * dz + 1, 0 );
+            std::vector<uint32_t> ca( n );
+            for ( uint32_t a = 0; a < n; a++ )
+            {
+                ca[ a ] = cellOf( c[ a ] );
+                cellStart[ ca[ a ] + 1 ]++;
+            }
+            for ( size_t q = 0; q + 1 < cellStart.size(); q++ )
+                cellStart[ q + 1 ] += cellStart[ q ];
+            cellAtoms.resize( n );
+            std::vector<uint32_t> f( cellStart.begin(), cellStart.end() - 1 );
+            for ( uint32_t a = 0; a < n; a++ )
+                cellAtoms[ f[ ca[ a ] ]++ ] = a;
+        }
+        int clampc( double v, double o, int d ) const { return std::min( std::max( int( std::floor( ( v - o ) / cell ) ), 0 ), d - 1 ); }
+        uint32_t cellOf( V3<double> p ) const
+        {
+            return uint32_t( ( size_t( clampc( p.z, oz, dz ) ) * dy + clampc( p.y, oy, dy ) ) * dx + clampc( p.x, ox, dx ) );
+        }
+
+        V3<double> pos64( uint32_t q ) const { return { r.concavePos64[ 3 * q ], r.concavePos64[ 3 * q + 1 ], r.concavePos64[ 3 * q + 2 ] }; }
+
+        // circle (i, j): centre, normal (i -> j), radius
+        void circle( uint32_t i, uint32_t j, V3<double> & ctr, V3<double> & nrm, double & rad ) const
+        {
+            const Geometry<double> g = circleGeometry( c[ i ], R[ i ], c[ j ], R[ j ] );
+            ctr = g.center, nrm = g.normal, rad = g.radius;
+        }
+
+        double eval( V3<double> q, double band, bool exterior, int64_t & code ) const
+        {
+            const double D    = rp + band;
+            double       g    = 1e300;
+            int64_t      gA   = -1;
+            double       best = D;
+            int64_t      bc   = int64_t( 5 ) << 40;
+            const int    cx = clampc( q.x, ox, dx ), cy = clampc( q.y, oy, dy ), cz = clampc( q.z, oz, dz );
+            for ( int z = std::max( cz - 1, 0 ); z <= std::min( cz + 1, dz - 1 ); z++ )
+                for ( int y = std::max( cy - 1, 0 ); y <= std::min( cy + 1, dy - 1 ); y++ )
+                    for ( int x = std::max( cx - 1, 0 ); x <= std::min( cx + 1, dx - 1 ); x++ )
+                    {
+                        const size_t cc = ( size_t( z ) * dy + y ) * dx + x;
+                        for ( uint32_t e = cellStart[ cc ]; e < cellStart[ cc + 1 ]; e++ )
+                        {
+                            const uint32_t   a    = cellAtoms[ e ];
+                            const V3<double> dq   = q - c[ a ];
+                            const double     dist = len( dq );
+                            const double     sd   = dist - R[ a ];
+                            if ( sd < g || ( sd == g && int64_t( a ) < gA ) )
+                                g = sd, gA = a;
+                            if ( exterior && comp[ a ] != extComp )
+                                continue;
+                            if ( std::fabs( sd ) < best && dist > 0 )
+                            {
+                                const V3<double> p   = c[ a ] + dq * ( R[ a ] / dist );
+                                bool             exp = true;
+                                for ( uint32_t l = 0; l < r.nbCount[ a ] && exp; l++ )
+                                {
+                                    const uint32_t j = r.nbIds[ size_t( a ) * r.K + l ];
+                                    exp              = len2( p - c[ j ] ) - R[ j ] * R[ j ] >= 0;
+                                }
+                                if ( exp )
+                                    best = std::fabs( sd ), bc = ( int64_t( 1 ) << 40 ) | a;
+                            }
+                            for ( uint32_t f = fullStart[ a ]; f < fullStart[ a + 1 ]; f++ )
+                            {
+                                V3<double> ctr, nrm;
+                                double     rad;
+                                circle( r.fullCircles[ 2 * f ], r.fullCircles[ 2 * f + 1 ], ctr, nrm, rad );
+                                const V3<double> v  = q - ctr;
+                                const double     ax = dot( v, nrm );
+                                const V3<double> w  = v - nrm * ax;
+                                const double     rho = len( w );
+                                const double     d   = std::sqrt( ax * ax + ( rho - rad ) * ( rho - rad ) );
+                                if ( d < best )
+                                    best = d, bc = ( int64_t( 2 ) << 40 ) | f;
+                            }
+                            for ( uint32_t s = segStart[ a ]; s < segStart[ a + 1 ]; s++ )
+                            {
+                                const uint32_t * sg = &r.segments[ 4 * size_t( s ) ];
+                                V3<double>       ctr, nrm;
+                                double           rad;
+                                circle( sg[ 0 ], sg[ 1 ], ctr, nrm, rad );
+                                const V3<double> v   = q - ctr;
+                                const double     ax  = dot( v, nrm );
+                                const V3<double> w   = v - nrm * ax;
+                                const double     rho = len( w );
+                                if ( rho <= 0 )
+                                    continue;
+                                const double d = std::sqrt( ax * ax + ( rho - rad ) * ( rho - rad ) );
+                                if ( d >= best )
+                                    continue;
+                                const V3<double> A  = pos64( sg[ 2 ] ) - ctr, B = pos64( sg[ 3 ] ) - ctr;
+                                auto             an = [ & ]( V3<double> u ) {
+                                    double t = std::atan2( dot( cross( A, u ), nrm ), dot( A, u ) );
+                                    return t < 0 ? t + 2 * M_PI : t;
+                                };
+                                if ( an( w ) <= an( B ) )
+                                    best = d, bc = ( int64_t( 3 ) << 40 ) | s;
+                            }
+                            for ( uint32_t t = vtxStart[ a ]; t < vtxStart[ a + 1 ]; t++ )
+                            {
+                                const uint32_t vq = vtxList[ t ];
+                                const double   d  = len( q - pos64( vq ) );
+                                if ( d < best )
+                                    best = d, bc = ( int64_t( 4 ) << 40 ) | vq;
+                            }
+                        }
+                    }
+            double f;
+            if ( gA < 0 )
+            {
+                code = int64_t( 5 ) << 40;
+                return band;
+            }
+            if ( g >= 0 && ( !exterior || comp[ gA ] == extComp ) )
+            {
+                f    = g;
+                code = gA;
+            }
+            else
+            {
+                f    = -best;
+                code = bc;
+            }
+            const double s = rp + f;
+            if ( s >= band || s <= -band )
+            {
+                code = int64_t( 5 ) << 40;
+                return s >= band ? band : -band;
+            }
+            return s;
+        }
+    };
 } // namespace orc
 
 using namespace orc;
 
 extern "C"
 {
+    // FP64 SDF of a complete-surface oracle build at n query points (xyz doubles); out[n] doubles,
+    // codes[n] (optional) winner codes. flags bit 0: exterior variant.
+    int oracle_sdf( void * h, const double * pts, uint64_t n, int flags, double band, double * out, int64_t * codes, int threads )
+    {
+        Result * r = static_cast<Result *>( h );
+        if ( r->mode != 0 )
+            return 1;
+        SdfOracle o( *r );
+        o.buildGrid( band );
+        parallelFor( uint32_t( n ), threads, [ & ]( uint32_t q ) {
+            int64_t code;
+            out[ q ] = o.eval( { pts[ 3 * q ], pts[ 3 * q + 1 ], pts[ 3 * q + 2 ] }, band, flags & 1, code );
+            if ( codes )
+                codes[ q ] = code;
+        } );
+        return 0;
+    }
+
+    // Surface component label per sorted atom and the exterior label (last entry).
+    uint32_t oracle_components( void * h, uint32_t * out )
+    {
+        Result * r = static_cast<Result *>( h );
+        SdfOracle o( *r );
+        for ( uint32_t a = 0; a < r->n; a++ )
+            out[ a ] = o.comp[ a ];
+        return o.extComp;
+    }
+
     // Build the surface of n atoms (float4 xyzr, input order) on the CPU.
     //   mode      0 = complete surface (bcs::Sesdf), 1 = exterior path (bcs::Ssesdf)
     //   precision 32 or 64
