@@ -148,10 +148,15 @@ namespace sesdf
         ALLOC_TRY( h, b.nbIds, n * h.K );
         ALLOC_TRY( h, b.status, n * h.K + 4, true );
         ALLOC_TRY( h, b.intMask, n * ( h.K / 32 ) );
+        ALLOC_TRY( h, b.atomCnt, 4 * n );
+        ALLOC_TRY( h, b.vStart, n + 1 );
         ALLOC_TRY( h, b.trimBase, n + 1 );
         ALLOC_TRY( h, b.fullBase, n + 1 );
+        ALLOC_TRY( h, b.wStart, n + 1 );
         ALLOC_TRY( h, b.nStart, n + 1 );
         ALLOC_TRY( h, b.torusBase, n + 1 );
+        ALLOC_TRY( h, b.stage, n * h.stageStride );
+        b.stageStride = h.stageStride;
         ALLOC_TRY( h, b.convexPatches, n );
         ALLOC_TRY( h, b.sectors, h.cap.sectors );
         ALLOC_TRY( h, b.fullCircles, h.cap.full );
@@ -163,16 +168,16 @@ namespace sesdf
         ALLOC_TRY( h, b.circleLists, h.cap.lists );
         ALLOC_TRY( h, b.segments, h.cap.segments );
         ALLOC_TRY( h, b.records, h.cap.records );
-        ALLOC_TRY( h, b.wStart, n + 1 );
         ALLOC_TRY( h, b.concaveIds, h.cap.records );
         ALLOC_TRY( h, b.concavePos, h.cap.records );
         ALLOC_TRY( h, b.concaveNb, uint64_t( h.cap.probeNb ) * h.maxProbeNb );
         ALLOC_TRY( h, b.torusPos, h.cap.torus );
         ALLOC_TRY( h, b.torusAxis, h.cap.torus );
         ALLOC_TRY( h, b.torusVs, h.cap.torus );
-        const uint64_t tiles = std::max<uint64_t>( h.cellNb / 2048 + 2, h.cap.trimmed / 1024 + 2 );
-        b.flagStride         = uint32_t( std::max<uint64_t>( n + 64, tiles + 64 ) );
-        ALLOC_TRY( h, b.flags, uint64_t( b.flagStride ) * 3, true );
+        // look-back flags: one word per scan tile of 2048 items
+        const uint64_t tiles = std::max<uint64_t>( std::max<uint64_t>( h.cellNb, n ), h.cap.trimmed ) / 2048 + 2;
+        b.flagStride         = uint32_t( tiles + 64 );
+        ALLOC_TRY( h, b.flags, uint64_t( b.flagStride ) * 4, true );
 
         b.n           = h.n;
         b.K           = h.K;
@@ -279,6 +284,11 @@ namespace sesdf
         const uint64_t I = uint64_t( c.intersectionWithNbNb ) + c.intersectionNoNbNb;
         need( h.cap.records, I );
         need( h.cap.probeNb, c.intersectionWithNbNb );
+        if ( c.overflow & OverflowAtomRecords )
+        {
+            h.stageStride *= 2;
+            grown = true;
+        }
         if ( h.mode == 0 )
         {
             need( h.cap.lists, 3 * I );
@@ -292,8 +302,7 @@ namespace sesdf
     static bool overflowed( const Handle & h )
     {
         const Control & c = *h.hctl;
-        const uint32_t  hard = c.overflow & ~uint32_t( OverflowAtomRecords );
-        if ( hard )
+        if ( c.overflow )
             return true;
         const uint64_t I = uint64_t( c.intersectionWithNbNb ) + c.intersectionNoNbNb;
         return I > h.cap.records || c.intersectionWithNbNb > h.cap.probeNb;

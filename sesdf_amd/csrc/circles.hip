@@ -1,16 +1,16 @@
-// Circle stage: neighbour query + circle visibility + convex sectors, fused, one wave per atom.
+// Circle stage: neighbour query + circle visibility (one wave per atom), then convex sectors,
+// intersected-circle ids and full circles from per-atom offsets.
 //
 // Replaces, for one atom i:
-//   findCirclesBetweenAtoms   (sesdf/src/sesdf/operations.cu:34-124; ssesdf/src/ssesdf/operations.cu:77-167)
+//   findCirclesBetweenAtoms        (sesdf/src/sesdf/operations.cu:34-124; ssesdf/src/ssesdf/operations.cu:77-167)
 //   computeCirclesVisibilityStatus (sesdf operations.cu:126-204; ssesdf operations.cu:169-244)
 //   the scans / copy_if of findCircles (sesdf/include/bcs/sesdf/operations.inl:32-93)
-//   buildConvexPatchesSectors (operations.inl:175-228)
+//   buildConvexPatchesSectors      (operations.inl:175-228)
 // Atom i's neighbour atoms are staged in LDS in i's local frame (coordinates relative to c_i, so
 // FP32 keeps ~1e-6 A resolution even at |c| ~ 400 A), the candidate rows of the grid stencil are
-// flattened across the 64 lanes, neighbours are appended with a ballot in ascending sorted index
-// (the reference's order), and output offsets (sectors, intersected circles, full circles) come
-// from an ordered decoupled look-back over atoms, so the outputs are deterministic and no
-// N x K mask scan is needed.
+// flattened across the 64 lanes, and neighbours are appended with a ballot in ascending sorted
+// index (the reference's order). Output offsets come from a scan of per-atom counts, so outputs
+// are deterministic and no N x K mask scan is needed.
 #include "kernels.h"
 
 namespace sesdf
@@ -33,7 +33,7 @@ namespace sesdf
 
     __device__ __forceinline__ CircleTest make_circle( f3 dj, float Ri2, float Rj2 )
     {
-        CircleTest c;
+        CircleTest  c;
         const float d2 = len2( dj );
         const float t  = ( Ri2 - Rj2 + d2 ) / ( 2.f * d2 );
         c.centre       = dj * t;
@@ -53,263 +53,290 @@ namespace sesdf
     }
 
     template<int KMAX, int MODE>
-    __global__ void __launch_bounds__( CirclesWaves * WaveSize ) circles_kernel( Buffers b )
+    __global__ void __launch_bounds__( CirclesWaves * WaveSize ) circles_count_kernel( Buffers b )
     {
         __shared__ uint32_t sIdx[ CirclesWaves ][ KMAX ];
         __shared__ float4   sAtom[ CirclesWaves ][ KMAX ];   // (c_k - c_i, Rk^2)
-        __shared__ uint8_t  sStatus[ CirclesWaves ][ KMAX ];
         __shared__ uint32_t sRowStart[ CirclesWaves ][ WaveSize ];
         __shared__ uint32_t sRowPrefix[ CirclesWaves ][ WaveSize + 1 ];
 
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
+        const uint32_t i    = blockIdx.x * CirclesWaves + w;
+        if ( i >= b.n )
+            return;
         const float    rp   = b.probeRadius;
         const float    rmax = __uint_as_float( b.ctl->radiusMaxBits );
         const uint32_t K    = b.K;
-        const uint32_t tag  = stage_tag( b.ctl, 2 );
 
-        for ( ;; )
+        const float4   ai  = b.atoms[ i ];
+        const f3       ci  = xyz( ai );
+        const float    Ri  = ai.w + rp;
+        const float    Ri2 = Ri * Ri;
+        const Molecule M   = b.mols[ find_molecule( b.mols, b.molNb, i ) ];
+
+        // ---- neighbour query over the flattened rows of the cell stencil -------------------------
+        const float range = ai.w + rmax + 2.f * rp + 1e-3f;
+        const int   reach = max( int( ceilf( range / M.cell ) ), 1 );
+        int         gx, gy, gz;
+        cell_of( M, ci.x, ci.y, ci.z, gx, gy, gz );
+        const int x0 = max( gx - reach, 0 ), x1 = min( gx + reach, M.dx - 1 );
+        const int y0 = max( gy - reach, 0 ), y1 = min( gy + reach, M.dy - 1 );
+        const int z0 = max( gz - reach, 0 ), z1 = min( gz + reach, M.dz - 1 );
+        const int ny = y1 - y0 + 1, rows = ny * ( z1 - z0 + 1 );
+
+        uint32_t count = 0;
+        for ( int rowBase = 0; rowBase < rows; rowBase += WaveSize )
         {
-            const uint32_t i = wave_ticket( &b.ctl->tickets[ TicketCircles ] );
-            if ( i >= b.n )
+            const int r  = rowBase + lane;
+            uint32_t  st = 0, len = 0;
+            if ( r < rows )
+            {
+                const int      yy = y0 + r % ny, zz = z0 + r / ny;
+                const uint32_t h0 = cell_index( M, x0, yy, zz );
+                st                = b.cellStart[ h0 ];
+                len               = b.cellStart[ h0 + uint32_t( x1 - x0 ) + 1 ] - st;
+            }
+            const uint32_t incl  = wave_incl_scan( len );
+            const uint32_t total = wave_bcast( incl, WaveSize - 1 );
+            sRowStart[ w ][ lane ]      = st;
+            sRowPrefix[ w ][ lane + 1 ] = incl;
+            if ( lane == 0 )
+                sRowPrefix[ w ][ 0 ] = 0;
+            wave_sync();
+            const int rowsHere = min( rows - rowBase, WaveSize );
+            for ( uint32_t base = 0; base < total; base += WaveSize )
+            {
+                const uint32_t e    = base + lane;
+                bool           pass = false;
+                uint32_t       j    = 0;
+                float4         aj   = make_float4( 0.f, 0.f, 0.f, 0.f );
+                if ( e < total )
+                {
+                    // row r: largest r with prefix[r] <= e
+                    int lo = 0, hi = rowsHere;
+                    while ( hi - lo > 1 )
+                    {
+                        const int mid = ( lo + hi ) >> 1;
+                        if ( sRowPrefix[ w ][ mid ] <= e )
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    j                = sRowStart[ w ][ lo ] + ( e - sRowPrefix[ w ][ lo ] );
+                    aj               = b.atoms[ j ];
+                    const f3    d    = xyz( aj ) - ci;
+                    const float Rs   = Ri + ( aj.w + rp );
+                    pass             = j != i && len2( d ) - Rs * Rs <= PredicateEps;
+                    aj.x = d.x, aj.y = d.y, aj.z = d.z;
+                    aj.w = ( aj.w + rp ) * ( aj.w + rp );
+                }
+                const uint64_t mask = ballot( pass );
+                const uint32_t slot = count + rank_below( mask );
+                if ( pass && slot < K )
+                {
+                    sIdx[ w ][ slot ]  = j;
+                    sAtom[ w ][ slot ] = aj;
+                }
+                count += uint32_t( __popcll( mask ) );
+            }
+            wave_sync();
+        }
+        if ( count > K && lane == 0 )
+            atomicAdd( &b.ctl->nbOverflowAtoms, 1u ); // rare: truncated list, as the reference
+        const uint32_t ni = min( count, K );
+        wave_sync();
+
+        if ( lane == 0 )
+            b.nbCount[ i ] = ni;
+        for ( uint32_t l = lane; l < ni; l += WaveSize )
+            b.nbIds[ size_t( i ) * K + l ] = sIdx[ w ][ l ];
+
+        // ---- visibility of circles (i, j): lanes over j, sequential over k in LDS -----------------
+        uint32_t nV = 0, nT = 0, nF = 0;
+        for ( uint32_t lb = 0; lb < K; lb += WaveSize )
+        {
+            const uint32_t l  = lb + lane;
+            uint32_t       st = 0;
+            uint32_t       j  = 0;
+            if ( l < ni )
+            {
+                j = sIdx[ w ][ l ];
+                if ( MODE == 0 || i < j )
+                {
+                    const float4     aj      = sAtom[ w ][ l ];
+                    const CircleTest c       = make_circle( xyz( aj ), Ri2, aj.w );
+                    uint32_t         visible = 1, complete = 1;
+                    for ( uint32_t k = 0; k < ni; k++ )
+                    {
+                        if ( k == l )
+                            continue;
+                        const float4 ak  = sAtom[ w ][ k ];
+                        const f3     v   = c.centre - xyz( ak );
+                        const f3     vx  = cross( v, c.normal );
+                        const float  h2  = len2( vx );
+                        const float  A   = ( ak.w + PredicateEps ) - len2( v ) - c.radius2;
+                        const float  lhs = c.fourR2 * h2;
+                        const float  A2  = A * A;
+                        if ( A > 0.f && lhs < A2 )
+                        {
+                            visible = 0;
+                            break;
+                        }
+                        if ( A > 0.f || lhs > A2 )
+                            complete = 0;
+                    }
+                    st = visible * ( 1 + complete );
+                }
+                b.status[ size_t( i ) * K + l ] = uint8_t( st );
+            }
+            const uint64_t mT = ballot( l < ni && st == 1 && j > i );
+            nV += uint32_t( __popcll( ballot( l < ni && st != 0 ) ) );
+            nT += uint32_t( __popcll( mT ) );
+            nF += uint32_t( __popcll( ballot( l < ni && st == 2 && j > i ) ) );
+            if ( MODE == 0 && lane == 0 )
+            {
+                b.intMask[ size_t( i ) * ( K / 32 ) + lb / 32 ] = uint32_t( mT );
+                if ( lb / 32 + 1 < K / 32 )
+                    b.intMask[ size_t( i ) * ( K / 32 ) + lb / 32 + 1 ] = uint32_t( mT >> 32 );
+            }
+            if ( lb + WaveSize >= ni && MODE == 1 )
                 break;
-            const float4   ai  = b.atoms[ i ];
-            const f3       ci  = xyz( ai );
-            const float    Ri  = ai.w + rp;
-            const float    Ri2 = Ri * Ri;
-            const Molecule M   = b.mols[ find_molecule( b.mols, b.molNb, i ) ];
-
-            // ---- neighbour query over the flattened rows of the cell stencil ---------------------
-            const float range = ai.w + rmax + 2.f * rp + 1e-3f;
-            const int   reach = max( int( ceilf( range / M.cell ) ), 1 );
-            int         gx, gy, gz;
-            cell_of( M, ci.x, ci.y, ci.z, gx, gy, gz );
-            const int x0 = max( gx - reach, 0 ), x1 = min( gx + reach, M.dx - 1 );
-            const int y0 = max( gy - reach, 0 ), y1 = min( gy + reach, M.dy - 1 );
-            const int z0 = max( gz - reach, 0 ), z1 = min( gz + reach, M.dz - 1 );
-            const int ny = y1 - y0 + 1, rows = ny * ( z1 - z0 + 1 );
-
-            uint32_t count = 0;
-            for ( int rowBase = 0; rowBase < rows; rowBase += WaveSize )
+        }
+        if ( lane == 0 )
+        {
+            b.atomCnt[ i ] = nV;
+            if ( MODE == 0 )
             {
-                const int r   = rowBase + lane;
-                uint32_t  st  = 0, len = 0;
-                if ( r < rows )
-                {
-                    const int      yy = y0 + r % ny, zz = z0 + r / ny;
-                    const uint32_t h0 = cell_index( M, x0, yy, zz );
-                    st                = b.cellStart[ h0 ];
-                    len               = b.cellStart[ h0 + uint32_t( x1 - x0 ) + 1 ] - st;
-                }
-                const uint32_t incl  = wave_incl_scan( len );
-                const uint32_t total = wave_bcast( incl, WaveSize - 1 );
-                sRowStart[ w ][ lane ]      = st;
-                sRowPrefix[ w ][ lane + 1 ] = incl;
-                if ( lane == 0 )
-                    sRowPrefix[ w ][ 0 ] = 0;
-                wave_sync();
-                const int rowsHere = min( rows - rowBase, WaveSize );
-                for ( uint32_t base = 0; base < total; base += WaveSize )
-                {
-                    const uint32_t e    = base + lane;
-                    bool           pass = false;
-                    uint32_t       j    = 0;
-                    float4         aj   = make_float4( 0.f, 0.f, 0.f, 0.f );
-                    if ( e < total )
-                    {
-                        // row r: largest r with prefix[r] <= e
-                        int lo = 0, hi = rowsHere;
-                        while ( hi - lo > 1 )
-                        {
-                            const int mid = ( lo + hi ) >> 1;
-                            if ( sRowPrefix[ w ][ mid ] <= e )
-                                lo = mid;
-                            else
-                                hi = mid;
-                        }
-                        j  = sRowStart[ w ][ lo ] + ( e - sRowPrefix[ w ][ lo ] );
-                        aj = b.atoms[ j ];
-                        const f3    d  = xyz( aj ) - ci;
-                        const float Rs = Ri + ( aj.w + rp );
-                        pass           = j != i && len2( d ) - Rs * Rs <= PredicateEps;
-                        aj.x = d.x, aj.y = d.y, aj.z = d.z;
-                        aj.w = ( aj.w + rp ) * ( aj.w + rp );
-                    }
-                    const uint64_t mask = ballot( pass );
-                    const uint32_t slot = count + rank_below( mask );
-                    if ( pass && slot < K )
-                    {
-                        sIdx[ w ][ slot ]  = j;
-                        sAtom[ w ][ slot ] = aj;
-                    }
-                    count += uint32_t( __popcll( mask ) );
-                }
-                wave_sync();
+                b.atomCnt[ b.n + i ]     = nT;
+                b.atomCnt[ 2 * b.n + i ] = nF;
             }
-            if ( count > K && lane == 0 )
-                atomicAdd( &b.ctl->nbOverflowAtoms, 1u );
-            const uint32_t ni = min( count, K );
-            wave_sync();
+        }
+    }
 
-            if ( lane == 0 )
-                b.nbCount[ i ] = ni;
-            for ( uint32_t l = lane; l < ni; l += WaveSize )
-                b.nbIds[ size_t( i ) * K + l ] = sIdx[ w ][ l ];
-
-            // ---- visibility of circles (i, j): lanes over j, sequential over k in LDS -------------
-            uint32_t nV = 0, nT = 0, nF = 0;
-            for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+    // Sectors, intersected circles and full circles of atom i at the offsets of the scan.
+    __global__ void __launch_bounds__( CirclesWaves * WaveSize ) circles_emit_kernel( Buffers b )
+    {
+        const int      w    = threadIdx.x / WaveSize;
+        const int      lane = lane_id();
+        const uint32_t i    = blockIdx.x * CirclesWaves + w;
+        if ( i >= b.n )
+            return;
+        const uint32_t K    = b.K;
+        const float    rp   = b.probeRadius;
+        const uint32_t ni   = b.nbCount[ i ];
+        const uint32_t v0   = b.vStart[ i ];
+        if ( lane == 0 )
+            b.convexPatches[ i ] = make_uint2( v0, b.vStart[ i + 1 ] );
+        if ( ni == 0 )
+            return;
+        const float4 ai  = b.atoms[ i ];
+        const f3     ci  = xyz( ai );
+        const float  Ri  = ai.w + rp;
+        const float  Ri2 = Ri * Ri;
+        uint32_t     runV = v0, runT = b.trimBase[ i ], runF = b.fullBase[ i ];
+        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+        {
+            const uint32_t l    = lb + lane;
+            const bool     live = l < ni;
+            const uint32_t st   = live ? b.status[ size_t( i ) * K + l ] : 0u;
+            const uint32_t j    = live ? b.nbIds[ size_t( i ) * K + l ] : 0u;
+            const bool     isV = live && st != 0, isT = live && st == 1 && j > i, isF = live && st == 2 && j > i;
+            const uint64_t mV = ballot( isV ), mT = ballot( isT ), mF = ballot( isF );
+            if ( isV )
             {
-                const uint32_t l  = lb + lane;
-                uint32_t       st = 0;
-                uint32_t       j  = 0;
-                if ( l < ni )
+                const uint32_t dst = runV + rank_below( mV );
+                if ( dst < b.capSectors )
                 {
-                    j = sIdx[ w ][ l ];
-                    if ( MODE == 0 || i < j )
-                    {
-                        const float4     aj = sAtom[ w ][ l ];
-                        const CircleTest c  = make_circle( xyz( aj ), Ri2, aj.w );
-                        uint32_t         visible = 1, complete = 1;
-                        for ( uint32_t k = 0; k < ni; k++ )
-                        {
-                            if ( k == l )
-                                continue;
-                            const float4 ak  = sAtom[ w ][ k ];
-                            const f3     v   = c.centre - xyz( ak );
-                            const f3     vx  = cross( v, c.normal );
-                            const float  h2  = len2( vx );
-                            const float  A   = ( ak.w + PredicateEps ) - len2( v ) - c.radius2;
-                            const float  lhs = c.fourR2 * h2;
-                            const float  A2  = A * A;
-                            if ( A > 0.f && lhs < A2 )
-                            {
-                                visible = 0;
-                                break;
-                            }
-                            if ( A > 0.f || lhs > A2 )
-                                complete = 0;
-                        }
-                        st = visible * ( 1 + complete );
-                    }
-                    b.status[ size_t( i ) * K + l ] = uint8_t( st );
-                    sStatus[ w ][ l ]               = uint8_t( st );
+                    const float4     aj = b.atoms[ j ];
+                    const float      Rj = aj.w + rp;
+                    const CircleTest c  = make_circle( xyz( aj ) - ci, Ri2, Rj * Rj );
+                    const f3         o  = orthogonal_vector( c.normal );
+                    const f3         x  = c.centre + o * sqrtf( c.radius2 );
+                    const f3         xr = mk( x.x / Ri, x.y / Ri, x.z / Ri );
+                    const float      ca = fminf( fmaxf( dot( c.normal, xr ), -1.f ), 1.f );
+                    b.sectors[ dst ]    = make_float4( c.normal.x, c.normal.y, c.normal.z, acosf( ca ) );
                 }
-                nV += uint32_t( __popcll( ballot( l < ni && st != 0 ) ) );
-                nT += uint32_t( __popcll( ballot( l < ni && st == 1 && j > i ) ) );
-                nF += uint32_t( __popcll( ballot( l < ni && st == 2 && j > i ) ) );
+                else
+                    set_overflow( b.ctl, OverflowSectors );
             }
-
-            if ( MODE == 1 )
+            if ( isT )
             {
-                if ( lane == 0 && nV )
-                    atomicAdd( &b.ctl->visibleCircleNb, nV );
-                continue;
+                const uint32_t dst = runT + rank_below( mT );
+                if ( dst < b.capTrimmed )
+                {
+                    b.trimmedToGlobal[ dst ] = i * K + l;
+                    b.circleIntNb[ dst ]     = 0u;
+                }
+                else
+                    set_overflow( b.ctl, OverflowTrimmed );
             }
-
-            // ---- ordered offsets, then sectors / intersected circles / full circles --------------
-            const uint32_t local[ 3 ] = { nV, nT, nF };
-            uint32_t       excl[ 3 ];
-            wave_lookback<3>( b.flags, b.flagStride, i, tag, local, excl );
-            if ( lane == 0 )
+            if ( isF )
             {
-                b.convexPatches[ i ] = make_uint2( excl[ 0 ], excl[ 0 ] + nV );
-                b.trimBase[ i ]      = excl[ 1 ];
-                b.fullBase[ i ]      = excl[ 2 ];
-                if ( i == b.n - 1 )
-                {
-                    b.ctl->sectorNb            = excl[ 0 ] + nV;
-                    b.ctl->intersectedCircleNb = excl[ 1 ] + nT;
-                    b.ctl->fullCircleNb        = excl[ 2 ] + nF;
-                }
+                const uint32_t dst = runF + rank_below( mF );
+                if ( dst < b.capFull )
+                    b.fullCircles[ dst ] = make_uint2( i, j );
+                else
+                    set_overflow( b.ctl, OverflowFullCircles );
             }
-            wave_sync();
-            uint32_t runV = excl[ 0 ], runT = excl[ 1 ], runF = excl[ 2 ];
-            for ( uint32_t lb = 0; lb < K; lb += WaveSize )
-            {
-                const uint32_t l    = lb + lane;
-                const bool     live = l < ni;
-                const uint32_t st   = live ? sStatus[ w ][ l ] : 0u;
-                const uint32_t j    = live ? sIdx[ w ][ l ] : 0u;
-                const bool     isV = live && st != 0, isT = live && st == 1 && j > i, isF = live && st == 2 && j > i;
-                const uint64_t mV = ballot( isV ), mT = ballot( isT ), mF = ballot( isF );
-                if ( lane == 0 )
-                {
-                    b.intMask[ size_t( i ) * ( K / 32 ) + lb / 32 ] = uint32_t( mT );
-                    if ( lb / 32 + 1 < K / 32 )
-                        b.intMask[ size_t( i ) * ( K / 32 ) + lb / 32 + 1 ] = uint32_t( mT >> 32 );
-                }
-                if ( isV )
-                {
-                    const uint32_t dst = runV + rank_below( mV );
-                    if ( dst < b.capSectors )
-                    {
-                        const float4     aj = sAtom[ w ][ l ];
-                        const CircleTest c  = make_circle( xyz( aj ), Ri2, aj.w );
-                        const f3         o  = orthogonal_vector( c.normal );
-                        const f3         x  = c.centre + o * sqrtf( c.radius2 );
-                        const f3         xr = mk( x.x / Ri, x.y / Ri, x.z / Ri );
-                        const float      ca = fminf( fmaxf( dot( c.normal, xr ), -1.f ), 1.f );
-                        b.sectors[ dst ]    = make_float4( c.normal.x, c.normal.y, c.normal.z, acosf( ca ) );
-                    }
-                    else
-                        set_overflow( b.ctl, OverflowSectors );
-                }
-                if ( isT )
-                {
-                    const uint32_t dst = runT + rank_below( mT );
-                    if ( dst < b.capTrimmed )
-                    {
-                        b.trimmedToGlobal[ dst ] = i * K + l;
-                        b.circleIntNb[ dst ]     = 0u;
-                    }
-                    else
-                        set_overflow( b.ctl, OverflowTrimmed );
-                }
-                if ( isF )
-                {
-                    const uint32_t dst = runF + rank_below( mF );
-                    if ( dst < b.capFull )
-                        b.fullCircles[ dst ] = make_uint2( i, j );
-                    else
-                        set_overflow( b.ctl, OverflowFullCircles );
-                }
-                runV += uint32_t( __popcll( mV ) );
-                runT += uint32_t( __popcll( mT ) );
-                runF += uint32_t( __popcll( mF ) );
-            }
+            runV += uint32_t( __popcll( mV ) );
+            runT += uint32_t( __popcll( mT ) );
+            runF += uint32_t( __popcll( mF ) );
         }
     }
 
     int device_cu_count()
     {
+        static int cached = 0;
+        if ( cached )
+            return cached;
         int dev = 0, cus = 0;
         if ( hipGetDevice( &dev ) != hipSuccess )
             return 256;
         if ( hipDeviceGetAttribute( &cus, hipDeviceAttributeMultiprocessorCount, dev ) != hipSuccess || cus <= 0 )
             return 256;
+        cached = cus;
         return cus;
-    }
-
-    template<int KMAX>
-    static void launch_circles_k( const Buffers & b, hipStream_t s )
-    {
-        const uint32_t blocksWanted = ( b.n + CirclesWaves - 1 ) / CirclesWaves;
-        const uint32_t cap          = uint32_t( device_cu_count() ) * 16u;
-        const uint32_t blocks       = blocksWanted < cap ? blocksWanted : cap;
-        if ( b.mode == 0 )
-            circles_kernel<KMAX, 0><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
-        else
-            circles_kernel<KMAX, 1><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
     }
 
     void launch_circles( const Buffers & b, hipStream_t s )
     {
         if ( b.n == 0 )
             return;
+        const uint32_t blocks = ( b.n + CirclesWaves - 1 ) / CirclesWaves;
         if ( b.K <= 128 )
-            launch_circles_k<128>( b, s );
+        {
+            if ( b.mode == 0 )
+                circles_count_kernel<128, 0><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
+            else
+                circles_count_kernel<128, 1><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
+        }
         else
-            launch_circles_k<320>( b, s );
+        {
+            if ( b.mode == 0 )
+                circles_count_kernel<320, 0><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
+            else
+                circles_count_kernel<320, 1><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
+        }
+        ScanArgs a {};
+        a.n      = b.n;
+        a.ticket = TicketCircles;
+        a.stage  = 2;
+        a.in[ 0 ]     = b.atomCnt;
+        a.out[ 0 ]    = b.vStart;
+        a.totals[ 0 ] = b.mode == 0 ? &b.ctl->sectorNb : &b.ctl->visibleCircleNb;
+        if ( b.mode == 0 )
+        {
+            a.in[ 1 ]     = b.atomCnt + b.n;
+            a.in[ 2 ]     = b.atomCnt + 2 * size_t( b.n );
+            a.out[ 1 ]    = b.trimBase;
+            a.out[ 2 ]    = b.fullBase;
+            a.totals[ 1 ] = &b.ctl->intersectedCircleNb;
+            a.totals[ 2 ] = &b.ctl->fullCircleNb;
+            launch_scan<3>( b, a, s );
+            circles_emit_kernel<<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
+        }
+        else
+            launch_scan<1>( b, a, s );
     }
 } // namespace sesdf

@@ -1,10 +1,13 @@
-// Uniform-grid build: counting sort of atoms by cell (stable in input order), one pass per step.
+// Uniform-grid build (counting sort of atoms by cell, stable in input order) and the generic
+// tile scan used by every stage for its per-item offsets.
 //
 // Replaces AccelerationGrid::build (sesdf/src/cuda/grid.cu:99-132: hash -> radix sort -> cell
-// start/end -> gather). Here the cell table is a dense per-molecule brick whose edge is chosen by
-// the caller (not the reference's fixed 16^3 / 32^3 heuristic): histogram with atomics, decoupled
-// look-back scan of the counts, scatter, then an in-cell rank by input index so that the sorted
-// order (and therefore every output order) is deterministic.
+// start/end -> gather) and the thrust scans / copy_if / count_if of the reference stages
+// (sesdf/include/bcs/sesdf/operations.inl:46-70, 149-156, 500-517; ssesdf/src/ssesdf/operations.cu:44-54).
+// The cell table is a dense per-molecule brick whose edge is chosen by the caller (not the
+// reference's fixed 16^3 / 32^3 heuristic): histogram with atomics, decoupled look-back scan of
+// the counts, scatter, then an in-cell rank by input index so that the sorted order (and every
+// output order) is deterministic.
 #include "kernels.h"
 
 namespace sesdf
@@ -23,8 +26,9 @@ namespace sesdf
 
     __global__ void __launch_bounds__( 256 ) grid_hash_kernel( Buffers b )
     {
-        const uint32_t a     = blockIdx.x * blockDim.x + threadIdx.x;
-        float          r     = 0.f;
+        __shared__ float waveMax[ 4 ];
+        const uint32_t   a = blockIdx.x * blockDim.x + threadIdx.x;
+        float            r = 0.f;
         if ( a < b.n )
         {
             const float4   p = b.input[ a ];
@@ -37,77 +41,167 @@ namespace sesdf
             atomicAdd( b.cellCount + c, 1u );
             r = fmaxf( p.w, 0.f );
         }
-        // wave max of the radii, one atomic per wave
         for ( int o = 32; o > 0; o >>= 1 )
             r = fmaxf( r, __shfl_xor( r, o, WaveSize ) );
         if ( lane_id() == 0 )
-            atomicMax( &b.ctl->radiusMaxBits, __float_as_uint( r ) );
+            waveMax[ threadIdx.x / WaveSize ] = r;
+        __syncthreads();
+        if ( threadIdx.x == 0 )
+        {
+            const float m = fmaxf( fmaxf( waveMax[ 0 ], waveMax[ 1 ] ), fmaxf( waveMax[ 2 ], waveMax[ 3 ] ) );
+            atomicMax( &b.ctl->radiusMaxBits, __float_as_uint( m ) ); // one atomic per block
+        }
     }
 
-    // Exclusive scan of cellCount into cellStart (tiles of 2048 cells, ticketed look-back);
-    // clears cellCount for the next build.
     constexpr int ScanThreads = 256;
     constexpr int ScanItems   = 8;
     constexpr int ScanTile    = ScanThreads * ScanItems;
 
-    __global__ void __launch_bounds__( ScanThreads ) cell_scan_kernel( Buffers b )
+    template<int NC>
+    __global__ void __launch_bounds__( ScanThreads ) scan_kernel( ScanArgs a, Control * ctl, uint64_t * flags, uint32_t flagStride )
     {
-        __shared__ uint32_t waveSums[ ScanThreads / WaveSize ];
-        __shared__ uint32_t tileBase;
+        __shared__ uint32_t waveSums[ NC ][ ScanThreads / WaveSize ];
+        __shared__ uint32_t tileBase[ NC ];
         __shared__ uint32_t ticket;
-        const uint32_t      tiles = ( b.cellNb + ScanTile - 1 ) / ScanTile;
-        const uint32_t      tag   = stage_tag( b.ctl, 1 );
+        __shared__ uint32_t redMax[ ScanThreads / WaveSize ], redSum[ ScanThreads / WaveSize ];
+        const uint32_t      n     = a.countPtr ? min( *a.countPtr, a.n ) : a.n;
+        const uint32_t      tiles = ( n + ScanTile - 1 ) / ScanTile;
+        const uint32_t      tag   = stage_tag( ctl, a.stage );
+        const int           w     = threadIdx.x / WaveSize;
+        if ( n == 0 )
+        {
+            if ( blockIdx.x == 0 && threadIdx.x == 0 )
+                for ( int c = 0; c < NC; c++ )
+                {
+                    a.out[ c ][ 0 ] = 0;
+                    if ( a.totals[ c ] )
+                        *a.totals[ c ] = 0;
+                }
+            return;
+        }
         for ( ;; )
         {
             if ( threadIdx.x == 0 )
-                ticket = atomicAdd( &b.ctl->tickets[ TicketCellScan ], 1u );
+                ticket = atomicAdd( &ctl->tickets[ a.ticket ], 1u );
             __syncthreads();
             const uint32_t t = ticket;
             if ( t >= tiles )
                 break;
             const uint32_t base = t * ScanTile + threadIdx.x * ScanItems;
-            uint32_t       v[ ScanItems ];
-            uint32_t       sum = 0;
+            uint32_t       v[ NC ][ ScanItems ];
+            uint32_t       s[ NC ], incl[ NC ];
+            uint32_t       mx = 0, sm = 0;
+#pragma unroll
+            for ( int c = 0; c < NC; c++ )
+                s[ c ] = 0;
 #pragma unroll
             for ( int q = 0; q < ScanItems; q++ )
             {
-                v[ q ] = base + q < b.cellNb ? b.cellCount[ base + q ] : 0u;
-                sum += v[ q ];
+                const bool in = base + q < n;
+#pragma unroll
+                for ( int c = 0; c < NC; c++ )
+                {
+                    uint32_t x = 0;
+                    if ( in )
+                        x = ( a.halfSecond && c == 1 ) ? a.in[ 0 ][ base + q ] / 2 : a.in[ c ][ base + q ];
+                    v[ c ][ q ] = x;
+                    s[ c ] += x;
+                }
+                if ( in && a.maxOut )
+                    mx = max( mx, v[ 0 ][ q ] + ( NC > 1 && !a.halfSecond ? v[ 1 ][ q ] : 0u ) );
+                if ( in && a.sumOut )
+                    sm += a.in[ NC ][ base + q ];
             }
-            const uint32_t incl = wave_incl_scan( sum );
-            const int      w    = threadIdx.x / WaveSize;
-            if ( lane_id() == WaveSize - 1 )
-                waveSums[ w ] = incl;
+#pragma unroll
+            for ( int c = 0; c < NC; c++ )
+            {
+                incl[ c ] = wave_incl_scan( s[ c ] );
+                if ( lane_id() == WaveSize - 1 )
+                    waveSums[ c ][ w ] = incl[ c ];
+            }
+            if ( a.maxOut || a.sumOut )
+            {
+                for ( int o = 32; o > 0; o >>= 1 )
+                {
+                    mx = max( mx, __shfl_xor( mx, o, WaveSize ) );
+                    sm += __shfl_xor( sm, o, WaveSize );
+                }
+                if ( lane_id() == 0 )
+                    redMax[ w ] = mx, redSum[ w ] = sm;
+            }
             __syncthreads();
             if ( w == 0 )
             {
-                uint32_t tot = 0;
-                for ( int q = 0; q < ScanThreads / WaveSize; q++ )
-                    tot += waveSums[ q ];
-                uint32_t local[ 1 ] = { tot }, excl[ 1 ];
-                wave_lookback<1>( b.flags, b.flagStride, t, tag, local, excl );
+                uint32_t local[ NC ], excl[ NC ];
+#pragma unroll
+                for ( int c = 0; c < NC; c++ )
+                {
+                    local[ c ] = 0;
+                    for ( int q = 0; q < ScanThreads / WaveSize; q++ )
+                        local[ c ] += waveSums[ c ][ q ];
+                }
+                wave_lookback<NC>( flags, flagStride, t, tag, local, excl );
                 if ( lane_id() == 0 )
                 {
-                    tileBase = excl[ 0 ];
-                    if ( t == tiles - 1 )
-                        b.cellStart[ b.cellNb ] = excl[ 0 ] + tot;
+#pragma unroll
+                    for ( int c = 0; c < NC; c++ )
+                    {
+                        tileBase[ c ] = excl[ c ];
+                        if ( t == tiles - 1 )
+                        {
+                            a.out[ c ][ n ] = excl[ c ] + local[ c ];
+                            if ( a.totals[ c ] )
+                                *a.totals[ c ] = excl[ c ] + local[ c ];
+                        }
+                    }
+                    if ( a.maxOut )
+                        atomicMax( a.maxOut, max( max( redMax[ 0 ], redMax[ 1 ] ), max( redMax[ 2 ], redMax[ 3 ] ) ) );
+                    if ( a.sumOut )
+                        atomicAdd( a.sumOut, redSum[ 0 ] + redSum[ 1 ] + redSum[ 2 ] + redSum[ 3 ] );
                 }
             }
             __syncthreads();
-            uint32_t run = tileBase + incl - sum;
-            for ( int q = 0; q < w; q++ )
-                run += waveSums[ q ];
 #pragma unroll
-            for ( int q = 0; q < ScanItems; q++ )
-                if ( base + q < b.cellNb )
-                {
-                    b.cellStart[ base + q ] = run;
-                    b.cellCount[ base + q ] = 0u;
-                    run += v[ q ];
-                }
+            for ( int c = 0; c < NC; c++ )
+            {
+                uint32_t run = tileBase[ c ] + incl[ c ] - s[ c ];
+                for ( int q = 0; q < w; q++ )
+                    run += waveSums[ c ][ q ];
+#pragma unroll
+                for ( int q = 0; q < ScanItems; q++ )
+                    if ( base + q < n )
+                    {
+                        a.out[ c ][ base + q ] = run;
+                        run += v[ c ][ q ];
+                    }
+            }
+            if ( a.clearInput || a.clearAlso )
+#pragma unroll
+                for ( int q = 0; q < ScanItems; q++ )
+                    if ( base + q < n )
+                    {
+                        if ( a.clearInput )
+                            for ( int c = 0; c < NC; c++ )
+                                if ( !( a.halfSecond && c == 1 ) )
+                                    a.in[ c ][ base + q ] = 0u;
+                        if ( a.clearAlso )
+                            a.clearAlso[ base + q ] = 0u;
+                    }
             __syncthreads();
         }
     }
+
+    template<int NC>
+    void launch_scan( const Buffers & b, const ScanArgs & a, hipStream_t s )
+    {
+        const uint32_t tiles = ( a.n + ScanTile - 1 ) / ScanTile;
+        const uint32_t cap   = uint32_t( device_cu_count() ) * 4u;
+        const uint32_t grid  = tiles < 1 ? 1 : ( tiles < cap ? tiles : cap );
+        scan_kernel<NC><<<grid, ScanThreads, 0, s>>>( a, b.ctl, b.flags, b.flagStride );
+    }
+    template void launch_scan<1>( const Buffers &, const ScanArgs &, hipStream_t );
+    template void launch_scan<2>( const Buffers &, const ScanArgs &, hipStream_t );
+    template void launch_scan<3>( const Buffers &, const ScanArgs &, hipStream_t );
 
     __global__ void __launch_bounds__( 256 ) grid_scatter_kernel( Buffers b )
     {
@@ -132,7 +226,7 @@ namespace sesdf
         uint32_t       rank = 0;
         for ( uint32_t q = st; q < en; q++ )
             rank += b.slotInput[ q ] < id;
-        const uint32_t dst      = st + rank;
+        const uint32_t dst       = st + rank;
         b.sortedToInitial[ dst ] = id;
         b.atoms[ dst ]           = b.input[ id ];
         if ( s == st )
@@ -148,9 +242,14 @@ namespace sesdf
         if ( b.n == 0 )
             return;
         grid_hash_kernel<<<blocks_for( b.n, 256 ), 256, 0, s>>>( b );
-        const uint32_t tiles = ( b.cellNb + ScanTile - 1 ) / ScanTile;
-        const uint32_t grid  = tiles < 1024u ? tiles : 1024u;
-        cell_scan_kernel<<<grid, ScanThreads, 0, s>>>( b );
+        ScanArgs a {};
+        a.in[ 0 ]    = b.cellCount;
+        a.out[ 0 ]   = b.cellStart;
+        a.n          = b.cellNb;
+        a.ticket     = TicketCellScan;
+        a.stage      = 1;
+        a.clearInput = 1;
+        launch_scan<1>( b, a, s );
         grid_scatter_kernel<<<blocks_for( b.n, 256 ), 256, 0, s>>>( b );
         grid_rank_kernel<<<blocks_for( b.n, 256 ), 256, 0, s>>>( b );
     }

@@ -44,6 +44,7 @@ namespace sesdf
         bool        built     = false;
         bool        inputValid = false;
         uint32_t    rebuilds  = 0;
+        uint32_t    stageStride = 32;    // staged concave records per atom (grows on overflow)
 
         const float *         hostAtoms = nullptr;
         std::vector<Molecule> mols;

@@ -1,11 +1,13 @@
 // Concave patches, toroidal segments, probe neighbourhoods and exterior torus records.
 //
-//   intersections_kernel  <- findIntersectionsBetweenCircles (sesdf/include/bcs/sesdf/operations.inl:230-486;
-//                            ssesdf/include/bcs/ssesdf/operation.inl:100-351) + the |w| sort of
-//                            findIntersections (operations.inl:128-141): one wave per atom i,
+//   intersections_count_kernel <- findIntersectionsBetweenCircles (sesdf/include/bcs/sesdf/operations.inl:230-486;
+//                            ssesdf/include/bcs/ssesdf/operation.inl:100-351): one wave per atom i,
 //                            candidate pairs (ij, ik) flattened over the lanes, N(j) membership by
-//                            binary search in j's ascending list, occluders N(i) from LDS.
-//   circle_scan_kernel    <- the per-circle offset scans (operations.inl:149-156, 500-517)
+//                            binary search in j's ascending list, occluders N(i) from LDS; records
+//                            staged per atom (4 bytes each).
+//   intersections_emit_kernel <- the |w| sort of findIntersections (operations.inl:128-141) and
+//                            the per-circle counts; offsets from a scan of per-atom counts.
+//   launch_circle_scan    <- the per-circle offset scans (operations.inl:149-156, 500-517)
 //   fill_kernel           <- fillIntersections (sesdf/src/sesdf/operations.cu:206-269;
 //                            ssesdf/src/ssesdf/operations.cu:287-329)
 //   segments_kernel       <- buildSegments (operations.inl:552-668)
@@ -18,7 +20,6 @@
 namespace sesdf
 {
     constexpr int      InterWaves  = 4;
-    constexpr int      MaxAtomRecs = 256;   // concave records held in LDS per atom
     constexpr uint32_t RecMinus    = 1u << 31;
     constexpr uint32_t RecHasNb    = 1u << 30;
 
@@ -103,317 +104,231 @@ namespace sesdf
     }
 
     template<int KMAX, int MODE>
-    __global__ void __launch_bounds__( InterWaves * WaveSize ) intersections_kernel( Buffers b )
+    __global__ void __launch_bounds__( InterWaves * WaveSize ) intersections_count_kernel( Buffers b )
     {
         __shared__ uint32_t sIdx[ InterWaves ][ KMAX ];
         __shared__ float4   sAtom[ InterWaves ][ KMAX ];   // (c_l - c_i, R_l^2)
         __shared__ float    sRad[ InterWaves ][ KMAX ];    // vdW radius r_l
         __shared__ uint16_t sCand[ InterWaves ][ KMAX ];
-        __shared__ uint4    sRec[ InterWaves ][ MaxAtomRecs ];
 
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
+        const uint32_t i    = blockIdx.x * InterWaves + w;
+        if ( i >= b.n )
+            return;
         const float    rp   = b.probeRadius;
         const uint32_t K    = b.K;
-        const uint32_t tag  = stage_tag( b.ctl, 3 );
+        const float4   ai   = b.atoms[ i ];
+        const f3       ci   = xyz( ai );
+        const float    Ri   = ai.w + rp;
+        const float    Ri2  = Ri * Ri;
+        const uint32_t ni   = b.nbCount[ i ];
+        uint32_t * const stage = b.stage + size_t( i ) * b.stageStride;
 
-        for ( ;; )
+        uint32_t m = 0;
+        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
         {
-            const uint32_t i = wave_ticket( &b.ctl->tickets[ TicketIntersections ] );
-            if ( i >= b.n )
-                break;
-            const float4   ai  = b.atoms[ i ];
-            const f3       ci  = xyz( ai );
-            const float    Ri  = ai.w + rp;
-            const float    Ri2 = Ri * Ri;
-            const uint32_t ni  = b.nbCount[ i ];
-
-            uint32_t m = 0;
-            for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+            const uint32_t l    = lb + lane;
+            bool           cand = false;
+            if ( l < ni )
             {
-                const uint32_t l    = lb + lane;
-                bool           cand = false;
-                if ( l < ni )
-                {
-                    const uint32_t j  = b.nbIds[ size_t( i ) * K + l ];
-                    const float4   aj = b.atoms[ j ];
-                    const float    Rj = aj.w + rp;
-                    sIdx[ w ][ l ]    = j;
-                    sAtom[ w ][ l ]   = make_float4( aj.x - ci.x, aj.y - ci.y, aj.z - ci.z, Rj * Rj );
-                    sRad[ w ][ l ]    = aj.w;
-                    cand              = j > i && b.status[ size_t( i ) * K + l ] == 1;
-                }
-                const uint64_t mask = ballot( cand );
-                if ( cand )
-                    sCand[ w ][ m + rank_below( mask ) ] = uint16_t( l );
-                m += uint32_t( __popcll( mask ) );
+                const uint32_t j  = b.nbIds[ size_t( i ) * K + l ];
+                const float4   aj = b.atoms[ j ];
+                const float    Rj = aj.w + rp;
+                sIdx[ w ][ l ]    = j;
+                sAtom[ w ][ l ]   = make_float4( aj.x - ci.x, aj.y - ci.y, aj.z - ci.z, Rj * Rj );
+                sRad[ w ][ l ]    = aj.w;
+                cand              = j > i && b.status[ size_t( i ) * K + l ] == 1;
             }
-            wave_sync();
+            const uint64_t mask = ballot( cand );
+            if ( cand )
+                sCand[ w ][ m + rank_below( mask ) ] = uint16_t( l );
+            m += uint32_t( __popcll( mask ) );
+        }
+        wave_sync();
 
-            // ---- candidate pairs (a < b over the intersected circles of i), flattened ------------
-            const uint32_t P    = m * ( m - 1 ) / 2;
-            uint32_t       nRec = 0;
-            for ( uint32_t pb = 0; pb < P; pb += WaveSize )
+        // ---- candidate pairs (a < c over the intersected circles of i), flattened over the lanes ----
+        const uint32_t P     = m * ( m - 1 ) / 2;
+        uint32_t       nRec  = 0, nW = 0, nFp64 = 0;
+        for ( uint32_t pb = 0; pb < P; pb += WaveSize )
+        {
+            const uint32_t p    = pb + lane;
+            int            nOut = 0;
+            uint32_t       out0 = 0, out1 = 0;
+            bool           fp64 = false;
+            if ( p < P )
             {
-                const uint32_t p    = pb + lane;
-                int            nOut = 0;
-                uint4          out0 = make_uint4( 0, 0, 0, 0 ), out1 = out0;
-                if ( p < P )
+                // row-major triangular index -> (a, c) with a < c
+                const float fm = float( 2 * m - 1 );
+                int         a  = int( ( fm - sqrtf( fm * fm - 8.f * float( p ) ) ) * 0.5f );
+                a              = max( 0, min( a, int( m ) - 2 ) );
+                while ( a > 0 && uint32_t( a ) * ( 2 * m - a - 1 ) / 2 > p )
+                    a--;
+                while ( uint32_t( a + 1 ) * ( 2 * m - a - 2 ) / 2 <= p )
+                    a++;
+                const uint32_t cc = p - uint32_t( a ) * ( 2 * m - a - 1 ) / 2 + a + 1;
+                const uint32_t la = sCand[ w ][ a ];
+                const uint32_t lk = sCand[ w ][ cc ];
+                const uint32_t j  = sIdx[ w ][ la ];
+                const uint32_t k  = sIdx[ w ][ lk ];
+                // k must be a neighbour of j (operations.inl:330-376): binary search in j's ascending list
+                const int jk = find_in_list( b.nbIds + size_t( j ) * K, b.nbCount[ j ], k );
+                if ( jk >= 0 )
                 {
-                    // row-major triangular index -> (a, c) with a < c
-                    const float fm = float( 2 * m - 1 );
-                    int         a  = int( ( fm - sqrtf( fm * fm - 8.f * float( p ) ) ) * 0.5f );
-                    a              = max( 0, min( a, int( m ) - 2 ) );
-                    while ( a > 0 && uint32_t( a ) * ( 2 * m - a - 1 ) / 2 > p )
-                        a--;
-                    while ( uint32_t( a + 1 ) * ( 2 * m - a - 2 ) / 2 <= p )
-                        a++;
-                    const uint32_t cc  = p - uint32_t( a ) * ( 2 * m - a - 1 ) / 2 + a + 1;
-                    const uint32_t la  = sCand[ w ][ a ];
-                    const uint32_t lk  = sCand[ w ][ cc ];
-                    const uint32_t j   = sIdx[ w ][ la ];
-                    const uint32_t k   = sIdx[ w ][ lk ];
-                    const int      jk  = find_in_list( b.nbIds + size_t( j ) * K, b.nbCount[ j ], k );
-                    if ( jk >= 0 )
+                    const float4    aj  = sAtom[ w ][ la ];
+                    const float4    ak  = sAtom[ w ][ lk ];
+                    const f3        dj  = xyz( aj ), dk = xyz( ak );
+                    const CircleGeo ij  = circle_geo( dj, Ri2, aj.w );
+                    const CircleGeo ik  = circle_geo( dk, Ri2, ak.w );
+                    const f3        u   = ik.normal - ij.normal * dot( ik.normal, ij.normal );
+                    const float     den = dot( u, ik.normal );
+                    const float     q   = dot( ik.centre - ij.centre, ik.normal );
+                    const f3        xc  = mk( ( u.x * q ) / den, ( u.y * q ) / den, ( u.z * q ) / den ) + ij.centre;
+                    const float     t   = Ri2 - len2( xc );
+                    f3              x[ 2 ];
+                    bool            ok  = false;
+                    if ( ill_conditioned( den, t, Ri2 ) )
                     {
-                        const float4    aj  = sAtom[ w ][ la ];
-                        const float4    ak  = sAtom[ w ][ lk ];
-                        const f3        dj  = xyz( aj ), dk = xyz( ak );
-                        const CircleGeo ij  = circle_geo( dj, Ri2, aj.w );
-                        const CircleGeo ik  = circle_geo( dk, Ri2, ak.w );
-                        const f3        u   = ik.normal - ij.normal * dot( ik.normal, ij.normal );
-                        const float     den = dot( u, ik.normal );
-                        const float     q   = dot( ik.centre - ij.centre, ik.normal );
-                        f3              xc  = mk( ( u.x * q ) / den, ( u.y * q ) / den, ( u.z * q ) / den ) + ij.centre;
-                        float           t   = Ri2 - len2( xc );
-                        f3              x[ 2 ];
-                        bool            ok  = false;
-                        if ( ill_conditioned( den, t, Ri2 ) )
+                        // FP64 re-solve of xc, t and the two probe positions
+                        fp64             = true;
+                        const double RiD = double( ai.w ) + double( rp );
+                        const double RjD = double( sRad[ w ][ la ] ) + double( rp );
+                        const double RkD = double( sRad[ w ][ lk ] ) + double( rp );
+                        const d3     Dj = mkd( dj ), Dk = mkd( dk );
+                        const double dj2 = dot( Dj, Dj ), dk2 = dot( Dk, Dk );
+                        const d3     cij = Dj * ( ( RiD * RiD - RjD * RjD + dj2 ) / ( 2.0 * dj2 ) );
+                        const d3     cik = Dk * ( ( RiD * RiD - RkD * RkD + dk2 ) / ( 2.0 * dk2 ) );
+                        const d3     nij = Dj * ( 1.0 / sqrt( dj2 ) ), nik = Dk * ( 1.0 / sqrt( dk2 ) );
+                        const d3     uu  = nik - nij * dot( nik, nij );
+                        const d3     xcD = cij + uu * ( dot( cik - cij, nik ) / dot( uu, nik ) );
+                        const double tD  = RiD * RiD - dot( xcD, xcD );
+                        if ( tD >= 0.0 )
                         {
-                            // FP64 re-solve of xc, t and the two probe positions
-                            atomicAdd( &b.ctl->fp64ResolveNb, 1u );
-                            const double RiD = double( ai.w ) + double( rp );
-                            const double RjD = double( sRad[ w ][ la ] ) + double( rp );
-                            const double RkD = double( sRad[ w ][ lk ] ) + double( rp );
-                            const d3     Dj = mkd( dj ), Dk = mkd( dk );
-                            const double dj2 = dot( Dj, Dj ), dk2 = dot( Dk, Dk );
-                            const d3     cij = Dj * ( ( RiD * RiD - RjD * RjD + dj2 ) / ( 2.0 * dj2 ) );
-                            const d3     cik = Dk * ( ( RiD * RiD - RkD * RkD + dk2 ) / ( 2.0 * dk2 ) );
-                            const d3     nij = Dj * ( 1.0 / sqrt( dj2 ) ), nik = Dk * ( 1.0 / sqrt( dk2 ) );
-                            const d3     uu  = nik - nij * dot( nik, nij );
-                            const d3     xcD = cij + uu * ( dot( cik - cij, nik ) / dot( uu, nik ) );
-                            const double tD  = RiD * RiD - dot( xcD, xcD );
-                            if ( tD >= 0.0 )
-                            {
-                                const d3 nn = unit( cross( nik, nij ) ) * sqrt( tD );
-                                const d3 x1 = xcD - nn, x2 = xcD + nn;
-                                x[ 0 ]      = mk( float( x1.x ), float( x1.y ), float( x1.z ) );
-                                x[ 1 ]      = mk( float( x2.x ), float( x2.y ), float( x2.z ) );
-                                ok          = true;
-                            }
-                        }
-                        else if ( t >= 0.f )
-                        {
-                            const f3 nn = unit( cross( ik.normal, ij.normal ) ) * sqrtf( t );
-                            x[ 0 ]      = xc - nn;
-                            x[ 1 ]      = xc + nn;
+                            const d3 nn = unit( cross( nik, nij ) ) * sqrt( tD );
+                            const d3 x1 = xcD - nn, x2 = xcD + nn;
+                            x[ 0 ]      = mk( float( x1.x ), float( x1.y ), float( x1.z ) );
+                            x[ 1 ]      = mk( float( x2.x ), float( x2.y ), float( x2.z ) );
                             ok          = true;
                         }
-                        if ( ok )
+                    }
+                    else if ( t >= 0.f )
+                    {
+                        const f3 nn = unit( cross( ik.normal, ij.normal ) ) * sqrtf( t );
+                        x[ 0 ]      = xc - nn;
+                        x[ 1 ]      = xc + nn;
+                        ok          = true;
+                    }
+                    if ( ok )
+                    {
+                        // visible iff outside every other SAS sphere of N(i) (operations.inl:415-430)
+                        bool vis0 = true, vis1 = true;
+                        for ( uint32_t l = 0; l < ni && ( vis0 || vis1 ); l++ )
                         {
-                            bool vis0 = true, vis1 = true;
-                            for ( uint32_t l = 0; l < ni && ( vis0 || vis1 ); l++ )
-                            {
-                                if ( l == la || l == lk )
-                                    continue;
-                                const float4 al = sAtom[ w ][ l ];
-                                const f3     cl = xyz( al );
-                                vis0            = vis0 && len2( cl - x[ 0 ] ) - al.w > 0.f;
-                                vis1            = vis1 && len2( cl - x[ 1 ] ) - al.w > 0.f;
-                            }
-                            const f3       pn     = unit( cross( ij.normal, unit( dj - dk ) ) );
-                            const uint32_t packed = la | ( lk << 10 ) | ( uint32_t( jk ) << 20 );
-                            for ( int s = 0; s < 2; s++ )
-                            {
-                                if ( !( s == 0 ? vis0 : vis1 ) )
-                                    continue;
-                                const float    d     = dot( -x[ s ], pn );
-                                const bool     hasNb = d * d - rp * rp < -PredicateEps;
-                                const uint4    rec   = make_uint4( i, j, k, packed | ( s ? RecMinus : 0u ) | ( hasNb ? RecHasNb : 0u ) );
-                                if ( nOut == 0 )
-                                    out0 = rec;
-                                else
-                                    out1 = rec;
-                                nOut++;
-                            }
+                            if ( l == la || l == lk )
+                                continue;
+                            const float4 al = sAtom[ w ][ l ];
+                            const f3     cl = xyz( al );
+                            vis0            = vis0 && len2( cl - x[ 0 ] ) - al.w > 0.f;
+                            vis1            = vis1 && len2( cl - x[ 1 ] ) - al.w > 0.f;
+                        }
+                        // hasNeighbors: the probe is closer than rp to the plane of the centres (:447-463)
+                        const f3       pn     = unit( cross( ij.normal, unit( dj - dk ) ) );
+                        const uint32_t packed = la | ( lk << 10 ) | ( uint32_t( jk ) << 20 );
+                        for ( int s = 0; s < 2; s++ )
+                        {
+                            if ( !( s == 0 ? vis0 : vis1 ) )
+                                continue;
+                            const float    d     = dot( -x[ s ], pn );
+                            const bool     hasNb = d * d - rp * rp < -PredicateEps;
+                            const uint32_t rec   = packed | ( s ? RecMinus : 0u ) | ( hasNb ? RecHasNb : 0u );
+                            if ( nOut == 0 )
+                                out0 = rec;
+                            else
+                                out1 = rec;
+                            nOut++;
                         }
                     }
                 }
-                const uint64_t has1 = ballot( nOut >= 1 ), has2 = ballot( nOut == 2 );
-                const uint32_t off  = nRec + rank_below( has1 ) + rank_below( has2 );
-                if ( nOut >= 1 && off < MaxAtomRecs )
-                    sRec[ w ][ off ] = out0;
-                if ( nOut == 2 && off + 1 < MaxAtomRecs )
-                    sRec[ w ][ off + 1 ] = out1;
-                nRec += uint32_t( __popcll( has1 ) + __popcll( has2 ) );
             }
-            if ( nRec > MaxAtomRecs )
-            {
-                if ( lane == 0 )
-                    set_overflow( b.ctl, OverflowAtomRecords );
-                nRec = MaxAtomRecs;
-            }
-            if ( lane == 0 && nRec )
-                atomicMax( &b.ctl->maxAtomRecords, nRec );
-            wave_sync();
-
-            // ---- bucket counts, ordered offsets, emission ----------------------------------------
-            uint32_t nW = 0;
-            for ( uint32_t rb = 0; rb < nRec; rb += WaveSize )
-            {
-                const uint32_t r = rb + lane;
-                nW += uint32_t( __popcll( ballot( r < nRec && ( sRec[ w ][ r ].w & RecHasNb ) ) ) );
-            }
-            const uint32_t local[ 2 ] = { nW, nRec - nW };
-            uint32_t       excl[ 2 ];
-            wave_lookback<2>( b.flags, b.flagStride, i, tag, local, excl );
+            const uint64_t has1 = ballot( nOut >= 1 ), has2 = ballot( nOut == 2 );
+            const uint32_t off  = nRec + rank_below( has1 ) + rank_below( has2 );
+            if ( nOut >= 1 && off < b.stageStride )
+                stage[ off ] = out0;
+            if ( nOut == 2 && off + 1 < b.stageStride )
+                stage[ off + 1 ] = out1;
+            nRec += uint32_t( __popcll( has1 ) + __popcll( has2 ) );
+            nW += uint32_t( __popcll( ballot( nOut >= 1 && ( out0 & RecHasNb ) ) ) + __popcll( ballot( nOut == 2 && ( out1 & RecHasNb ) ) ) );
+            nFp64 += uint32_t( __popcll( ballot( fp64 ) ) );
+        }
+        if ( nRec > b.stageStride )
+        {
+            // per-atom staging too small: counted, the host grows it and rebuilds
             if ( lane == 0 )
-            {
-                b.wStart[ i ] = excl[ 0 ];
-                b.nStart[ i ] = excl[ 1 ];
-                if ( i == b.n - 1 )
-                {
-                    b.wStart[ b.n ]              = excl[ 0 ] + nW;
-                    b.nStart[ b.n ]              = excl[ 1 ] + nRec - nW;
-                    b.ctl->intersectionWithNbNb = excl[ 0 ] + nW;
-                    b.ctl->intersectionNoNbNb   = excl[ 1 ] + nRec - nW;
-                }
-            }
-            uint32_t runW = excl[ 0 ], runN = excl[ 1 ];
-            for ( uint32_t rb = 0; rb < nRec; rb += WaveSize )
-            {
-                const uint32_t r    = rb + lane;
-                const bool     live = r < nRec;
-                const uint4    rec  = live ? sRec[ w ][ r ] : make_uint4( 0, 0, 0, 0 );
-                const bool     isW  = live && ( rec.w & RecHasNb );
-                const uint64_t mW = ballot( isW ), mN = ballot( live && !isW );
-                if ( live )
-                {
-                    const uint32_t idx = isW ? runW + rank_below( mW ) : runN + rank_below( mN );
-                    if ( idx < b.capRecords )
-                        b.records[ isW ? idx : b.capRecords - 1 - idx ] = rec;
-                    else
-                        set_overflow( b.ctl, OverflowIntersections );
-                    if ( MODE == 0 )
-                    {
-                        // per-circle intersection counts on (ij, jk, ik)
-                        const uint32_t la = rec.w & 1023u, lk = ( rec.w >> 10 ) & 1023u, jk = ( rec.w >> 20 ) & 1023u;
-                        const uint32_t ci = min( b.ctl->intersectedCircleNb, b.capTrimmed );
-                        const uint32_t t0 = trimmed_id( b, i, la ), t1 = trimmed_id( b, rec.y, jk ), t2 = trimmed_id( b, i, lk );
-                        if ( !( b.intMask[ size_t( rec.y ) * ( K / 32 ) + jk / 32 ] & ( 1u << ( jk % 32 ) ) ) )
-                            atomicAdd( &b.ctl->jkAnomalyNb, 1u );
-                        if ( t0 < ci )
-                            atomicAdd( b.circleIntNb + t0, 1u );
-                        if ( t1 < ci )
-                            atomicAdd( b.circleIntNb + t1, 1u );
-                        if ( t2 < ci )
-                            atomicAdd( b.circleIntNb + t2, 1u );
-                    }
-                }
-                runW += uint32_t( __popcll( mW ) );
-                runN += uint32_t( __popcll( mN ) );
-            }
+                set_overflow( b.ctl, OverflowAtomRecords );
+            nRec = nW = 0;
+        }
+        if ( lane == 0 )
+        {
+            b.atomCnt[ i ]                   = nW;
+            b.atomCnt[ b.n + i ]             = nRec - nW;
+            b.atomCnt[ 2 * size_t( b.n ) + i ] = nFp64;
         }
     }
 
-    // Exclusive scans of the per-circle intersection counts (list offsets) and of count/2
-    // (segment offsets) over the Ci intersected circles; clears the list cursors.
-    constexpr int CScanThreads = 256;
-    constexpr int CScanItems   = 4;
-    constexpr int CScanTile    = CScanThreads * CScanItems;
-
-    __global__ void __launch_bounds__( CScanThreads ) circle_scan_kernel( Buffers b )
+    // Place the staged records of atom i: with-neighbour ones at wStart[i] + rank (from the front),
+    // the others at nStart[i] + rank (from the back) -- the |w|-descending order of the reference's
+    // thrust::sort, made deterministic -- and count intersections per circle (complete surface).
+    template<int MODE>
+    __global__ void __launch_bounds__( InterWaves * WaveSize ) intersections_emit_kernel( Buffers b )
     {
-        __shared__ uint32_t waveSums[ 2 ][ CScanThreads / WaveSize ];
-        __shared__ uint32_t tileBase[ 2 ];
-        __shared__ uint32_t ticket;
-        const uint32_t      Ci    = min( b.ctl->intersectedCircleNb, b.capTrimmed );
-        const uint32_t      tiles = ( Ci + CScanTile - 1 ) / CScanTile;
-        const uint32_t      tag   = stage_tag( b.ctl, 4 );
-        if ( Ci == 0 )
-        {
-            if ( blockIdx.x == 0 && threadIdx.x == 0 )
-            {
-                b.circleStart[ 0 ] = 0;
-                b.segStart[ 0 ]    = 0;
-            }
+        const int      w    = threadIdx.x / WaveSize;
+        const int      lane = lane_id();
+        const uint32_t i    = blockIdx.x * InterWaves + w;
+        if ( i >= b.n )
             return;
-        }
-        for ( ;; )
+        const uint32_t nW   = b.wStart[ i + 1 ] - b.wStart[ i ];
+        const uint32_t nRec = nW + b.nStart[ i + 1 ] - b.nStart[ i ];
+        if ( nRec == 0 )
+            return;
+        const uint32_t   K     = b.K;
+        const uint32_t * stage = b.stage + size_t( i ) * b.stageStride;
+        const uint32_t   ci    = min( b.ctl->intersectedCircleNb, b.capTrimmed );
+        uint32_t         runW = b.wStart[ i ], runN = b.nStart[ i ];
+        for ( uint32_t rb = 0; rb < nRec; rb += WaveSize )
         {
-            if ( threadIdx.x == 0 )
-                ticket = atomicAdd( &b.ctl->tickets[ TicketCircleScan ], 1u );
-            __syncthreads();
-            const uint32_t t = ticket;
-            if ( t >= tiles )
-                break;
-            const uint32_t base = t * CScanTile + threadIdx.x * CScanItems;
-            uint32_t       v[ CScanItems ];
-            uint32_t       s0 = 0, s1 = 0;
-#pragma unroll
-            for ( int q = 0; q < CScanItems; q++ )
+            const uint32_t r    = rb + lane;
+            const bool     live = r < nRec;
+            const uint32_t rec  = live ? stage[ r ] : 0u;
+            const bool     isW  = live && ( rec & RecHasNb );
+            const uint64_t mW = ballot( isW ), mN = ballot( live && !isW );
+            if ( live )
             {
-                v[ q ] = base + q < Ci ? b.circleIntNb[ base + q ] : 0u;
-                s0 += v[ q ];
-                s1 += v[ q ] / 2;
-            }
-            const uint32_t i0 = wave_incl_scan( s0 ), i1 = wave_incl_scan( s1 );
-            const int      w  = threadIdx.x / WaveSize;
-            if ( lane_id() == WaveSize - 1 )
-            {
-                waveSums[ 0 ][ w ] = i0;
-                waveSums[ 1 ][ w ] = i1;
-            }
-            __syncthreads();
-            if ( w == 0 )
-            {
-                uint32_t t0 = 0, t1 = 0;
-                for ( int q = 0; q < CScanThreads / WaveSize; q++ )
-                    t0 += waveSums[ 0 ][ q ], t1 += waveSums[ 1 ][ q ];
-                const uint32_t local[ 2 ] = { t0, t1 };
-                uint32_t       excl[ 2 ];
-                wave_lookback<2>( b.flags, b.flagStride, t, tag, local, excl );
-                if ( lane_id() == 0 )
+                const uint32_t la = rec & 1023u, lk = ( rec >> 10 ) & 1023u, jk = ( rec >> 20 ) & 1023u;
+                const uint32_t j  = b.nbIds[ size_t( i ) * K + la ];
+                const uint32_t k  = b.nbIds[ size_t( i ) * K + lk ];
+                const uint32_t idx = isW ? runW + rank_below( mW ) : runN + rank_below( mN );
+                if ( idx < b.capRecords )
+                    b.records[ isW ? idx : b.capRecords - 1 - idx ] = make_uint4( i, j, k, rec );
+                else
+                    set_overflow( b.ctl, OverflowIntersections );
+                if ( MODE == 0 )
                 {
-                    tileBase[ 0 ] = excl[ 0 ];
-                    tileBase[ 1 ] = excl[ 1 ];
-                    if ( t == tiles - 1 )
-                    {
-                        b.circleStart[ Ci ]   = excl[ 0 ] + t0;
-                        b.segStart[ Ci ]      = excl[ 1 ] + t1;
-                        b.ctl->circleListNb   = excl[ 0 ] + t0;
-                        b.ctl->segmentNb      = excl[ 1 ] + t1;
-                    }
+                    // per-circle intersection counts on (ij, jk, ik) (operations.inl:476-482)
+                    const uint32_t t0 = trimmed_id( b, i, la ), t1 = trimmed_id( b, j, jk ), t2 = trimmed_id( b, i, lk );
+                    if ( !( b.intMask[ size_t( j ) * ( K / 32 ) + jk / 32 ] & ( 1u << ( jk % 32 ) ) ) )
+                        atomicAdd( &b.ctl->jkAnomalyNb, 1u ); // near-degenerate only
+                    if ( t0 < ci )
+                        atomicAdd( b.circleIntNb + t0, 1u );
+                    if ( t1 < ci )
+                        atomicAdd( b.circleIntNb + t1, 1u );
+                    if ( t2 < ci )
+                        atomicAdd( b.circleIntNb + t2, 1u );
                 }
             }
-            __syncthreads();
-            uint32_t r0 = tileBase[ 0 ] + i0 - s0, r1 = tileBase[ 1 ] + i1 - s1;
-            for ( int q = 0; q < w; q++ )
-                r0 += waveSums[ 0 ][ q ], r1 += waveSums[ 1 ][ q ];
-#pragma unroll
-            for ( int q = 0; q < CScanItems; q++ )
-                if ( base + q < Ci )
-                {
-                    b.circleStart[ base + q ]  = r0;
-                    b.segStart[ base + q ]     = r1;
-                    b.circleCursor[ base + q ] = 0u;
-                    r0 += v[ q ];
-                    r1 += v[ q ] / 2;
-                }
-            __syncthreads();
+            runW += uint32_t( __popcll( mW ) );
+            runN += uint32_t( __popcll( mN ) );
         }
     }
-
     // 8-bit compare-and-swap through the enclosing aligned 32-bit word.
     __device__ __forceinline__ void promote_status( uint8_t * status, size_t slot )
     {
@@ -641,76 +556,75 @@ namespace sesdf
         }
     }
 
-    template<int KMAX>
-    __global__ void __launch_bounds__( 256 ) torus_kernel( Buffers b )
+    // Exterior path: status-2 circles per atom (full, or promoted by a visible intersection).
+    __global__ void __launch_bounds__( 256 ) torus_count_kernel( Buffers b )
     {
+        const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t K    = b.K;
-        const float    rp   = b.probeRadius;
-        const uint32_t tag  = stage_tag( b.ctl, 5 );
-        for ( ;; )
+        const uint32_t i    = blockIdx.x * 4 + w;
+        if ( i >= b.n )
+            return;
+        const uint32_t ni  = b.nbCount[ i ];
+        uint32_t       cnt = 0;
+        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
         {
-            const uint32_t i = wave_ticket( &b.ctl->tickets[ TicketTorus ] );
-            if ( i >= b.n )
-                break;
-            const uint32_t ni  = b.nbCount[ i ];
-            uint32_t       cnt = 0;
-            for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+            const uint32_t l = lb + lane;
+            cnt += uint32_t( __popcll( ballot( l < ni && b.status[ size_t( i ) * b.K + l ] == 2 ) ) );
+        }
+        if ( lane == 0 )
+            b.atomCnt[ i ] = cnt;
+    }
+
+    // Torus records (MegaMol layout) of atom i's status-2 circles, in slot order.
+    __global__ void __launch_bounds__( 256 ) torus_emit_kernel( Buffers b )
+    {
+        const int      w    = threadIdx.x / WaveSize;
+        const int      lane = lane_id();
+        const uint32_t i    = blockIdx.x * 4 + w;
+        if ( i >= b.n )
+            return;
+        const uint32_t cnt = b.torusBase[ i + 1 ] - b.torusBase[ i ];
+        if ( cnt == 0 )
+            return;
+        const uint32_t K   = b.K;
+        const float    rp  = b.probeRadius;
+        const uint32_t ni  = b.nbCount[ i ];
+        const float4   ai  = b.atoms[ i ];
+        const f3       ci  = xyz( ai );
+        const float    Ri  = ai.w + rp;
+        uint32_t       run = b.torusBase[ i ];
+        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+        {
+            const uint32_t l    = lb + lane;
+            const bool     is2  = l < ni && b.status[ size_t( i ) * K + l ] == 2;
+            const uint64_t mask = ballot( is2 );
+            if ( is2 )
             {
-                const uint32_t l = lb + lane;
-                cnt += uint32_t( __popcll( ballot( l < ni && b.status[ size_t( i ) * K + l ] == 2 ) ) );
-            }
-            const uint32_t local[ 1 ] = { cnt };
-            uint32_t       excl[ 1 ];
-            wave_lookback<1>( b.flags, b.flagStride, i, tag, local, excl );
-            if ( lane == 0 )
-            {
-                b.torusBase[ i ] = excl[ 0 ];
-                if ( i == b.n - 1 )
+                const uint32_t dst = run + rank_below( mask );
+                if ( dst < b.capTorus )
                 {
-                    b.ctl->torusNb       = excl[ 0 ] + cnt;
-                    b.torusBase[ b.n ]   = excl[ 0 ] + cnt;
+                    const uint32_t  j     = b.nbIds[ size_t( i ) * K + l ];
+                    const float4    aj    = b.atoms[ j ];
+                    const f3        dj    = xyz( aj ) - ci;
+                    const float     Rj    = aj.w + rp;
+                    const CircleGeo geo   = circle_geo( dj, Ri * Ri, Rj * Rj );
+                    const float     rad   = sqrtf( geo.radius2 );
+                    const f3        ta    = unit( geo.centre );
+                    const f3        ortho = unit( cross( ta, mk( 0.f, 0.f, 1.f ) ) );
+                    const f3        P     = geo.centre + ortho * rad;
+                    const f3        X     = unit( P ) * ai.w;
+                    const float     lpi   = sqrtf( len2( P ) );
+                    const f3        C     = dj * ( lpi / ( sqrtf( len2( P - dj ) ) + lpi ) );
+                    const float     dist  = sqrtf( len2( X - C ) );
+                    const f3        Cout  = C - geo.centre;
+                    b.torusPos[ dst ]     = make_float4( ci.x + geo.centre.x, ci.y + geo.centre.y, ci.z + geo.centre.z, rad );
+                    b.torusAxis[ dst ]    = make_float4( ta.x, ta.y, ta.z, rp );
+                    b.torusVs[ dst ]      = make_float4( Cout.x, Cout.y, Cout.z, dist );
                 }
+                else
+                    set_overflow( b.ctl, OverflowTorus );
             }
-            if ( cnt == 0 )
-                continue;
-            const float4 ai  = b.atoms[ i ];
-            const f3     ci  = xyz( ai );
-            const float  Ri  = ai.w + rp;
-            uint32_t     run = excl[ 0 ];
-            for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
-            {
-                const uint32_t l    = lb + lane;
-                const bool     is2  = l < ni && b.status[ size_t( i ) * K + l ] == 2;
-                const uint64_t mask = ballot( is2 );
-                if ( is2 )
-                {
-                    const uint32_t dst = run + rank_below( mask );
-                    if ( dst < b.capTorus )
-                    {
-                        const uint32_t  j     = b.nbIds[ size_t( i ) * K + l ];
-                        const float4    aj    = b.atoms[ j ];
-                        const f3        dj    = xyz( aj ) - ci;
-                        const float     Rj    = aj.w + rp;
-                        const CircleGeo geo   = circle_geo( dj, Ri * Ri, Rj * Rj );
-                        const float     rad   = sqrtf( geo.radius2 );
-                        const f3        ta    = unit( geo.centre );
-                        const f3        ortho = unit( cross( ta, mk( 0.f, 0.f, 1.f ) ) );
-                        const f3        P     = geo.centre + ortho * rad;
-                        const f3        X     = unit( P ) * ai.w;
-                        const float     lpi   = sqrtf( len2( P ) );
-                        const f3        C     = dj * ( lpi / ( sqrtf( len2( P - dj ) ) + lpi ) );
-                        const float     dist  = sqrtf( len2( X - C ) );
-                        const f3        Cout  = C - geo.centre;
-                        b.torusPos[ dst ]     = make_float4( ci.x + geo.centre.x, ci.y + geo.centre.y, ci.z + geo.centre.z, rad );
-                        b.torusAxis[ dst ]    = make_float4( ta.x, ta.y, ta.z, rp );
-                        b.torusVs[ dst ]      = make_float4( Cout.x, Cout.y, Cout.z, dist );
-                    }
-                    else
-                        set_overflow( b.ctl, OverflowTorus );
-                }
-                run += uint32_t( __popcll( mask ) );
-            }
+            run += uint32_t( __popcll( mask ) );
         }
     }
 
@@ -726,29 +640,50 @@ namespace sesdf
     {
         if ( b.n == 0 )
             return;
-        const uint32_t blocks = persistent_blocks( uint64_t( b.n ) * WaveSize, InterWaves * WaveSize, 16 );
+        const uint32_t blocks = ( b.n + InterWaves - 1 ) / InterWaves;
         if ( b.K <= 128 )
-        {
-            if ( b.mode == 0 )
-                intersections_kernel<128, 0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
-            else
-                intersections_kernel<128, 1><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
-        }
+            intersections_count_kernel<128, 0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
         else
-        {
-            if ( b.mode == 0 )
-                intersections_kernel<320, 0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
-            else
-                intersections_kernel<320, 1><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
-        }
+            intersections_count_kernel<320, 0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+        ScanArgs a {};
+        a.n           = b.n;
+        a.ticket      = TicketIntersections;
+        a.stage       = 3;
+        a.in[ 0 ]     = b.atomCnt;
+        a.in[ 1 ]     = b.atomCnt + b.n;
+        a.in[ 2 ]     = b.atomCnt + 2 * size_t( b.n ); // FP64 re-solve counts (summed, not scanned)
+        a.out[ 0 ]    = b.wStart;
+        a.out[ 1 ]    = b.nStart;
+        a.totals[ 0 ] = &b.ctl->intersectionWithNbNb;
+        a.totals[ 1 ] = &b.ctl->intersectionNoNbNb;
+        a.maxOut      = &b.ctl->maxAtomRecords;
+        a.sumOut      = &b.ctl->fp64ResolveNb;
+        launch_scan<2>( b, a, s );
+        if ( b.mode == 0 )
+            intersections_emit_kernel<0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+        else
+            intersections_emit_kernel<1><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
     }
 
+    // Per-circle list offsets (exclusive scan of the intersection counts) and segment offsets
+    // (scan of count / 2) over the Ci intersected circles; clears the list cursors.
     void launch_circle_scan( const Buffers & b, hipStream_t s )
     {
         if ( b.n == 0 || b.mode != 0 )
             return;
-        const uint32_t blocks = persistent_blocks( b.capTrimmed, CScanTile, 4 );
-        circle_scan_kernel<<<blocks, CScanThreads, 0, s>>>( b );
+        ScanArgs a {};
+        a.n           = b.capTrimmed;
+        a.countPtr    = &b.ctl->intersectedCircleNb;
+        a.ticket      = TicketCircleScan;
+        a.stage       = 4;
+        a.in[ 0 ]     = b.circleIntNb;
+        a.halfSecond  = 1;
+        a.out[ 0 ]    = b.circleStart;
+        a.out[ 1 ]    = b.segStart;
+        a.totals[ 0 ] = &b.ctl->circleListNb;
+        a.totals[ 1 ] = &b.ctl->segmentNb;
+        a.clearAlso   = b.circleCursor;
+        launch_scan<2>( b, a, s );
     }
 
     void launch_fill( const Buffers & b, hipStream_t s )
@@ -785,7 +720,16 @@ namespace sesdf
     {
         if ( b.n == 0 || b.mode != 1 )
             return;
-        const uint32_t blocks = persistent_blocks( uint64_t( b.n ) * WaveSize, 256, 16 );
-        torus_kernel<128><<<blocks, 256, 0, s>>>( b );
+        const uint32_t blocks = ( b.n + 3 ) / 4;
+        torus_count_kernel<<<blocks, 256, 0, s>>>( b );
+        ScanArgs a {};
+        a.n           = b.n;
+        a.ticket      = TicketTorus;
+        a.stage       = 5;
+        a.in[ 0 ]     = b.atomCnt;
+        a.out[ 0 ]    = b.torusBase;
+        a.totals[ 0 ] = &b.ctl->torusNb;
+        launch_scan<1>( b, a, s );
+        torus_emit_kernel<<<blocks, 256, 0, s>>>( b );
     }
 } // namespace sesdf

@@ -29,10 +29,17 @@ namespace sesdf
         uint32_t * nbIds;              // N * K (sorted indices, ascending)
         uint8_t *  status;             // N * K (0 hidden, 1 intersected, 2 full)
         uint32_t * intMask;            // N * K/32: bit set for slots with status 1 and j > i
-        uint32_t * trimBase;           // N: first trimmed id of atom i
-        uint32_t * fullBase;           // N: first full circle of atom i
-        uint32_t * nStart;             // N + 1: first without-neighbour record of each atom
-        uint32_t * torusBase;          // N + 1: first torus record of each atom
+
+        // per-atom counts of the current stage (4 arrays of N) and their exclusive prefixes
+        uint32_t * atomCnt;            // 4 * N
+        uint32_t * vStart;             // N + 1: first sector of atom i
+        uint32_t * trimBase;           // N + 1: first intersected-circle (trimmed) id of atom i
+        uint32_t * fullBase;           // N + 1: first full circle of atom i
+        uint32_t * wStart;             // N + 1: first with-neighbour concave record of atom i
+        uint32_t * nStart;             // N + 1: first without-neighbour concave record of atom i
+        uint32_t * torusBase;          // N + 1: first torus record of atom i
+        uint32_t * stage;              // N * stageStride: packed concave records per atom
+        uint32_t   stageStride;
 
         // complete-surface outputs
         uint2 *    convexPatches;      // N: [start, end) sector range
@@ -48,7 +55,6 @@ namespace sesdf
 
         // concave patches
         uint4 *    records;            // capRecords: with-neighbour records grow up, others grow down
-        uint32_t * wStart;             // N + 1: first with-neighbour record of each atom
         int4 *     concaveIds;         // capRecords
         float4 *   concavePos;         // capRecords
         float4 *   concaveNb;          // capProbeNb * maxIntersectionNeighbors
@@ -58,7 +64,7 @@ namespace sesdf
         float4 * torusAxis;            // capTorus
         float4 * torusVs;              // capTorus
 
-        uint64_t * flags;              // look-back flags: 3 arrays of flagStride words
+        uint64_t * flags;              // look-back flags: 4 arrays of flagStride words
         uint32_t   flagStride;
 
         uint32_t n;                    // atoms
@@ -68,6 +74,26 @@ namespace sesdf
         int      mode;                 // 0 complete, 1 exterior path
         uint32_t capSectors, capFull, capTrimmed, capLists, capSegments, capRecords, capProbeNb, capTorus;
     };
+
+    // Exclusive scan of up to 4 per-item count arrays (tiles of 2048 items, decoupled look-back on
+    // one ticket per tile). out[c] receives n + 1 values (total last); totals[c], when set, too.
+    struct ScanArgs
+    {
+        uint32_t *       in[ 4 ];
+        uint32_t *       out[ 4 ];
+        uint32_t *       totals[ 4 ];
+        const uint32_t * countPtr;     // when set, n = min(*countPtr, n)
+        uint32_t         n;
+        uint32_t         ticket;       // Ticket slot
+        uint32_t         stage;        // look-back tag
+        int              clearInput;   // zero the inputs after reading (self-clearing counters)
+        int              halfSecond;   // out[1] scans in[0] / 2 instead of in[1]
+        uint32_t *       clearAlso;    // zeroed alongside (same length), may be null
+        uint32_t *       maxOut;       // max of in[0] over the items (atomicMax once per tile)
+        uint32_t *       sumOut;       // sum of in[3] (atomicAdd once per tile), scanned arrays < 3
+    };
+    template<int NC>
+    void launch_scan( const Buffers & b, const ScanArgs & a, hipStream_t s );
 
     // Host launch wrappers (one per stage); all are stream-ordered and capturable.
     void launch_begin( const Buffers & b, hipStream_t s );
