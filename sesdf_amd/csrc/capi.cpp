@@ -157,6 +157,7 @@ namespace sesdf
         ALLOC_TRY( h, b.torusBase, n + 1 );
         ALLOC_TRY( h, b.stage, n * h.stageStride );
         b.stageStride = h.stageStride;
+        ALLOC_TRY( h, b.touch, h.K <= 128 ? n * TouchStride : 1 );
         ALLOC_TRY( h, b.convexPatches, n );
         ALLOC_TRY( h, b.sectors, h.cap.sectors );
         ALLOC_TRY( h, b.fullCircles, h.cap.full );

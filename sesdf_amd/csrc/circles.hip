@@ -159,6 +159,7 @@ namespace sesdf
             const uint32_t l  = lb + lane;
             uint32_t       st = 0;
             uint32_t       j  = 0;
+            uint64_t       touch0 = 0, touch1 = 0; // spheres touching circle (i, j), by slot (K <= 128)
             if ( l < ni )
             {
                 j = sIdx[ w ][ l ];
@@ -184,13 +185,28 @@ namespace sesdf
                             break;
                         }
                         if ( A > 0.f || lhs > A2 )
+                        {
                             complete = 0;
+                            if ( k < 64 )
+                                touch0 |= uint64_t( 1 ) << k;
+                            else
+                                touch1 |= uint64_t( 1 ) << ( k - 64 );
+                        }
                     }
                     st = visible * ( 1 + complete );
                 }
                 b.status[ size_t( i ) * K + l ] = uint8_t( st );
             }
             const uint64_t mT = ballot( l < ni && st == 1 && j > i );
+            if ( l < ni && st == 1 && j > i )
+            {
+                // an occluder of a probe on this circle must touch it: keep the set for the
+                // intersection stage (candidate order = slot order of the intersected circles)
+                const uint32_t r = nT + rank_below( mT );
+                if ( r < TouchStride && K <= 128 )
+                    b.touch[ size_t( i ) * TouchStride + r ] =
+                        make_uint4( uint32_t( touch0 ), uint32_t( touch0 >> 32 ), uint32_t( touch1 ), uint32_t( touch1 >> 32 ) );
+            }
             nV += uint32_t( __popcll( ballot( l < ni && st != 0 ) ) );
             nT += uint32_t( __popcll( mT ) );
             nF += uint32_t( __popcll( ballot( l < ni && st == 2 && j > i ) ) );

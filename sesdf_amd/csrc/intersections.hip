@@ -110,6 +110,7 @@ namespace sesdf
         __shared__ float4   sAtom[ InterWaves ][ KMAX ];   // (c_l - c_i, R_l^2)
         __shared__ float    sRad[ InterWaves ][ KMAX ];    // vdW radius r_l
         __shared__ uint16_t sCand[ InterWaves ][ KMAX ];
+        __shared__ uint4    sTouch[ InterWaves ][ TouchStride ];
 
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
@@ -145,6 +146,9 @@ namespace sesdf
                 sCand[ w ][ m + rank_below( mask ) ] = uint16_t( l );
             m += uint32_t( __popcll( mask ) );
         }
+        const bool haveTouch = K <= 128;
+        if ( haveTouch && uint32_t( lane ) < min( m, TouchStride ) )
+            sTouch[ w ][ lane ] = b.touch[ size_t( i ) * TouchStride + lane ];
         wave_sync();
 
         // ---- candidate pairs (a < c over the intersected circles of i), flattened over the lanes ----
@@ -220,17 +224,46 @@ namespace sesdf
                     }
                     if ( ok )
                     {
-                        // visible iff outside every other SAS sphere of N(i) (operations.inl:415-430)
+                        // visible iff outside every other SAS sphere of N(i) (operations.inl:415-430).
+                        // A sphere containing a point of both circles touches both, so only the
+                        // spheres in touch(ij) & touch(ik) can occlude (exact pruning).
                         bool vis0 = true, vis1 = true;
-                        for ( uint32_t l = 0; l < ni && ( vis0 || vis1 ); l++ )
+                        if ( haveTouch && uint32_t( a ) < TouchStride && cc < TouchStride )
                         {
-                            if ( l == la || l == lk )
-                                continue;
-                            const float4 al = sAtom[ w ][ l ];
-                            const f3     cl = xyz( al );
-                            vis0            = vis0 && len2( cl - x[ 0 ] ) - al.w > 0.f;
-                            vis1            = vis1 && len2( cl - x[ 1 ] ) - al.w > 0.f;
+                            const uint4 ta = sTouch[ w ][ a ], tb = sTouch[ w ][ cc ];
+                            uint64_t    o0 = ( uint64_t( ta.y & tb.y ) << 32 ) | ( ta.x & tb.x );
+                            uint64_t    o1 = ( uint64_t( ta.w & tb.w ) << 32 ) | ( ta.z & tb.z );
+                            while ( ( o0 | o1 ) && ( vis0 || vis1 ) )
+                            {
+                                uint32_t l;
+                                if ( o0 )
+                                {
+                                    l = uint32_t( __ffsll( (unsigned long long)o0 ) - 1 );
+                                    o0 &= o0 - 1;
+                                }
+                                else
+                                {
+                                    l = 64u + uint32_t( __ffsll( (unsigned long long)o1 ) - 1 );
+                                    o1 &= o1 - 1;
+                                }
+                                if ( l == la || l == lk )
+                                    continue;
+                                const float4 al = sAtom[ w ][ l ];
+                                const f3     cl = xyz( al );
+                                vis0            = vis0 && len2( cl - x[ 0 ] ) - al.w > 0.f;
+                                vis1            = vis1 && len2( cl - x[ 1 ] ) - al.w > 0.f;
+                            }
                         }
+                        else
+                            for ( uint32_t l = 0; l < ni && ( vis0 || vis1 ); l++ )
+                            {
+                                if ( l == la || l == lk )
+                                    continue;
+                                const float4 al = sAtom[ w ][ l ];
+                                const f3     cl = xyz( al );
+                                vis0            = vis0 && len2( cl - x[ 0 ] ) - al.w > 0.f;
+                                vis1            = vis1 && len2( cl - x[ 1 ] ) - al.w > 0.f;
+                            }
                         // hasNeighbors: the probe is closer than rp to the plane of the centres (:447-463)
                         const f3       pn     = unit( cross( ij.normal, unit( dj - dk ) ) );
                         const uint32_t packed = la | ( lk << 10 ) | ( uint32_t( jk ) << 20 );

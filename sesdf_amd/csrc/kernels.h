@@ -5,6 +5,9 @@
 
 namespace sesdf
 {
+    // Intersected circles per atom whose touch masks are kept (more fall back to testing all of N(i)).
+    constexpr uint32_t TouchStride = 32;
+
     // Every device buffer of one build (a handle owns them in one arena). Passed by value to
     // every kernel; counts that only the device knows are read from `ctl`.
     struct Buffers
@@ -40,6 +43,8 @@ namespace sesdf
         uint32_t * torusBase;          // N + 1: first torus record of atom i
         uint32_t * stage;              // N * stageStride: packed concave records per atom
         uint32_t   stageStride;
+        uint4 *    touch;              // N * TouchStride: per intersected circle (i, j > i), in slot order,
+                                       // the bit set of neighbour slots whose SAS sphere touches the circle
 
         // complete-surface outputs
         uint2 *    convexPatches;      // N: [start, end) sector range
