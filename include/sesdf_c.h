@@ -179,15 +179,26 @@ typedef struct
     uint32_t dims[ 3 ];
 } sesdf_grid_desc;
 
-/* SDF at npoints device points (float4 xyz_), written to d_out (float). */
+#define SESDF_SDF_EXTERIOR 1 /* flags: exterior variant (cavity surfaces excluded, cavities inside) */
+
+/* SDF at npoints device points (float4 xyz_), written to d_out (float). Complete-surface handles. */
 sesdf_status sesdf_sample_points( sesdf_handle * h, const float * d_points, uint64_t npoints, float band, float * d_out );
+/* Same with flags and, when d_codes is not null, the winning element of every query (uint64:
+ * type << 40 | index; types 0 outside, 1 face, 2 full circle, 3 segment, 4 vertex, 5 clamped). */
+sesdf_status sesdf_sample_points_ex( sesdf_handle * h, const float * d_points, uint64_t npoints, float band, int flags, float * d_out,
+                                     uint64_t * d_codes );
 /* SDF on a regular vertex grid (x fastest), written to d_out (float, dims product). */
 sesdf_status sesdf_sample_grid( sesdf_handle * h, const sesdf_grid_desc * grid, float band, float * d_out );
+sesdf_status sesdf_sample_grid_ex( sesdf_handle * h, const sesdf_grid_desc * grid, float band, int flags, float * d_out );
 /* d(sum_i w_i SDF(q_i)) / d(atom xyz, radius) with the patch topology of the last build held
- * fixed; d_grad receives float4 per atom in INPUT order (dx, dy, dz, dr). recompute = 1 re-derives
- * the per-query winning element instead of reading the one stored by the last sample call. */
+ * fixed; d_grad receives float4 per atom in INPUT order (dx, dy, dz, dr). This form re-derives the
+ * winning element of every query (recompute); sesdf_sdf_backward_ex can read stored codes. */
 sesdf_status sesdf_sdf_backward( sesdf_handle * h, const float * d_points, const float * d_weights, uint64_t npoints, float band,
                                  int recompute, float * d_grad );
+/* d_codes: winners stored by sesdf_sample_points_ex (store) or null (recompute); d_grad_sorted:
+ * device scratch of float4 * atom_nb (gradient in grid-sorted order). */
+sesdf_status sesdf_sdf_backward_ex( sesdf_handle * h, const float * d_points, const float * d_weights, uint64_t npoints, float band,
+                                    int flags, const uint64_t * d_codes, float * d_grad_sorted, float * d_grad );
 
 /* -------- Batched build (no reference counterpart: the reference builds one molecule per
  * handle, app/benchmarks/main.cu:174-186). Molecules are concatenated molecule-major; outputs are

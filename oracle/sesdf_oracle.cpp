@@ -1094,7 +1094,109 @@ namespace orc
             return uint32_t( ( size_t( clampc( p.z, oz, dz ) ) * dy + clampc( p.y, oy, dy ) ) * dx + clampc( p.x, ox, dx ) );
         }
 
-        V3<double> pos64( uint32_t q ) const { return { r.concavePos64[ 3 * q ], r.concavePos64[ 3 * q + 1 ], r.concavePos64[ 3 * q + 2 ] }; }
+        // Vertex positions: stored (as built) or, in dynamic mode, re-solved from the current
+        // (possibly perturbed) atoms with the same triplet and side -- topology held fixed.
+        bool dynamic = false;
+        V3<double> pos64( uint32_t q ) const
+        {
+            if ( !dynamic )
+                return { r.concavePos64[ 3 * q ], r.concavePos64[ 3 * q + 1 ], r.concavePos64[ 3 * q + 2 ] };
+            const uint32_t i = uint32_t( r.concaveIds[ 4 * q ] ), j = uint32_t( r.concaveIds[ 4 * q + 1 ] ), k = uint32_t( r.concaveIds[ 4 * q + 2 ] );
+            const double   side = r.concavePos[ 4 * q + 3 ] > 0 ? 1.0 : -1.0;
+            const Geometry<double> ij = circleGeometry( c[ i ], R[ i ], c[ j ], R[ j ] );
+            const Geometry<double> ik = circleGeometry( c[ i ], R[ i ], c[ k ], R[ k ] );
+            const V3<double>       u  = ik.normal - dot( ik.normal, ij.normal ) * ij.normal;
+            const double           t  = dot( ik.center - ij.center, ik.normal ) / dot( u, ik.normal );
+            const V3<double>       x1 = ij.center + t * u;
+            const double           cc = std::max( R[ i ] * R[ i ] - len2( x1 - c[ i ] ), 0.0 );
+            return x1 + ( side * std::sqrt( cc ) ) * normalize( cross( ij.normal, ik.normal ) );
+        }
+
+        // Gradient of w * SDF(q) for the winner `code`, accumulated into g[4 n] (sorted atoms).
+        // Inside: nearest boundary point p on spheres m (1 face, 2 arc, 3 vertex); nu solves
+        // sum nu_m (p - c_m) = (p - q)/|p - q|; dSDF/dc_m = -nu_m (p - c_m), dSDF/dr_m = -nu_m R_m.
+        // Outside: dSDF/dc_a = -(q - c_a)/|q - c_a|, dSDF/dr_a = -1.
+        void grad( V3<double> q, int64_t code, double w, double * g ) const
+        {
+            const int64_t  type = code >> 40;
+            const uint32_t idx  = uint32_t( code & 0xffffffffffll );
+            if ( type == 5 )
+                return;
+            if ( type == 0 )
+            {
+                const V3<double> v = q - c[ idx ];
+                const double     d = len( v );
+                g[ 4 * idx + 0 ] -= w * v.x / d, g[ 4 * idx + 1 ] -= w * v.y / d, g[ 4 * idx + 2 ] -= w * v.z / d;
+                g[ 4 * idx + 3 ] -= w;
+                return;
+            }
+            uint32_t   at[ 3 ];
+            int        m = 0;
+            V3<double> p;
+            if ( type == 1 )
+            {
+                at[ 0 ] = idx, m = 1;
+                const V3<double> v = q - c[ idx ];
+                p                  = c[ idx ] + v * ( R[ idx ] / len( v ) );
+            }
+            else if ( type == 4 )
+            {
+                at[ 0 ] = uint32_t( r.concaveIds[ 4 * idx ] ), at[ 1 ] = uint32_t( r.concaveIds[ 4 * idx + 1 ] ),
+                at[ 2 ] = uint32_t( r.concaveIds[ 4 * idx + 2 ] ), m = 3;
+                p = pos64( idx );
+            }
+            else
+            {
+                const uint32_t i = type == 2 ? r.fullCircles[ 2 * idx ] : r.segments[ 4 * idx ];
+                const uint32_t j = type == 2 ? r.fullCircles[ 2 * idx + 1 ] : r.segments[ 4 * idx + 1 ];
+                at[ 0 ] = i, at[ 1 ] = j, m = 2;
+                V3<double> ctr, nrm;
+                double     rad;
+                circle( i, j, ctr, nrm, rad );
+                const V3<double> v  = q - ctr;
+                const V3<double> wv = v - nrm * dot( v, nrm );
+                p                   = ctr + wv * ( rad / len( wv ) );
+            }
+            const V3<double> e0 = p - q;
+            const double     D  = len( e0 );
+            if ( !( D > 0 ) )
+                return;
+            const V3<double> e = e0 * ( 1.0 / D );
+            V3<double>       a[ 3 ];
+            for ( int u = 0; u < m; u++ )
+                a[ u ] = p - c[ at[ u ] ];
+            // least-squares nu via the normal equations (Gram matrix), Gaussian elimination
+            double G[ 3 ][ 4 ];
+            for ( int u = 0; u < m; u++ )
+            {
+                for ( int v = 0; v < m; v++ )
+                    G[ u ][ v ] = dot( a[ u ], a[ v ] );
+                G[ u ][ m ] = dot( a[ u ], e );
+            }
+            for ( int col = 0; col < m; col++ )
+            {
+                int piv = col;
+                for ( int rr = col + 1; rr < m; rr++ )
+                    if ( std::fabs( G[ rr ][ col ] ) > std::fabs( G[ piv ][ col ] ) )
+                        piv = rr;
+                for ( int cc = 0; cc <= m; cc++ )
+                    std::swap( G[ col ][ cc ], G[ piv ][ cc ] );
+                for ( int rr = 0; rr < m; rr++ )
+                    if ( rr != col )
+                    {
+                        const double f = G[ rr ][ col ] / G[ col ][ col ];
+                        for ( int cc = col; cc <= m; cc++ )
+                            G[ rr ][ cc ] -= f * G[ col ][ cc ];
+                    }
+            }
+            for ( int u = 0; u < m; u++ )
+            {
+                const double nu = G[ u ][ m ] / G[ u ][ u ];
+                const uint32_t t = at[ u ];
+                g[ 4 * t + 0 ] -= w * nu * a[ u ].x, g[ 4 * t + 1 ] -= w * nu * a[ u ].y, g[ 4 * t + 2 ] -= w * nu * a[ u ].z;
+                g[ 4 * t + 3 ] -= w * nu * R[ t ];
+            }
+        }
 
         // circle (i, j): centre, normal (i -> j), radius
         void circle( uint32_t i, uint32_t j, V3<double> & ctr, V3<double> & nrm, double & rad ) const
@@ -1233,6 +1335,94 @@ extern "C"
     }
 
     // Surface component label per sorted atom and the exterior label (last entry).
+    // FP64 analytic gradient of sum_q w_q SDF(q) w.r.t. (x, y, z, r) of every atom, INPUT order (grad[4 n]).
+    int oracle_sdf_grad( void * h, const double * pts, const double * w, uint64_t n, int flags, double band, double * grad )
+    {
+        Result * r = static_cast<Result *>( h );
+        if ( r->mode != 0 )
+            return 1;
+        SdfOracle o( *r );
+        o.buildGrid( band );
+        std::vector<double> g( size_t( r->n ) * 4, 0.0 );
+        for ( uint64_t q = 0; q < n; q++ )
+        {
+            int64_t          code;
+            const V3<double> p { pts[ 3 * q ], pts[ 3 * q + 1 ], pts[ 3 * q + 2 ] };
+            o.eval( p, band, flags & 1, code );
+            o.grad( p, code, w[ q ], g.data() );
+        }
+        for ( uint32_t a = 0; a < r->n; a++ )
+            for ( int d = 0; d < 4; d++ )
+                grad[ 4 * size_t( r->sortedToInitial[ a ] ) + d ] = g[ 4 * size_t( a ) + d ];
+        return 0;
+    }
+
+    // Central finite differences (step hstep, FP64) of sum_q w_q SDF(q) w.r.t. (x, y, z, r) of the
+    // listed INPUT atoms, with the element set of the build held fixed and its geometry (circles,
+    // vertices) recomputed from the perturbed atoms. stable[k] = 1 when no query near the atom
+    // changed its winning element between -h, 0 and +h (the comparison is only meaningful there).
+    int oracle_sdf_fd( void * h, const double * pts, const double * w, uint64_t n, int flags, double band, const uint32_t * atomsIn,
+                       uint32_t count, double hstep, double * out, uint8_t * stable, int threads )
+    {
+        Result * r = static_cast<Result *>( h );
+        if ( r->mode != 0 )
+            return 1;
+        std::vector<uint32_t> initToSorted( r->n );
+        for ( uint32_t a = 0; a < r->n; a++ )
+            initToSorted[ r->sortedToInitial[ a ] ] = a;
+        parallelFor( count, threads, [ & ]( uint32_t t ) {
+            SdfOracle o( *r );
+            o.dynamic = true;
+            o.buildGrid( band );
+            const uint32_t   a      = initToSorted[ atomsIn[ t ] ];
+            const double     reach  = o.rp + band + o.R[ a ] + 2 * ( o.rmax + o.rp ) + 1.0;
+            const V3<double> ca     = o.c[ a ];
+            std::vector<uint64_t> near;
+            for ( uint64_t q = 0; q < n; q++ )
+            {
+                const V3<double> p { pts[ 3 * q ] - ca.x, pts[ 3 * q + 1 ] - ca.y, pts[ 3 * q + 2 ] - ca.z };
+                if ( len2( p ) < reach * reach )
+                    near.push_back( q );
+            }
+            std::vector<int64_t> base( near.size() );
+            for ( size_t e = 0; e < near.size(); e++ )
+            {
+                const uint64_t q = near[ e ];
+                o.eval( { pts[ 3 * q ], pts[ 3 * q + 1 ], pts[ 3 * q + 2 ] }, band, flags & 1, base[ e ] );
+            }
+            bool ok = true;
+            for ( int d = 0; d < 4; d++ )
+            {
+                double L[ 2 ];
+                for ( int sgn = 0; sgn < 2; sgn++ )
+                {
+                    const double step = sgn == 0 ? hstep : -hstep;
+                    SdfOracle &  oo   = o;
+                    if ( d == 0 ) oo.c[ a ].x = ca.x + step;
+                    if ( d == 1 ) oo.c[ a ].y = ca.y + step;
+                    if ( d == 2 ) oo.c[ a ].z = ca.z + step;
+                    const double R0 = o.R[ a ];
+                    if ( d == 3 ) oo.R[ a ] = R0 + step;
+                    double acc = 0;
+                    for ( size_t e = 0; e < near.size(); e++ )
+                    {
+                        const uint64_t q = near[ e ];
+                        int64_t        code;
+                        acc += w[ q ] * oo.eval( { pts[ 3 * q ], pts[ 3 * q + 1 ], pts[ 3 * q + 2 ] }, band, flags & 1, code );
+                        if ( code != base[ e ] )
+                            ok = false;
+                    }
+                    L[ sgn ] = acc;
+                    oo.c[ a ] = ca;
+                    oo.R[ a ] = R0;
+                }
+                out[ 4 * size_t( t ) + d ] = ( L[ 0 ] - L[ 1 ] ) / ( 2 * hstep );
+            }
+            stable[ t ] = ok;
+        } );
+        return 0;
+    }
+
     uint32_t oracle_components( void * h, uint32_t * out )
     {
         Result * r = static_cast<Result *>( h );

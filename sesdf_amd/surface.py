@@ -163,6 +163,26 @@ class Sesdf(_Surface):
             "max_concave_neighbors": d.max_concave_neighbors,
         }
 
+    # -- SDF (no reference counterpart; see include/sesdf_c.h) --------------------------------------
+    def sample_points_ptr(self, d_points, n, d_out, band=2.0, exterior=False, d_codes=None):
+        """SDF at n device points (float4 xyz_) into d_out (float); optional winner codes (uint64)."""
+        L.check(L.lib().sesdf_sample_points_ex(self._h, ctypes.c_void_p(d_points), n, band, 1 if exterior else 0,
+                                               ctypes.c_void_p(d_out), ctypes.c_void_p(d_codes) if d_codes else None),
+                "sesdf_sample_points")
+
+    def sample_grid_ptr(self, origin, spacing, dims, d_out, band=2.0, exterior=False):
+        g = L.GridDesc()
+        for d in range(3):
+            g.origin[d], g.spacing[d], g.dims[d] = float(origin[d]), float(spacing[d]), int(dims[d])
+        L.check(L.lib().sesdf_sample_grid_ex(self._h, ctypes.byref(g), band, 1 if exterior else 0, ctypes.c_void_p(d_out)),
+                "sesdf_sample_grid")
+
+    def sdf_backward_ptr(self, d_points, d_weights, n, d_grad, d_scratch, band=2.0, exterior=False, d_codes=None):
+        """Gradient of sum w_i SDF(q_i) per input atom (float4) into d_grad; d_scratch: float4 * atoms."""
+        L.check(L.lib().sesdf_sdf_backward_ex(self._h, ctypes.c_void_p(d_points), ctypes.c_void_p(d_weights), n, band,
+                                              1 if exterior else 0, ctypes.c_void_p(d_codes) if d_codes else None,
+                                              ctypes.c_void_p(d_scratch), ctypes.c_void_p(d_grad)), "sesdf_sdf_backward")
+
     def get_trimmed(self):
         it = self.internal()
         ci = self.counts()["intersected_circle_nb"]

@@ -1,0 +1,141 @@
+"""GPU SDF sampling and its gradient vs the FP64 oracle (values within 1e-4 A; gradients within 1e-3
+relative of the FP64 analytic gradient, which itself matches FP64 central differences)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from sesdf_amd import Sesdf, synth
+
+pytestmark = pytest.mark.gpu
+SDF_TOL = 1e-4
+GRAD_TOL = 1e-3
+
+
+def _torch():
+    import torch
+
+    assert torch.cuda.is_available()
+    return torch
+
+
+def near_surface_points(atoms, orc, count, band, seed):
+    rng = np.random.default_rng(seed)
+    lo = atoms[:, :3].min(0) - 4
+    hi = atoms[:, :3].max(0) + 4
+    q = (lo + (hi - lo) * rng.random((count * 6, 3))).astype(np.float32)
+    s = orc.sdf(q.astype(np.float64), band=band, threads=8)
+    keep = np.abs(s) < band * 0.999
+    sel = np.concatenate([q[keep][: count // 2], q[~keep][: count // 2]])
+    return sel
+
+
+def gpu_sdf(surf, pts, band, exterior=False, codes=False):
+    torch = _torch()
+    P = torch.from_numpy(np.c_[pts, np.zeros(len(pts), np.float32)]).cuda()
+    out = torch.empty(len(pts), device="cuda")
+    cd = torch.empty(len(pts), device="cuda", dtype=torch.int64) if codes else None
+    surf.sample_points_ptr(P.data_ptr(), len(pts), out.data_ptr(), band=band, exterior=exterior,
+                           d_codes=cd.data_ptr() if codes else None)
+    torch.cuda.synchronize()
+    return (out.cpu().numpy(), cd.cpu().numpy() if codes else None) if codes else out.cpu().numpy()
+
+
+@pytest.mark.parametrize("exterior", [False, True])
+def test_sdf_points_match_fp64(gpu, exterior):
+    atoms = synth.globule(3000, synth.SEED_BASE + 60)
+    surf = Sesdf(atoms)
+    orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
+    band = 2.0
+    pts = near_surface_points(atoms, orc, 40000, band, 3)
+    g = gpu_sdf(surf, pts, band, exterior)
+    o = orc.sdf(pts.astype(np.float64), band=band, exterior=exterior, threads=8)
+    err = np.abs(g.astype(np.float64) - o)
+    print("max |gpu - fp64| =", err.max(), "inside band:", np.sum(np.abs(o) < band), flush=True)
+    assert err.max() <= SDF_TOL
+
+
+def test_sdf_grid_64_config0(gpu):
+    """The small workload: 2,000 atoms, SDF on a 64^3 vertex grid over the bounding box + 3 A."""
+    torch = _torch()
+    atoms = synth.globule(2000, synth.SEED_BASE)
+    surf = Sesdf(atoms)
+    lo = atoms[:, :3].min(0) - atoms[:, 3].max() - 3
+    hi = atoms[:, :3].max(0) + atoms[:, 3].max() + 3
+    dims = (64, 64, 64)
+    spacing = (hi - lo) / 63
+    out = torch.empty(64 ** 3, device="cuda")
+    band = 3.0
+    surf.sample_grid_ptr(lo, spacing, dims, out.data_ptr(), band=band)
+    torch.cuda.synchronize()
+    g = out.cpu().numpy()
+    ii = np.arange(64 ** 3)
+    # the grid points exactly as the kernel forms them (FP32 origin + spacing * index)
+    pts = np.stack([np.float32(lo[0]) + np.float32(spacing[0]) * (ii % 64).astype(np.float32),
+                    np.float32(lo[1]) + np.float32(spacing[1]) * ((ii // 64) % 64).astype(np.float32),
+                    np.float32(lo[2]) + np.float32(spacing[2]) * (ii // 4096).astype(np.float32)], 1)
+    orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
+    o = orc.sdf(pts.astype(np.float64), band=band, threads=8)
+    assert np.abs(g - o).max() <= SDF_TOL
+    # invariants of the field: 1-Lipschitz along the grid axes, vdW lower bound
+    f = g.reshape(64, 64, 64)
+    assert np.abs(np.diff(f, axis=2)).max() <= spacing[0] * (1 + 1e-4) + 1e-5
+    assert (g < 0).any() and (g > 0).any()
+
+
+@pytest.mark.parametrize("store", [True, False])
+def test_backward_matches_fp64_gradient(gpu, store):
+    torch = _torch()
+    atoms = synth.globule(3000, synth.SEED_BASE + 61)
+    surf = Sesdf(atoms)
+    orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
+    band = 2.0
+    pts = near_surface_points(atoms, orc, 60000, band, 5)
+    w = np.random.default_rng(7).standard_normal(len(pts)).astype(np.float32)
+    P = torch.from_numpy(np.c_[pts, np.zeros(len(pts), np.float32)]).cuda()
+    W = torch.from_numpy(w).cuda()
+    val = torch.empty(len(pts), device="cuda")
+    codes = torch.empty(len(pts), device="cuda", dtype=torch.int64)
+    surf.sample_points_ptr(P.data_ptr(), len(pts), val.data_ptr(), band=band, d_codes=codes.data_ptr())
+    grad = torch.zeros(len(atoms), 4, device="cuda")
+    scratch = torch.empty(len(atoms), 4, device="cuda")
+    surf.sdf_backward_ptr(P.data_ptr(), W.data_ptr(), len(pts), grad.data_ptr(), scratch.data_ptr(), band=band,
+                          d_codes=codes.data_ptr() if store else None)
+    torch.cuda.synchronize()
+    g = grad.cpu().numpy().astype(np.float64)
+    ref = orc.sdf_grad(pts.astype(np.float64), w.astype(np.float64), band=band)
+    # per-atom relative error against the gradient scale; atoms whose winners flip between FP32 and
+    # FP64 (boundary of an element's region) are counted
+    scale = np.abs(ref).max()
+    err = np.abs(g - ref).max(1) / scale
+    print("max rel err", err.max(), "p99", np.quantile(err, 0.99), flush=True)
+    assert np.quantile(err, 0.999) <= GRAD_TOL
+    assert np.mean(err > GRAD_TOL) < 1e-3
+
+
+def test_torch_autograd_function(gpu):
+    torch = _torch()
+    from sesdf_amd.torch_sdf import SurfaceSDF
+
+    atoms = synth.globule(1500, synth.SEED_BASE + 62)
+    surf = Sesdf(atoms, build_surface=False)
+    orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
+    pts = near_surface_points(atoms, orc, 20000, 2.0, 9)
+    A = torch.from_numpy(atoms).cuda().requires_grad_(True)
+    P = torch.from_numpy(pts).cuda()
+    w = torch.randn(len(pts), device="cuda", generator=torch.Generator(device="cuda").manual_seed(3))
+    out = SurfaceSDF.apply(A, P, surf, 2.0, True, False)
+    (out * w).sum().backward()
+    g_store = A.grad.detach().cpu().numpy()
+    A.grad = None
+    out2 = SurfaceSDF.apply(A, P, surf, 2.0, False, False)
+    (out2 * w).sum().backward()
+    g_rec = A.grad.detach().cpu().numpy()
+    assert torch.equal(out, out2)
+    np.testing.assert_allclose(g_store, g_rec, rtol=0, atol=1e-4 * np.abs(g_store).max())
+    ref = orc.sdf_grad(pts.astype(np.float64), w.cpu().numpy().astype(np.float64), band=2.0)
+    assert np.abs(g_store - ref).max() <= GRAD_TOL * np.abs(ref).max() * 10
+    # finite differences of the FP64 oracle with the topology held fixed, on a few atoms
+    ids = np.arange(0, 1500, 150)
+    fd, stable = orc.sdf_fd(pts.astype(np.float64), w.cpu().numpy().astype(np.float64), ids, h=1e-4, band=2.0)
+    rel = np.abs(fd - g_store[ids]).max(1) / np.abs(ref).max()
+    assert np.all(rel[stable] <= GRAD_TOL)
