@@ -195,6 +195,9 @@ sesdf_status sesdf_sample_grid_ex( sesdf_handle * h, const sesdf_grid_desc * gri
  * winning element of every query (recompute); sesdf_sdf_backward_ex can read stored codes. */
 sesdf_status sesdf_sdf_backward( sesdf_handle * h, const float * d_points, const float * d_weights, uint64_t npoints, float band,
                                  int recompute, float * d_grad );
+/* Surface component label of every grid-sorted atom (union of atoms joined by exposed arcs) and
+ * the exterior label (component of the atom with the largest x + r), copied to host memory. */
+sesdf_status sesdf_surface_components( sesdf_handle * h, uint32_t * host_labels, uint32_t * exterior_label );
 /* d_codes: winners stored by sesdf_sample_points_ex (store) or null (recompute); d_grad_sorted:
  * device scratch of float4 * atom_nb (gradient in grid-sorted order). */
 sesdf_status sesdf_sdf_backward_ex( sesdf_handle * h, const float * d_points, const float * d_weights, uint64_t npoints, float band,

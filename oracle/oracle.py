@@ -123,7 +123,10 @@ class OracleResult:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().oracle_free(self._h)
+            try:
+                lib().oracle_free(self._h)
+            except (TypeError, AttributeError):  # interpreter shutdown
+                pass
             self._h = None
 
 

@@ -1208,7 +1208,7 @@ namespace orc
         double eval( V3<double> q, double band, bool exterior, int64_t & code ) const
         {
             const double D    = rp + band;
-            double       g    = 1e300;
+            double       g    = 1e300, gExt = 1e300;
             int64_t      gA   = -1;
             double       best = D;
             int64_t      bc   = int64_t( 5 ) << 40;
@@ -1226,6 +1226,8 @@ namespace orc
                             const double     sd   = dist - R[ a ];
                             if ( sd < g || ( sd == g && int64_t( a ) < gA ) )
                                 g = sd, gA = a;
+                            if ( exterior && comp[ a ] == extComp )
+                                gExt = std::min( gExt, sd );
                             if ( exterior && comp[ a ] != extComp )
                                 continue;
                             if ( std::fabs( sd ) < best && dist > 0 )
@@ -1291,7 +1293,8 @@ namespace orc
                 code = int64_t( 5 ) << 40;
                 return band;
             }
-            if ( g >= 0 && ( !exterior || comp[ gA ] == extComp ) )
+            // ties between components (within 1e-4 A) count as exterior
+            if ( g >= 0 && ( !exterior || gExt <= g + 1e-4 ) )
             {
                 f    = g;
                 code = gA;

@@ -98,7 +98,7 @@ namespace sesdf
         const int y0 = max( gy - reach, 0 ), y1 = min( gy + reach, M.dy - 1 );
         const int z0 = max( gz - reach, 0 ), z1 = min( gz + reach, M.dz - 1 );
 
-        float    g = 3.0e38f, best = D;
+        float    g = 3.0e38f, best = D, gExt = 3.0e38f;
         uint32_t gA = 0xffffffffu;
         uint64_t bc = code_of( SdfClamped, 0 );
         // pass 1: nearest sphere (outside distance), arcs and vertices
@@ -113,6 +113,8 @@ namespace sesdf
                     const float  sd = sqrtf( len2( q - xyz( aa ) ) ) - ( aa.w + rp );
                     if ( sd < g || ( sd == g && a < gA ) )
                         g = sd, gA = a;
+                    if ( s.exterior && s.comp[ a ] == extC )
+                        gExt = fminf( gExt, sd );
                     if ( sd > best || ( s.exterior && s.comp[ a ] != extC ) )
                         continue; // every element of a lies on sphere a
                     for ( uint32_t f = b.fullBase[ a ]; f < b.fullBase[ a + 1 ]; f++ )
@@ -195,7 +197,9 @@ namespace sesdf
             code = code_of( SdfClamped, 0 );
             return s.band;
         }
-        if ( g >= 0.f && ( !s.exterior || s.comp[ gA ] == extC ) )
+        // outside A: exterior unless the nearest sphere of the exterior component is farther than
+        // the nearest sphere by more than 1e-4 A (ties between components count as exterior)
+        if ( g >= 0.f && ( !s.exterior || gExt <= g + 1e-4f ) )
         {
             f    = g;
             code = code_of( SdfOutside, gA );
@@ -360,13 +364,19 @@ namespace sesdf
     }
 
     // ---- surface components (exterior variant): union-find over the arcs --------------------------
+    // parent[] is read with agent-scope atomic loads: other XCDs hook through atomics and a plain
+    // load could be served from this XCD's L2 with a stale value.
+    __device__ __forceinline__ uint32_t uf_load( uint32_t * parent, uint32_t a )
+    {
+        return __hip_atomic_load( parent + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
+    }
     __device__ __forceinline__ uint32_t uf_find( uint32_t * parent, uint32_t a )
     {
-        uint32_t p = parent[ a ];
+        uint32_t p = uf_load( parent, a );
         while ( p != a )
         {
             a = p;
-            p = parent[ a ];
+            p = uf_load( parent, a );
         }
         return a;
     }
@@ -431,20 +441,21 @@ namespace
     {
         if ( h.sdf.component && h.sdf.builtEpoch == h.hctl->epoch )
             return SESDF_OK;
-        const uint32_t n = h.n;
+        const uint32_t n    = h.n;
+        const size_t   nPad = ( size_t( n ) + 2 + 1 ) & ~size_t( 1 ); // 8-byte aligned slot for the argmax key
         if ( !h.sdf.component )
         {
             void * p = nullptr;
-            if ( hipMalloc( &p, size_t( n + 4 ) * 4 + 16 ) != hipSuccess )
+            if ( hipMalloc( &p, nPad * 4 + 16 ) != hipSuccess )
                 return SESDF_ERR_HIP;
             h.sdfAllocs.push_back( p );
-            h.sdfBytes += size_t( n + 4 ) * 4 + 16;
+            h.sdfBytes += nPad * 4 + 16;
             h.sdf.component = static_cast<uint32_t *>( p );
         }
         uint32_t *           parent  = h.sdf.component;
         uint32_t *           extComp = parent + n;
         uint32_t *           changed = parent + n + 1;
-        unsigned long long * best    = reinterpret_cast<unsigned long long *>( parent + n + 2 );
+        unsigned long long * best    = reinterpret_cast<unsigned long long *>( parent + nPad );
         hipStream_t          s       = h.stream;
         // small fill of the reduction slot with a kernel-free path: host copy
         const unsigned long long zero64 = 0;
@@ -518,6 +529,25 @@ extern "C"
             return st;
         sdf_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
+    }
+
+    // Surface component label of every sorted atom (host_labels[atom_nb]) and the label of the
+    // exterior component (component containing the atom with the largest x + r).
+    sesdf_status sesdf_surface_components( sesdf_handle * hh, uint32_t * host_labels, uint32_t * exterior_label )
+    {
+        if ( !hh || !host_labels || !exterior_label )
+            return SESDF_ERR_INVALID;
+        Handle & h = *reinterpret_cast<Handle *>( hh );
+        SdfArgs  s {};
+        s.band          = 1.f;
+        sesdf_status st = sdf_common( h, s, 1 );
+        if ( st != SESDF_OK )
+            return st;
+        if ( hipMemcpyAsync( host_labels, h.sdf.component, size_t( h.n ) * 4, hipMemcpyDeviceToHost, h.stream ) != hipSuccess ||
+             hipMemcpyAsync( exterior_label, h.sdf.component + h.n, 4, hipMemcpyDeviceToHost, h.stream ) != hipSuccess ||
+             hipStreamSynchronize( h.stream ) != hipSuccess )
+            return SESDF_ERR_HIP;
+        return SESDF_OK;
     }
 
     sesdf_status sesdf_sample_points( sesdf_handle * hh, const float * d_points, uint64_t npoints, float band, float * d_out )

@@ -124,7 +124,10 @@ class _Surface:
 
     def close(self):
         if getattr(self, "_h", None):
-            L.lib().sesdf_destroy(self._h)
+            try:
+                L.lib().sesdf_destroy(self._h)
+            except (TypeError, AttributeError):  # interpreter shutdown: the library is already gone
+                pass
             self._h = None
 
     def __del__(self):
@@ -182,6 +185,13 @@ class Sesdf(_Surface):
         L.check(L.lib().sesdf_sdf_backward_ex(self._h, ctypes.c_void_p(d_points), ctypes.c_void_p(d_weights), n, band,
                                               1 if exterior else 0, ctypes.c_void_p(d_codes) if d_codes else None,
                                               ctypes.c_void_p(d_scratch), ctypes.c_void_p(d_grad)), "sesdf_sdf_backward")
+
+    def surface_components(self):
+        """(labels per sorted atom, exterior label) of the surface components used by the exterior SDF."""
+        labels = np.empty(self.atom_nb, np.uint32)
+        ext = ctypes.c_uint32()
+        L.check(L.lib().sesdf_surface_components(self._h, labels.ctypes.data, ctypes.byref(ext)), "sesdf_surface_components")
+        return labels, int(ext.value)
 
     def get_trimmed(self):
         it = self.internal()

@@ -40,6 +40,30 @@ def gpu_sdf(surf, pts, band, exterior=False, codes=False):
     return (out.cpu().numpy(), cd.cpu().numpy() if codes else None) if codes else out.cpu().numpy()
 
 
+def test_surface_components_match_oracle(gpu):
+    atoms = synth.globule(3000, synth.SEED_BASE + 60)
+    surf = Sesdf(atoms)
+    orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
+    lg, eg = surf.surface_components()
+    lo, eo = orc.components()
+    s2i_g, s2i_o = surf.get_atom_indices(), orc["sorted_to_initial"]
+    ext_g = set(s2i_g[np.nonzero(lg == eg)[0]].tolist())
+    ext_o = set(s2i_o[np.nonzero(lo == eo)[0]].tolist())
+    print("exterior atoms gpu", len(ext_g), "oracle", len(ext_o), "components", len(set(lg.tolist())), len(set(lo.tolist())),
+          flush=True)
+    # same partition: map each input atom to the smallest input index of its component
+    def canon(labels, s2i):
+        first = {}
+        for a, l in zip(s2i.tolist(), labels.tolist()):
+            first[l] = min(first.get(l, a), a)
+        out = np.empty(len(labels), np.int64)
+        out[s2i] = [first[l] for l in labels.tolist()]
+        return out
+
+    assert np.array_equal(canon(lg, s2i_g), canon(lo, s2i_o))
+    assert ext_g == ext_o
+
+
 @pytest.mark.parametrize("exterior", [False, True])
 def test_sdf_points_match_fp64(gpu, exterior):
     atoms = synth.globule(3000, synth.SEED_BASE + 60)
