@@ -222,6 +222,20 @@ typedef struct
 } sesdf_batch_info;
 
 sesdf_status sesdf_batch_create( const sesdf_batch_info * info, sesdf_handle ** out );
+/* One dim^3 SDF vertex grid per molecule over its vdW bounding box grown by margin, written
+ * molecule-major to d_out (molecule_nb * dim^3 floats). Complete-surface batches. */
+sesdf_status sesdf_batch_sample_grids( sesdf_handle * h, uint32_t dim, float margin, float band, float * d_out );
+
+/* Launch tuning written by the autotuner (sesdf_amd/autotune.py): grid cell edge (A, 0 keeps the
+ * current one) and waves per workgroup of the circle and intersection kernels (2, 4 or 8; 0 keeps 4).
+ * A new cell size re-lays out the grid; the next build recaptures the graph. */
+typedef struct
+{
+    float    cell_size;
+    uint32_t circle_waves;
+    uint32_t intersection_waves;
+} sesdf_tuning;
+sesdf_status sesdf_set_tuning( sesdf_handle * h, const sesdf_tuning * t );
 /* Per-molecule ranges into the concatenated outputs: out_offsets[(molecule_nb + 1) * 7] holds,
  * for each of atoms, sectors, full circles, concave with neighbours, concave without neighbours,
  * segments, torus records, the molecule_nb + 1 prefix offsets (host memory). */

@@ -92,6 +92,8 @@ SIGNATURES = {
     "sesdf_sample_grid_ex": (i32, [vp, ctypes.POINTER(GridDesc), f32, ctypes.c_int, vp]),
     "sesdf_sdf_backward_ex": (i32, [vp, vp, vp, u64, f32, ctypes.c_int, vp, vp, vp]),
     "sesdf_surface_components": (i32, [vp, vp, vp]),
+    "sesdf_batch_sample_grids": (i32, [vp, u32, f32, f32, vp]),
+    "sesdf_set_tuning": (i32, [vp, ctypes.c_void_p]),
     "sesdf_batch_create": (i32, [ctypes.POINTER(BatchInfo), ctypes.POINTER(vp)]),
     "sesdf_batch_offsets": (i32, [vp, vp]),
     "sesdf_status_string": (ctypes.c_char_p, [i32]),

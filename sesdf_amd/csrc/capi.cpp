@@ -180,8 +180,10 @@ namespace sesdf
         b.flagStride         = uint32_t( tiles + 64 );
         ALLOC_TRY( h, b.flags, uint64_t( b.flagStride ) * 4, true );
 
-        b.n           = h.n;
-        b.K           = h.K;
+        b.n            = h.n;
+        b.wavesCircles = h.wavesCircles;
+        b.wavesInter   = h.wavesInter;
+        b.K            = h.K;
         b.maxProbeNb  = h.maxProbeNb;
         b.probeRadius = h.rp;
         b.mode        = h.mode;
@@ -513,6 +515,43 @@ extern "C"
         if ( h->ownStream && h->stream )
             hipStreamDestroy( h->stream );
         delete h;
+        return SESDF_OK;
+    }
+
+    sesdf_status sesdf_set_tuning( sesdf_handle * hh, const sesdf_tuning * t )
+    {
+        if ( !hh || !t )
+            return SESDF_ERR_INVALID;
+        Handle & h    = *reinterpret_cast<Handle *>( hh );
+        auto     okW  = []( uint32_t w ) { return w == 0 || w == 2 || w == 4 || w == 8; };
+        if ( !okW( t->circle_waves ) || !okW( t->intersection_waves ) || t->cell_size < 0.f )
+            return SESDF_ERR_INVALID;
+        HIP_TRY( hipSetDevice( h.device ) );
+        h.wavesCircles = t->circle_waves ? t->circle_waves : 4;
+        h.wavesInter   = t->intersection_waves ? t->intersection_waves : 4;
+        h.b.wavesCircles = h.wavesCircles;
+        h.b.wavesInter   = h.wavesInter;
+        if ( h.graphExec )
+        {
+            HIP_TRY( hipGraphExecDestroy( h.graphExec ) );
+            h.graphExec = nullptr;
+        }
+        if ( t->cell_size > 0.f && t->cell_size != h.cellSize )
+        {
+            // new grid bricks: re-lay out and reallocate, keeping the uploaded input
+            std::vector<float4> keep( h.n );
+            if ( h.n )
+                HIP_TRY( hipMemcpy( keep.data(), h.input, size_t( h.n ) * sizeof( float4 ), hipMemcpyDeviceToHost ) );
+            h.cellSize = t->cell_size;
+            layout_grid( h );
+            sesdf_status st = allocate( h );
+            if ( st != SESDF_OK )
+                return st;
+            if ( h.n )
+                HIP_TRY( hipMemcpy( h.input, keep.data(), size_t( h.n ) * sizeof( float4 ), hipMemcpyHostToDevice ) );
+            h.built = false;
+            h.freeSdf();
+        }
         return SESDF_OK;
     }
 

@@ -52,17 +52,17 @@ namespace sesdf
         return unit( cross( n, ref ) );
     }
 
-    template<int KMAX, int MODE>
-    __global__ void __launch_bounds__( CirclesWaves * WaveSize ) circles_count_kernel( Buffers b )
+    template<int KMAX, int MODE, int WAVES>
+    __global__ void __launch_bounds__( WAVES * WaveSize ) circles_count_kernel( Buffers b )
     {
-        __shared__ uint32_t sIdx[ CirclesWaves ][ KMAX ];
-        __shared__ float4   sAtom[ CirclesWaves ][ KMAX ];   // (c_k - c_i, Rk^2)
-        __shared__ uint32_t sRowStart[ CirclesWaves ][ WaveSize ];
-        __shared__ uint32_t sRowPrefix[ CirclesWaves ][ WaveSize + 1 ];
+        __shared__ uint32_t sIdx[ WAVES ][ KMAX ];
+        __shared__ float4   sAtom[ WAVES ][ KMAX ];   // (c_k - c_i, Rk^2)
+        __shared__ uint32_t sRowStart[ WAVES ][ WaveSize ];
+        __shared__ uint32_t sRowPrefix[ WAVES ][ WaveSize + 1 ];
 
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t i    = blockIdx.x * CirclesWaves + w;
+        const uint32_t i    = blockIdx.x * WAVES + w;
         if ( i >= b.n )
             return;
         const float    rp   = b.probeRadius;
@@ -315,25 +315,48 @@ namespace sesdf
         return cus;
     }
 
+    template<int KMAX, int MODE, int WAVES>
+    static void launch_count_shape( const Buffers & b, hipStream_t s )
+    {
+        circles_count_kernel<KMAX, MODE, WAVES><<<( b.n + WAVES - 1 ) / WAVES, WAVES * WaveSize, 0, s>>>( b );
+    }
+
+    template<int KMAX, int MODE>
+    static void launch_count_waves( const Buffers & b, hipStream_t s )
+    {
+        // block shape (waves per workgroup) chosen by the autotuner
+        if ( b.wavesCircles == 2 )
+            launch_count_shape<KMAX, MODE, 2>( b, s );
+        else if ( b.wavesCircles == 8 )
+            launch_count_shape<KMAX, MODE, 8>( b, s );
+        else
+            launch_count_shape<KMAX, MODE, 4>( b, s );
+    }
+
+    static void launch_circles_count( const Buffers & b, hipStream_t s )
+    {
+        if ( b.K <= 128 )
+        {
+            if ( b.mode == 0 )
+                launch_count_waves<128, 0>( b, s );
+            else
+                launch_count_waves<128, 1>( b, s );
+        }
+        else
+        {
+            if ( b.mode == 0 )
+                launch_count_waves<320, 0>( b, s );
+            else
+                launch_count_waves<320, 1>( b, s );
+        }
+    }
+
     void launch_circles( const Buffers & b, hipStream_t s )
     {
         if ( b.n == 0 )
             return;
         const uint32_t blocks = ( b.n + CirclesWaves - 1 ) / CirclesWaves;
-        if ( b.K <= 128 )
-        {
-            if ( b.mode == 0 )
-                circles_count_kernel<128, 0><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
-            else
-                circles_count_kernel<128, 1><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
-        }
-        else
-        {
-            if ( b.mode == 0 )
-                circles_count_kernel<320, 0><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
-            else
-                circles_count_kernel<320, 1><<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
-        }
+        launch_circles_count( b, s );
         ScanArgs a {};
         a.n      = b.n;
         a.ticket = TicketCircles;

@@ -26,6 +26,9 @@ namespace sesdf
         uint32_t * component   = nullptr;  // N: surface component label (exterior classification)
         uint32_t * winner      = nullptr;  // per stored query: winning element code
         uint64_t   winnerCap   = 0;
+        float4 *   molGrids    = nullptr;  // batches: per-molecule grid origin / spacing
+        uint32_t   molGridDim  = 0;
+        float      molGridMargin = 0.f;
     };
 
     struct Handle
@@ -45,6 +48,8 @@ namespace sesdf
         bool        inputValid = false;
         uint32_t    rebuilds  = 0;
         uint32_t    stageStride = 32;    // staged concave records per atom (grows on overflow)
+        uint32_t    wavesCircles = 4;    // block shapes (autotuned)
+        uint32_t    wavesInter   = 4;
 
         const float *         hostAtoms = nullptr;
         std::vector<Molecule> mols;

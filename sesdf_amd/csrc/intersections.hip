@@ -103,18 +103,18 @@ namespace sesdf
         return -1;
     }
 
-    template<int KMAX, int MODE>
-    __global__ void __launch_bounds__( InterWaves * WaveSize ) intersections_count_kernel( Buffers b )
+    template<int KMAX, int WAVES>
+    __global__ void __launch_bounds__( WAVES * WaveSize ) intersections_count_kernel( Buffers b )
     {
-        __shared__ uint32_t sIdx[ InterWaves ][ KMAX ];
-        __shared__ float4   sAtom[ InterWaves ][ KMAX ];   // (c_l - c_i, R_l^2)
-        __shared__ float    sRad[ InterWaves ][ KMAX ];    // vdW radius r_l
-        __shared__ uint16_t sCand[ InterWaves ][ KMAX ];
-        __shared__ uint4    sTouch[ InterWaves ][ TouchStride ];
+        __shared__ uint32_t sIdx[ WAVES ][ KMAX ];
+        __shared__ float4   sAtom[ WAVES ][ KMAX ];   // (c_l - c_i, R_l^2)
+        __shared__ float    sRad[ WAVES ][ KMAX ];    // vdW radius r_l
+        __shared__ uint16_t sCand[ WAVES ][ KMAX ];
+        __shared__ uint4    sTouch[ WAVES ][ TouchStride ];
 
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t i    = blockIdx.x * InterWaves + w;
+        const uint32_t i    = blockIdx.x * WAVES + w;
         if ( i >= b.n )
             return;
         const float    rp   = b.probeRadius;
@@ -674,10 +674,20 @@ namespace sesdf
         if ( b.n == 0 )
             return;
         const uint32_t blocks = ( b.n + InterWaves - 1 ) / InterWaves;
+        // block shape (waves per workgroup) chosen by the autotuner
+        const uint32_t wv = b.wavesInter == 2 || b.wavesInter == 8 ? b.wavesInter : 4u;
+        const uint32_t cb = ( b.n + wv - 1 ) / wv;
         if ( b.K <= 128 )
-            intersections_count_kernel<128, 0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+        {
+            if ( wv == 2 )
+                intersections_count_kernel<128, 2><<<cb, 2 * WaveSize, 0, s>>>( b );
+            else if ( wv == 8 )
+                intersections_count_kernel<128, 8><<<cb, 8 * WaveSize, 0, s>>>( b );
+            else
+                intersections_count_kernel<128, 4><<<cb, 4 * WaveSize, 0, s>>>( b );
+        }
         else
-            intersections_count_kernel<320, 0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+            intersections_count_kernel<320, 4><<<( b.n + 3 ) / 4, 4 * WaveSize, 0, s>>>( b );
         ScanArgs a {};
         a.n           = b.n;
         a.ticket      = TicketIntersections;

@@ -73,6 +73,8 @@ namespace sesdf
         uint32_t   flagStride;
 
         uint32_t n;                    // atoms
+        uint32_t wavesCircles;         // waves per workgroup of the circle kernel (2, 4, 8)
+        uint32_t wavesInter;           // waves per workgroup of the intersection kernel (2, 4, 8)
         uint32_t K;                    // neighbour capacity per atom (multiple of 32)
         uint32_t maxProbeNb;           // probe-neighbour capacity per concave patch (reference: 32)
         float    probeRadius;

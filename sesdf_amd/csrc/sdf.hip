@@ -50,14 +50,26 @@ namespace sesdf
         const float *    weights;    // backward
         int              recompute;  // backward: re-derive the winners instead of reading codes
         float4 *         gradSorted; // backward: per sorted atom (dx, dy, dz, dr)
+        const float4 *   molGrids;   // batches: per molecule (origin, 0), (spacing, 0); grid dx^3 each
     };
 
-    __device__ __forceinline__ f3 query_point( const SdfArgs & s, uint64_t q )
+    // Query q and its molecule (batches sample one dx*dy*dz grid per molecule, molecule-major).
+    __device__ __forceinline__ f3 query_point( const SdfArgs & s, uint64_t q, uint32_t & mol )
     {
+        mol = 0;
         if ( s.points )
             return xyz( s.points[ q ] );
+        const uint64_t per = uint64_t( s.dx ) * s.dy * s.dz;
+        float          ox = s.ox, oy = s.oy, oz = s.oz, sx = s.sx, sy = s.sy, sz = s.sz;
+        if ( s.molGrids )
+        {
+            mol              = uint32_t( q / per );
+            const float4 o   = s.molGrids[ 2 * mol ], sp = s.molGrids[ 2 * mol + 1 ];
+            ox = o.x, oy = o.y, oz = o.z, sx = sp.x, sy = sp.y, sz = sp.z;
+            q -= uint64_t( mol ) * per;
+        }
         const uint64_t ix = q % s.dx, iy = ( q / s.dx ) % s.dy, iz = q / ( uint64_t( s.dx ) * s.dy );
-        return mk( s.ox + s.sx * float( ix ), s.oy + s.sy * float( iy ), s.oz + s.sz * float( iz ) );
+        return mk( ox + sx * float( ix ), oy + sy * float( iy ), oz + sz * float( iz ) );
     }
 
     // circle (i, j) in world coordinates: centre, unit normal (i -> j), radius
@@ -81,12 +93,12 @@ namespace sesdf
     }
 
     // Evaluates the field at q; returns the value and the winner code.
-    __device__ float sdf_eval( const Buffers & b, const SdfArgs & s, f3 q, uint64_t & code )
+    __device__ float sdf_eval( const Buffers & b, const SdfArgs & s, f3 q, uint32_t mol, uint64_t & code )
     {
         const float    rp   = b.probeRadius;
         const float    rmax = __uint_as_float( b.ctl->radiusMaxBits );
         const float    D    = rp + s.band;
-        const Molecule M    = b.mols[ 0 ];
+        const Molecule M    = b.mols[ mol ];
         const uint32_t Iw   = b.ctl->intersectionWithNbNb;
         const uint32_t extC = s.exterior ? *s.extComp : 0u;
         const float    search = D + rmax + rp + 1e-3f;
@@ -223,7 +235,9 @@ namespace sesdf
         for ( uint64_t q = uint64_t( blockIdx.x ) * blockDim.x + threadIdx.x; q < s.n; q += uint64_t( gridDim.x ) * blockDim.x )
         {
             uint64_t    code;
-            const float v = sdf_eval( b, s, query_point( s, q ), code );
+            uint32_t    mol;
+            const f3    qp = query_point( s, q, mol );
+            const float v  = sdf_eval( b, s, qp, mol, code );
             s.out[ q ]    = v;
             if ( s.codes )
                 s.codes[ q ] = code;
@@ -285,12 +299,13 @@ namespace sesdf
             const float w = s.weights[ q ];
             if ( w == 0.f )
                 continue;
-            const f3 qf = query_point( s, q );
+            uint32_t mol;
+            const f3 qf = query_point( s, q, mol );
             uint64_t code;
             if ( !s.recompute )
                 code = s.codes[ q ]; // stored by the forward pass
             else
-                (void)sdf_eval( b, s, qf, code );
+                (void)sdf_eval( b, s, qf, mol, code );
             const uint32_t type = uint32_t( code >> 40 ), idx = uint32_t( code & 0xffffffffffull );
             if ( type == SdfClamped )
                 continue;
@@ -484,8 +499,9 @@ namespace
     {
         if ( !h.built )
             return SESDF_ERR_NOT_BUILT;
-        if ( h.mode != 0 || h.mols.size() != 1 || h.K > 320 )
-            return SESDF_ERR_UNSUPPORTED; // the field is defined on the complete surface of one molecule
+        // the field is defined on the complete surface; batches sample per-molecule grids only
+        if ( h.mode != 0 || h.K > 320 || ( h.mols.size() != 1 && ( !s.molGrids || ( flags & 1 ) ) ) )
+            return SESDF_ERR_UNSUPPORTED;
         if ( !( s.band > 0.f ) )
             return SESDF_ERR_INVALID;
         if ( hipSetDevice( h.device ) != hipSuccess )
@@ -569,6 +585,51 @@ extern "C"
         s.out  = d_out;
         sesdf_status st = sdf_common( h, s, flags );
         if ( st != SESDF_OK || s.n == 0 )
+            return st;
+        sdf_kernel<<<grid_blocks( s.n ), 256, 0, h.stream>>>( h.b, s );
+        return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
+    }
+
+    // Batches: one dim^3 vertex grid per molecule over its vdW bounding box grown by `margin`,
+    // molecule-major into d_out (molecule_nb * dim^3 floats).
+    sesdf_status sesdf_batch_sample_grids( sesdf_handle * hh, uint32_t dim, float margin, float band, float * d_out )
+    {
+        if ( !hh || !d_out || dim < 2 )
+            return SESDF_ERR_INVALID;
+        Handle &     h = *reinterpret_cast<Handle *>( hh );
+        const size_t M = h.mols.size();
+        if ( !h.sdf.molGrids || h.sdf.molGridDim != dim || h.sdf.molGridMargin != margin )
+        {
+            std::vector<float4> g( 2 * M );
+            for ( size_t m = 0; m < M; m++ )
+            {
+                const float * bx = h.boxes.data() + 6 * m;
+                g[ 2 * m ]       = make_float4( bx[ 0 ] - margin, bx[ 1 ] - margin, bx[ 2 ] - margin, 0.f );
+                g[ 2 * m + 1 ]   = make_float4( ( bx[ 3 ] - bx[ 0 ] + 2 * margin ) / float( dim - 1 ), ( bx[ 4 ] - bx[ 1 ] + 2 * margin ) / float( dim - 1 ),
+                                                ( bx[ 5 ] - bx[ 2 ] + 2 * margin ) / float( dim - 1 ), 0.f );
+            }
+            if ( !h.sdf.molGrids )
+            {
+                void * p = nullptr;
+                if ( hipMalloc( &p, 2 * M * sizeof( float4 ) ) != hipSuccess )
+                    return SESDF_ERR_HIP;
+                h.sdfAllocs.push_back( p );
+                h.sdfBytes += 2 * M * sizeof( float4 );
+                h.sdf.molGrids = static_cast<float4 *>( p );
+            }
+            if ( hipMemcpy( h.sdf.molGrids, g.data(), 2 * M * sizeof( float4 ), hipMemcpyHostToDevice ) != hipSuccess )
+                return SESDF_ERR_HIP;
+            h.sdf.molGridDim    = dim;
+            h.sdf.molGridMargin = margin;
+        }
+        SdfArgs s {};
+        s.dx = s.dy = s.dz = dim;
+        s.n        = uint64_t( dim ) * dim * dim * M;
+        s.band     = band;
+        s.out      = d_out;
+        s.molGrids = h.sdf.molGrids;
+        sesdf_status st = sdf_common( h, s, 0 );
+        if ( st != SESDF_OK )
             return st;
         sdf_kernel<<<grid_blocks( s.n ), 256, 0, h.stream>>>( h.b, s );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
