@@ -1,12 +1,19 @@
-"""SES build throughput on one MI355X (atoms/s), with roofline and host-CPU baseline.
+"""SES build throughput on one MI355X, with roofline and host-CPU baseline.
 
-Workload: a 100,000-atom globule (density 0.05 / A^3, C/N/O/S mix, min pair distance 1.2 A) with
-10 empty voids of radius U(4, 8) A, probe 1.4 A, complete surface. A step is one full build()
+Headline (value): a 100,000-atom globule (density 0.05 / A^3, C/N/O/S mix, min pair distance 1.2 A)
+with 10 empty voids of radius U(4, 8) A, probe 1.4 A, complete surface. A step is one full build()
 (grid, circles, intersections, segments, probe neighbours) on atoms already resident in HBM,
-replayed as one hipGraph. Also reported: the exterior path on the same atoms and the exterior path
-on 2.5M atoms (500 globules of 5,000 on a 360 A Fibonacci sphere).
+replayed as one hipGraph. Further legs (same JSON line):
+  exterior   the exterior path on the same atoms;
+  sdf        SDF of the complete surface on a 256^3 grid over the bounding box + 3 A;
+  large      exterior path on 2.5M atoms (500 globules of 5,000 on a 360 A Fibonacci sphere) + memory;
+  batch      4,096 molecules (N ~ logU[20, 5000]) in one launch sequence + 32^3 SDF grid each, vs
+             one launch sequence per molecule (timed on the first 512 molecules);
+  grad       20,000-atom globule, 1M queries within |SDF| <= 2 A, loss = sum w_i SDF(q_i), w ~ N(0, 1):
+             forward (build + sampling) vs backward (stored / recomputed winners);
+  cpu        host C++ build of the same stage sequence (the FP32 oracle restatement), box's cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs main,exterior,...]
 """
 import argparse
 import json
@@ -20,6 +27,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+STAGES = ["grid", "circles_kernel", "circles_offsets_emit", "intersections_kernel", "intersections_offsets_emit",
+          "circle_list_scan", "fill", "segments", "probe_neighbors", "torus"]
+ALL_LEGS = "main,exterior,sdf,large,batch,grad,cpu"
 
 
 def parse():
@@ -27,18 +37,24 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
-    p.add_argument("--no-cpu", action="store_true", help="skip the host-CPU baseline leg")
-    p.add_argument("--no-large", action="store_true", help="skip the 2.5M-atom leg")
+    p.add_argument("--legs", default=ALL_LEGS)
     p.add_argument("--cell", type=float, default=0.0)
+    p.add_argument("--calibrate-copy-mib", type=int, default=0,
+                   help="run one device copy of this size first (counter calibration under rocprofv3)")
     return p.parse_args()
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def time_builds(surf, steps, warmup):
-    """Time `steps` graph replays of the whole stage sequence with hipEvents on the build stream."""
+    """Mean device time (ms) of `steps` graph replays of the whole stage sequence, hipEvents on the build stream."""
     import sesdf_amd._lib as L
 
     lib = L.lib()
     stream = surf.stream()
+    L.check(lib.sesdf_build(surf._h), "build")
     for _ in range(warmup):
         L.check(lib.sesdf_enqueue(surf._h), "enqueue")
     L.check(lib.sesdf_sync(surf._h), "sync")
@@ -55,27 +71,231 @@ def time_builds(surf, steps, warmup):
 
 
 def stage_times(surf, reps=10):
-    """Per-stage device times (ms) from event pairs around each stage (direct launches)."""
+    """Per-stage device times (ms): event pairs around each stage of a directly launched build."""
     import ctypes
 
     import sesdf_amd._lib as L
 
-    acc = np.zeros(8)
-    buf = (ctypes.c_float * 8)()
+    acc = np.zeros(len(STAGES))
+    buf = (ctypes.c_float * len(STAGES))()
     for _ in range(reps):
-        L.check(L.lib().sesdf_build_timed(surf._h, buf, 8), "build_timed")
+        L.check(L.lib().sesdf_build_timed(surf._h, buf, len(STAGES)), "build_timed")
         acc += np.array(buf[:])
-    return acc / reps
+    return dict(zip(STAGES, (acc / reps).tolist()))
 
 
-def circles_alg_bytes(c, n):
-    """Algorithmic HBM bytes of the fused circle kernel: atom read + one gather per atom (32 N),
-    neighbour ids + statuses (5 B per ordered circle), sectors (16 V), per-atom ranges (8 N),
-    intersected ids (4 Ci), full circles (8 Cf)."""
-    return 32 * n + 5 * c["neighbor_nb"] + 16 * c["sector_nb"] + 8 * n + 4 * c["intersected_circle_nb"] + 8 * c["full_circle_nb"]
+def intersections_alg_bytes(c, n):
+    """Algorithmic HBM bytes of one launch of the intersection kernel: atom i + one gather of its
+    neighbours (32 N), neighbour ids + statuses (5 B per ordered circle), the touch masks of the
+    intersected circles (16 Ci), staged records (4 I) and per-atom counts (12 N)."""
+    return 32 * n + 5 * c["neighbor_nb"] + 16 * c["intersected_circle_nb"] + 4 * c["concave_patch_nb"] + 12 * n
 
 
-def cpu_baseline(atoms, budget_s=20.0):
+def leg_main(args, atoms, out):
+    from sesdf_amd import Sesdf
+
+    n = atoms.shape[0]
+    surf = Sesdf(atoms, probe_radius=1.4, cell_size=args.cell, use_graph=True)
+    ms = time_builds(surf, args.steps, args.warmup)
+    counts = surf.counts()
+    stages = stage_times(surf)
+    kms = stages["intersections_kernel"]
+    alg = intersections_alg_bytes(counts, n)
+    achieved = alg / (kms * 1e-3) / 1e9
+    out.update({
+        "value": n / (ms * 1e-3), "ms_per_step": ms, "stage_ms": stages,
+        "counts": {k: int(v) for k, v in counts.items()},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "intersections_count_kernel",
+                     "alg_bytes_per_launch": int(alg), "kernel_ms": kms},
+    })
+    log(f"main: {ms:.3f} ms/build, {n / (ms * 1e-3) / 1e6:.1f} M atoms/s")
+    return surf
+
+
+def leg_exterior(args, atoms, out):
+    from sesdf_amd import Ssesdf
+
+    ext = Ssesdf(atoms, probe_radius=1.4, cell_size=args.cell, use_graph=True)
+    ms = time_builds(ext, args.steps, args.warmup)
+    out["exterior"] = {"atoms_per_s": atoms.shape[0] / (ms * 1e-3), "ms_per_build": ms,
+                       "torus": int(ext.counts()["torus_nb"]), "concave": int(ext.counts()["concave_patch_nb"])}
+    log(f"exterior: {ms:.3f} ms/build")
+
+
+def leg_sdf(args, atoms, surf, out):
+    import torch
+
+    import sesdf_amd._lib as L
+
+    lo = atoms[:, :3].min(0) - atoms[:, 3].max() - 3
+    hi = atoms[:, :3].max(0) + atoms[:, 3].max() + 3
+    dim = 256
+    sp = ((hi - lo) / (dim - 1)).astype(np.float32)
+    res = torch.empty(dim ** 3, device="cuda")
+    stream = surf.stream()
+    surf.sample_grid_ptr(lo, sp, (dim,) * 3, res.data_ptr(), band=2.0)  # warm-up
+    L.device_synchronize()
+    e0, e1 = L.Event(), L.Event()
+    reps = 5
+    e0.record(stream)
+    for _ in range(reps):
+        surf.sample_grid_ptr(lo, sp, (dim,) * 3, res.data_ptr(), band=2.0)
+    e1.record(stream)
+    e1.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    r = res.cpu().numpy()
+    out["sdf"] = {"grid": "256^3", "band_A": 2.0, "ms": ms, "samples_per_s": dim ** 3 / (ms * 1e-3),
+                  "inside_fraction": float((r < 0).mean()), "surface_band_fraction": float((np.abs(r) < 2).mean())}
+    log(f"sdf 256^3: {ms:.3f} ms")
+
+
+def leg_large(args, out):
+    from sesdf_amd import Ssesdf, synth
+    import sesdf_amd._lib as L
+
+    big = synth.fibonacci_globules(500, 5000, 360.0, synth.SEED_BASE + 2)
+    L.device_synchronize()
+    free0, _ = L.mem_info()
+    sbig = Ssesdf(big, probe_radius=1.4, cell_size=args.cell, use_graph=True)
+    ms = time_builds(sbig, max(5, args.steps // 5), 2)
+    free1, _ = L.mem_info()
+    cb = sbig.counts()
+    out["large_2p5M_exterior"] = {
+        "atoms": int(big.shape[0]), "ms_per_build": ms, "atoms_per_s": big.shape[0] / (ms * 1e-3),
+        "concave": int(cb["concave_patch_nb"]), "torus": int(cb["torus_nb"]),
+        "handle_device_bytes": int(cb["device_bytes"]), "device_mem_delta_bytes": int(free0 - free1),
+        "stage_ms": stage_times(sbig, 3)}
+    log(f"large: {ms:.3f} ms/build")
+    sbig.close()
+
+
+def leg_batch(args, out):
+    import torch
+
+    import sesdf_amd._lib as L
+    from sesdf_amd import Sesdf, synth
+    from sesdf_amd.autotune import autotune
+    from sesdf_amd.batch import SesdfBatch
+
+    sizes = synth.batch_sizes(4096, 20, 5000, synth.SEED_BASE + 3)
+    atoms, offs = synth.batch(sizes, synth.SEED_BASE + 3)
+    M, dim, margin, band = len(sizes), 32, 3.0, 2.0
+    batch = SesdfBatch(atoms, offs, build_surface=False)
+    tune = autotune(batch, log=log)
+    ms_build = time_builds(batch, max(5, args.steps // 5), 2)
+    grids = torch.empty(M * dim ** 3, device="cuda")
+    stream = batch.stream()
+    batch.sample_grids_ptr(dim, grids.data_ptr(), margin=margin, band=band)
+    L.device_synchronize()
+    e0, e1 = L.Event(), L.Event()
+    e0.record(stream)
+    L.check(L.lib().sesdf_enqueue(batch._h), "enqueue")
+    batch.sample_grids_ptr(dim, grids.data_ptr(), margin=margin, band=band)
+    e1.record(stream)
+    e1.sync()
+    L.check(L.lib().sesdf_sync(batch._h), "sync")
+    ms_batched = e0.elapsed_ms(e1)
+    # one launch sequence per molecule (own handle, own graph), same work, first P molecules
+    P = 512
+    singles = []
+    for m in range(P):
+        s = Sesdf(atoms[int(offs[m]):int(offs[m + 1])], use_graph=True)
+        singles.append(s)
+    outs = torch.empty(P * dim ** 3, device="cuda")
+    descs = []
+    for m in range(P):
+        mol = atoms[int(offs[m]):int(offs[m + 1])]
+        lo = (mol[:, :3] - mol[:, 3:4]).min(0) - margin
+        hi = (mol[:, :3] + mol[:, 3:4]).max(0) + margin
+        descs.append((lo.astype(np.float32), ((hi - lo) / (dim - 1)).astype(np.float32)))
+    L.device_synchronize()
+    t0 = time.perf_counter()
+    for m, s in enumerate(singles):
+        L.check(L.lib().sesdf_enqueue(s._h), "enqueue")
+        s.sample_grid_ptr(descs[m][0], descs[m][1], (dim,) * 3, outs.data_ptr() + 4 * m * dim ** 3, band=band)
+    L.device_synchronize()
+    ms_single = (time.perf_counter() - t0) * 1e3
+    # same accuracy: the batch's grids equal the standalone grids
+    same = bool(torch.allclose(grids[:P * dim ** 3], outs, atol=1e-5))
+    batched_rate = M / (ms_batched * 1e-3)
+    single_rate = P / (ms_single * 1e-3)
+    out["batch"] = {"molecules": M, "atoms": int(atoms.shape[0]), "sdf_grid": f"{dim}^3",
+                    "ms_batched_build_plus_sdf": ms_batched, "ms_batched_build": ms_build,
+                    "molecules_per_s_batched": batched_rate,
+                    "per_molecule_launch_molecules": P, "ms_per_molecule_launches": ms_single,
+                    "molecules_per_s_per_launch": single_rate, "speedup": batched_rate / single_rate,
+                    "grids_equal_standalone": same,
+                    "tuning": {k: tune[k] for k in ("cell_size", "circle_waves", "intersection_waves")}}
+    log(f"batch: {batched_rate:.0f} mol/s batched vs {single_rate:.0f} per launch")
+    for s in singles:
+        s.close()
+    batch.close()
+
+
+def leg_grad(args, out):
+    import torch
+
+    import sesdf_amd._lib as L
+    from sesdf_amd import Sesdf, synth
+
+    atoms = synth.globule(20000, synth.SEED_BASE + 4)
+    surf = Sesdf(atoms)
+    band = 2.0
+    g = torch.Generator(device="cuda").manual_seed(int(synth.SEED_BASE + 4))
+    lo = torch.tensor(atoms[:, :3].min(0) - 4, device="cuda")
+    hi = torch.tensor(atoms[:, :3].max(0) + 4, device="cuda")
+    keep = []
+    total = 0
+    while total < 1_000_000:
+        cand = torch.zeros(2_000_000, 4, device="cuda")
+        cand[:, :3] = lo + (hi - lo) * torch.rand(2_000_000, 3, device="cuda", generator=g)
+        s = torch.empty(cand.shape[0], device="cuda")
+        surf.sample_points_ptr(cand.data_ptr(), cand.shape[0], s.data_ptr(), band=band)
+        L.device_synchronize()
+        sel = cand[s.abs() < band]
+        keep.append(sel)
+        total += sel.shape[0]
+    Q = torch.cat(keep)[:1_000_000].contiguous()
+    nq = Q.shape[0]
+    w = torch.randn(nq, device="cuda", generator=g)
+    d_atoms = torch.from_numpy(atoms).cuda()
+    val = torch.empty(nq, device="cuda")
+    codes = torch.empty(nq, device="cuda", dtype=torch.int64)
+    grad = torch.zeros(len(atoms), 4, device="cuda")
+    scratch = torch.empty(len(atoms), 4, device="cuda")
+    stream = surf.stream()
+
+    def fwd():
+        surf.build_device(d_atoms.data_ptr())
+        surf.sample_points_ptr(Q.data_ptr(), nq, val.data_ptr(), band=band, d_codes=codes.data_ptr())
+
+    def bwd(store):
+        surf.sdf_backward_ptr(Q.data_ptr(), w.data_ptr(), nq, grad.data_ptr(), scratch.data_ptr(), band=band,
+                              d_codes=codes.data_ptr() if store else None)
+
+    def timed(fn, reps=5):
+        fn()
+        L.device_synchronize()
+        e0, e1 = L.Event(), L.Event()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.sync()
+        return e0.elapsed_ms(e1) / reps
+
+    t_f = timed(fwd)
+    t_bs = timed(lambda: bwd(True))
+    t_br = timed(lambda: bwd(False))
+    out["grad"] = {"atoms": len(atoms), "queries": nq, "band_A": band, "ms_forward_build_plus_sdf": t_f,
+                   "ms_backward_stored": t_bs, "ms_backward_recompute": t_br,
+                   "bwd_over_fwd_stored": t_bs / t_f, "bwd_over_fwd_recompute": t_br / t_f,
+                   "stored_winner_bytes": nq * 8}
+    log(f"grad: fwd {t_f:.3f} ms, bwd stored {t_bs:.3f} ms, recompute {t_br:.3f} ms")
+
+
+def leg_cpu(atoms, budget_s=20.0):
     """Host C++ build of the same stage sequence (the FP32 oracle restatement) on this box's cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -90,7 +310,8 @@ def cpu_baseline(atoms, budget_s=20.0):
         times.append(time.time() - t)
     med = float(np.median(times))
     return {"value": atoms.shape[0] / med, "unit": "atoms/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} full complete-surface builds of the {atoms.shape[0]}-atom workload, FP32, median {med:.3f} s"}
+            "sample": f"{len(times)} full complete-surface builds of the {atoms.shape[0]}-atom workload, FP32, "
+                      f"{threads} threads, median {med:.3f} s"}
 
 
 def main():
@@ -98,71 +319,40 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
-    from sesdf_amd import Sesdf, Ssesdf, synth
-    import sesdf_amd._lib as L
+    legs = set(args.legs.split(","))
+    from sesdf_amd import synth
 
-    atoms, voids = synth.globule_voids(100_000, 10, synth.SEED_BASE + 1)
-    n = atoms.shape[0]
+    if args.calibrate_copy_mib:
+        import torch
 
-    res = {}
-    surf = Sesdf(atoms, probe_radius=1.4, cell_size=args.cell, use_graph=True)
-    ms = time_builds(surf, args.steps, args.warmup)
-    counts = surf.counts()
-    stages = stage_times(surf)
-    ext = Ssesdf(atoms, probe_radius=1.4, cell_size=args.cell, use_graph=True)
-    ms_ext = time_builds(ext, args.steps, args.warmup)
+        a = torch.ones(args.calibrate_copy_mib * 262144, device="cuda")
+        b = a.clone()
+        torch.cuda.synchronize()
+        del a, b
 
-    # dominant kernel of the complete build: the fused circle stage (stage index 1)
-    circ_ms = float(stages[1])
-    alg = circles_alg_bytes(counts, n)
-    achieved = alg / (circ_ms * 1e-3) / 1e9
-
-    large = {}
-    if not args.no_large:
-        big = synth.fibonacci_globules(500, 5000, 360.0, synth.SEED_BASE + 2)
-        free0, _ = L.mem_info()
-        sbig = Ssesdf(big, probe_radius=1.4, cell_size=args.cell, use_graph=True)
-        free1, _ = L.mem_info()
-        ms_big = time_builds(sbig, max(5, args.steps // 5), 2)
-        cb = sbig.counts()
-        large = {"atoms": int(big.shape[0]), "ms_per_build": ms_big, "atoms_per_s": big.shape[0] / (ms_big * 1e-3),
-                 "concave": cb["concave_patch_nb"], "torus": cb["torus_nb"],
-                 "device_bytes": cb["device_bytes"], "mem_delta_bytes": int(free0 - free1)}
-        del sbig
-
-    cpu = None if args.no_cpu else cpu_baseline(atoms)
-
+    atoms, _ = synth.globule_voids(100_000, 10, synth.SEED_BASE + 1)
     out = {
-        "metric": "SES build atoms/s (100k-atom globule, complete surface)",
-        "value": n / (ms * 1e-3),
-        "unit": "atoms/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "100k-atom globule with 10 voids, probe 1.4 A, complete surface (hipGraph replay)",
-                   "atoms": n, "probe_radius": 1.4, "cell_size": surf_cell(args.cell)},
-        "exterior_atoms_per_s": n / (ms_ext * 1e-3),
-        "exterior_ms_per_step": ms_ext,
-        "large_2p5M_exterior": large,
-        "stage_ms": dict(zip(["grid", "circles", "intersections", "circle_scan", "fill", "segments", "probe_nb", "torus"],
-                             [float(x) for x in stages])),
-        "counts": {k: int(v) for k, v in counts.items()},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "circles_kernel", "alg_bytes_per_launch": alg, "kernel_ms": circ_ms},
-        "cpu_baseline": cpu,
+        "metric": "SES build atoms/s (100k-atom globule, complete surface)", "value": None, "unit": "atoms/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "100k-atom globule with 10 voids, probe 1.4 A, complete surface, hipGraph replay",
+                   "atoms": int(atoms.shape[0]), "probe_radius": 1.4, "cell_size": args.cell if args.cell > 0 else 3.2},
     }
+    surf = leg_main(args, atoms, out) if "main" in legs else None
+    if "exterior" in legs:
+        leg_exterior(args, atoms, out)
+    if "sdf" in legs and surf is not None:
+        leg_sdf(args, atoms, surf, out)
+    if surf is not None:
+        surf.close()
+    if "large" in legs:
+        leg_large(args, out)
+    if "batch" in legs:
+        leg_batch(args, out)
+    if "grad" in legs:
+        leg_grad(args, out)
+    out["cpu_baseline"] = leg_cpu(atoms) if "cpu" in legs else None
     print(json.dumps(out), flush=True)
-
-
-def surf_cell(c):
-    return c if c > 0 else 3.2
 
 
 if __name__ == "__main__":

@@ -165,8 +165,10 @@ sesdf_status sesdf_get_atom_indices( sesdf_handle * h, uint32_t * host_out );
 sesdf_status sesdf_get_stream( sesdf_handle * h, void ** stream );
 
 /* Per-stage device times of the last timed build, milliseconds (app/detailed/sesdf_benchmark.cu:50-232:
- * Circle / Intersection / Segment / Neighborhood). stage_ms receives up to 8 values:
- * grid, circles, intersections, circle scan, fill, segments, probe neighbours, torus. */
+ * Circle / Intersection / Segment / Neighborhood). stage_ms receives up to SESDF_STAGE_COUNT values:
+ * grid, circle kernel (neighbours + visibility), circle offsets + sectors, intersection kernel,
+ * intersection offsets + placement, circle-list scan, fill, segments, probe neighbours, torus. */
+#define SESDF_STAGE_COUNT 10
 sesdf_status sesdf_build_timed( sesdf_handle * h, float * stage_ms, int stage_count );
 
 /* -------- SDF sampling and its gradient (no reference counterpart: the reference ships no SDF

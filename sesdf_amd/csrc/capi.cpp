@@ -215,7 +215,11 @@ namespace sesdf
         mark();
         launch_circles( b, s );
         mark();
+        launch_circles_finish( b, s );
+        mark();
         launch_intersections( b, s );
+        mark();
+        launch_intersections_finish( b, s );
         mark();
         launch_circle_scan( b, s );
         mark();
@@ -645,7 +649,7 @@ extern "C"
         sesdf_status st = readback( h );
         if ( st != SESDF_OK )
             return st;
-        for ( int q = 0; q < stageCount && q < 8; q++ )
+        for ( int q = 0; q < stageCount && q < SESDF_STAGE_COUNT; q++ )
             HIP_TRY( hipEventElapsedTime( stageMs + q, h.events[ q ], h.events[ q + 1 ] ) );
         h.built = true;
         return overflowed( h ) ? SESDF_ERR_CAPACITY : SESDF_OK;
@@ -659,6 +663,7 @@ extern "C"
         const Control & c = *h.hctl;
         std::memset( out, 0, sizeof( *out ) );
         out->atom_nb                      = h.n;
+        out->neighbor_nb                  = c.neighborNb;
         out->sector_nb                    = c.sectorNb;
         out->intersected_circle_nb        = c.intersectedCircleNb;
         out->full_circle_nb               = c.fullCircleNb;

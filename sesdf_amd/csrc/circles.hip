@@ -221,7 +221,8 @@ namespace sesdf
         }
         if ( lane == 0 )
         {
-            b.atomCnt[ i ] = nV;
+            b.atomCnt[ i ]                     = nV;
+            b.atomCnt[ 3 * size_t( b.n ) + i ] = ni; // summed by the scan (neighbour pairs)
             if ( MODE == 0 )
             {
                 b.atomCnt[ b.n + i ]     = nT;
@@ -355,19 +356,27 @@ namespace sesdf
     {
         if ( b.n == 0 )
             return;
-        const uint32_t blocks = ( b.n + CirclesWaves - 1 ) / CirclesWaves;
         launch_circles_count( b, s );
-        ScanArgs a {};
-        a.n      = b.n;
-        a.ticket = TicketCircles;
-        a.stage  = 2;
+    }
+
+    void launch_circles_finish( const Buffers & b, hipStream_t s )
+    {
+        if ( b.n == 0 )
+            return;
+        const uint32_t blocks = ( b.n + CirclesWaves - 1 ) / CirclesWaves;
+        ScanArgs       a {};
+        a.n           = b.n;
+        a.ticket      = TicketCircles;
+        a.stage       = 2;
         a.in[ 0 ]     = b.atomCnt;
         a.out[ 0 ]    = b.vStart;
         a.totals[ 0 ] = b.mode == 0 ? &b.ctl->sectorNb : &b.ctl->visibleCircleNb;
+        a.sumOut      = &b.ctl->neighborNb;
         if ( b.mode == 0 )
         {
             a.in[ 1 ]     = b.atomCnt + b.n;
             a.in[ 2 ]     = b.atomCnt + 2 * size_t( b.n );
+            a.in[ 3 ]     = b.atomCnt + 3 * size_t( b.n ); // neighbour counts (summed)
             a.out[ 1 ]    = b.trimBase;
             a.out[ 2 ]    = b.fullBase;
             a.totals[ 1 ] = &b.ctl->intersectedCircleNb;
@@ -376,6 +385,9 @@ namespace sesdf
             circles_emit_kernel<<<blocks, CirclesWaves * WaveSize, 0, s>>>( b );
         }
         else
+        {
+            a.in[ 1 ] = b.atomCnt + 3 * size_t( b.n ); // neighbour counts (summed)
             launch_scan<1>( b, a, s );
+        }
     }
 } // namespace sesdf

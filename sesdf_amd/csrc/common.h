@@ -119,7 +119,8 @@ namespace sesdf
         uint32_t circleIntOverflowNb;   // circles with > 15 intersections (segments truncated)
         uint32_t fp64ResolveNb;         // triplets re-solved in FP64
         uint32_t maxAtomRecords;        // largest per-atom concave record count
-        uint32_t pad[ 5 ];
+        uint32_t neighborNb;            // sum of neighbour counts (ordered circles)
+        uint32_t pad[ 4 ];
     };
 
     // ------------------------------------------------------------------------------------------

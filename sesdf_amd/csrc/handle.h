@@ -64,7 +64,7 @@ namespace sesdf
         Control *           hctl  = nullptr;  // pinned host copy of the control block
 
         hipGraphExec_t graphExec = nullptr;
-        hipEvent_t     events[ 9 ] = {};
+        hipEvent_t     events[ SESDF_STAGE_COUNT + 1 ] = {};
 
         SdfState            sdf;
         std::vector<void *> sdfAllocs;

@@ -673,7 +673,6 @@ namespace sesdf
     {
         if ( b.n == 0 )
             return;
-        const uint32_t blocks = ( b.n + InterWaves - 1 ) / InterWaves;
         // block shape (waves per workgroup) chosen by the autotuner
         const uint32_t wv = b.wavesInter == 2 || b.wavesInter == 8 ? b.wavesInter : 4u;
         const uint32_t cb = ( b.n + wv - 1 ) / wv;
@@ -688,7 +687,14 @@ namespace sesdf
         }
         else
             intersections_count_kernel<320, 4><<<( b.n + 3 ) / 4, 4 * WaveSize, 0, s>>>( b );
-        ScanArgs a {};
+    }
+
+    void launch_intersections_finish( const Buffers & b, hipStream_t s )
+    {
+        if ( b.n == 0 )
+            return;
+        const uint32_t blocks = ( b.n + InterWaves - 1 ) / InterWaves;
+        ScanArgs       a {};
         a.n           = b.n;
         a.ticket      = TicketIntersections;
         a.stage       = 3;

@@ -105,8 +105,10 @@ namespace sesdf
     // Host launch wrappers (one per stage); all are stream-ordered and capturable.
     void launch_begin( const Buffers & b, hipStream_t s );
     void launch_grid( const Buffers & b, hipStream_t s );
-    void launch_circles( const Buffers & b, hipStream_t s );
-    void launch_intersections( const Buffers & b, hipStream_t s );
+    void launch_circles( const Buffers & b, hipStream_t s );        // neighbour query + visibility
+    void launch_circles_finish( const Buffers & b, hipStream_t s ); // offsets + sectors / circle lists
+    void launch_intersections( const Buffers & b, hipStream_t s );  // probe candidates + occlusion
+    void launch_intersections_finish( const Buffers & b, hipStream_t s ); // offsets + record placement
     void launch_circle_scan( const Buffers & b, hipStream_t s );
     void launch_fill( const Buffers & b, hipStream_t s );
     void launch_segments( const Buffers & b, hipStream_t s );
