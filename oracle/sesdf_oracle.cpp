@@ -1288,7 +1288,9 @@ namespace orc
                         }
                     }
             double f;
-            if ( gA < 0 )
+            // no sphere within rp + band: nothing of the surface lies within the band (+band, also
+            // for the exterior variant)
+            if ( gA < 0 || g >= D )
             {
                 code = int64_t( 5 ) << 40;
                 return band;

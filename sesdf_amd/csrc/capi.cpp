@@ -44,10 +44,10 @@ namespace sesdf
     void Handle::freeAll()
     {
         if ( graphExec )
-            hipGraphExecDestroy( graphExec );
+            (void)hipGraphExecDestroy( graphExec );
         graphExec = nullptr;
         for ( void * p : allocs )
-            hipFree( p );
+            (void)hipFree( p );
         allocs.clear();
         deviceBytes = 0;
     }
@@ -207,7 +207,7 @@ namespace sesdf
         int             e = 0;
         auto            mark = [ & ]() {
             if ( ev )
-                hipEventRecord( ev[ e++ ], s );
+                (void)hipEventRecord( ev[ e++ ], s );
         };
         mark();
         launch_begin( b, s );
@@ -506,18 +506,18 @@ extern "C"
         if ( !hh )
             return SESDF_OK;
         Handle * h = reinterpret_cast<Handle *>( hh );
-        hipSetDevice( h->device );
+        (void)hipSetDevice( h->device );
         if ( h->stream )
-            hipStreamSynchronize( h->stream );
+            (void)hipStreamSynchronize( h->stream );
         h->freeAll();
         h->freeSdf();
         for ( auto & e : h->events )
             if ( e )
-                hipEventDestroy( e );
+                (void)hipEventDestroy( e );
         if ( h->hctl )
-            hipHostFree( h->hctl );
+            (void)hipHostFree( h->hctl );
         if ( h->ownStream && h->stream )
-            hipStreamDestroy( h->stream );
+            (void)hipStreamDestroy( h->stream );
         delete h;
         return SESDF_OK;
     }

@@ -84,6 +84,7 @@ namespace sesdf
         TicketCellScan,
         TicketCircleScan,
         TicketTorus,
+        TicketSdf,
         TicketCount = 8
     };
     enum Overflow : uint32_t

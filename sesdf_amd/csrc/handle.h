@@ -13,19 +13,26 @@ namespace sesdf
         uint32_t sectors = 0, full = 0, trimmed = 0, records = 0, lists = 0, segments = 0, probeNb = 0, torus = 0;
     };
 
-    // Device state of the SDF evaluator: trimmed SAS boundary elements binned on the atom grid.
+    // Device state of the SDF evaluator (allocated on first use, refreshed per build epoch).
     struct SdfState
     {
-        bool       valid       = false;
         uint32_t   builtEpoch  = 0;
-        uint32_t * exposedMask = nullptr;  // N * K/32: neighbour slots whose cap borders an exposed face
-        uint32_t * arcStart    = nullptr;  // N + 1: arcs (full circles + segments) owned by atom i
-        float4 *   arcs        = nullptr;  // per arc: 3 float4 (centre+radius, normal+owner, endpoints A/B as angles)
-        uint32_t   arcCap      = 0;
-        uint32_t * vtxStart    = nullptr;  // N + 1: concave vertices owned by atom i
         uint32_t * component   = nullptr;  // N: surface component label (exterior classification)
-        uint32_t * winner      = nullptr;  // per stored query: winning element code
-        uint64_t   winnerCap   = 0;
+        float4 *   gradScratch = nullptr;  // N: gradient in sorted order (backward without a caller buffer)
+        float4 *   packAtoms     = nullptr; // N: centre, R (rebuilt once per build epoch)
+        float4 *   surfAtoms     = nullptr; // surface atoms, compacted in cell order
+        int4 *     surfRanges    = nullptr; // 2 per surface atom: element ranges
+        uint32_t * surfIds       = nullptr; // per surface atom: sorted index | face flag << 31
+        uint32_t * surfFlag      = nullptr; // N: surface atom (scan input)
+        uint32_t * surfInfo      = nullptr; // N: face may be exposed
+        uint32_t * surfPrefix    = nullptr; // N + 1: exclusive scan of surfFlag
+        uint32_t * surfCellStart = nullptr; // cellNb + 1
+        uint32_t * surfTotal     = nullptr;
+        int2 *     surfCaps      = nullptr; // per surface atom: range of its visible circles' caps
+        float4 *   geo           = nullptr; // caps (cap.sectors) then segment geometry (3 per segment)
+        uint32_t   geoSectors    = 0;
+        uint32_t   geoSegments   = 0;
+        uint32_t   packEpoch     = 0xffffffffu;
         float4 *   molGrids    = nullptr;  // batches: per-molecule grid origin / spacing
         uint32_t   molGridDim  = 0;
         float      molGridMargin = 0.f;

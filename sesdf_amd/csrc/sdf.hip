@@ -51,6 +51,14 @@ namespace sesdf
         int              recompute;  // backward: re-derive the winners instead of reading codes
         float4 *         gradSorted; // backward: per sorted atom (dx, dy, dz, dr)
         const float4 *   molGrids;   // batches: per molecule (origin, 0), (spacing, 0); grid dx^3 each
+        const float4 *   sAtoms;        // per sorted atom: centre, R = r + rp
+        const float4 *   surfAtoms;     // surface atoms (compacted, cell order): centre, R
+        const int4 *     surfRanges;    // per surface atom: (full circles, segments), (W vertices, N vertices)
+        const uint32_t * surfIds;       // per surface atom: sorted index | face-may-be-exposed << 31
+        const uint32_t * surfCellStart; // cellNb + 1 offsets into the surface atoms
+        const int2 *     surfCaps;      // per surface atom: range of its visible circles' caps
+        const float4 *   caps;          // per visible circle (sector): axis, cos(half angle)
+        const float4 *   segGeo;        // per segment: (centre, radius), (normal, arc > pi), n x A', B' x n
     };
 
     // Query q and its molecule (batches sample one dx*dy*dz grid per molecule, molecule-major).
@@ -92,136 +100,423 @@ namespace sesdf
         return t < 0.f ? t + TwoPi : t;
     }
 
-    // Evaluates the field at q; returns the value and the winner code.
-    __device__ float sdf_eval( const Buffers & b, const SdfArgs & s, f3 q, uint32_t mol, uint64_t & code )
+    // Distance from q to the box of cells [x0, x1] x {y} x {z} of molecule brick M.
+    __device__ __forceinline__ float row_min_dist( const Molecule & M, f3 q, int x0, int x1, int y, int z )
     {
-        const float    rp   = b.probeRadius;
-        const float    rmax = __uint_as_float( b.ctl->radiusMaxBits );
-        const float    D    = rp + s.band;
-        const Molecule M    = b.mols[ mol ];
-        const uint32_t Iw   = b.ctl->intersectionWithNbNb;
-        const uint32_t extC = s.exterior ? *s.extComp : 0u;
-        const float    search = D + rmax + rp + 1e-3f;
-        const int      reach  = max( int( ceilf( search / M.cell ) ), 1 );
-        // nearest cell, clamped (queries may lie outside the brick)
-        int gx, gy, gz;
-        cell_of( M, q.x, q.y, q.z, gx, gy, gz );
-        const int x0 = max( gx - reach, 0 ), x1 = min( gx + reach, M.dx - 1 );
-        const int y0 = max( gy - reach, 0 ), y1 = min( gy + reach, M.dy - 1 );
-        const int z0 = max( gz - reach, 0 ), z1 = min( gz + reach, M.dz - 1 );
+        const float bx0 = M.ox + M.cell * float( x0 ), bx1 = M.ox + M.cell * float( x1 + 1 );
+        const float by0 = M.oy + M.cell * float( y ), by1 = by0 + M.cell;
+        const float bz0 = M.oz + M.cell * float( z ), bz1 = bz0 + M.cell;
+        const float dx  = fmaxf( fmaxf( bx0 - q.x, q.x - bx1 ), 0.f );
+        const float dy  = fmaxf( fmaxf( by0 - q.y, q.y - by1 ), 0.f );
+        const float dz  = fmaxf( fmaxf( bz0 - q.z, q.z - bz1 ), 0.f );
+        return sqrtf( dx * dx + dy * dy + dz * dz );
+    }
 
-        float    g = 3.0e38f, best = D, gExt = 3.0e38f;
-        uint32_t gA = 0xffffffffu;
-        uint64_t bc = code_of( SdfClamped, 0 );
-        // pass 1: nearest sphere (outside distance), arcs and vertices
-        for ( int zz = z0; zz <= z1; zz++ )
-            for ( int yy = y0; yy <= y1; yy++ )
+    // In a wave-cooperative evaluation (COOP) all lanes walk the same cells and atoms, so a test
+    // decides for the whole wave; otherwise every lane walks its own stencil.
+    template<bool COOP>
+    __device__ __forceinline__ bool any_lane( bool p )
+    {
+        if constexpr ( COOP )
+            return ballot( p ) != 0;
+        else
+            return p;
+    }
+    template<bool COOP>
+    __device__ __forceinline__ int uniform_int( int v )
+    {
+        if constexpr ( COOP )
+            return __builtin_amdgcn_readfirstlane( v );
+        else
+            return v;
+    }
+    __device__ __forceinline__ int wave_min_int( int v )
+    {
+        for ( int o = 32; o > 0; o >>= 1 )
+            v = min( v, __shfl_xor( v, o, WaveSize ) );
+        return v;
+    }
+    __device__ __forceinline__ int wave_max_int( int v )
+    {
+        for ( int o = 32; o > 0; o >>= 1 )
+            v = max( v, __shfl_xor( v, o, WaveSize ) );
+        return v;
+    }
+
+    struct SdfLane
+    {
+        f3       q;
+        bool     active;
+        float    g, gExt, best;
+        uint32_t gA;
+        uint64_t bc;
+    };
+
+    // Cells within `reach` of the query cells, as row segments along x: first the block of query
+    // cells grown by one (nearest atoms first, so that the bounds tighten early), then the rest.
+    struct Stencil
+    {
+        int x0, x1, y0, y1, z0, z1;       // whole stencil
+        int bx0, bx1, by0, by1, bz0, bz1; // near block
+    };
+
+    template<class F>
+    __device__ __forceinline__ void for_block( const Stencil & t, F && fn )
+    {
+        for ( int z = t.bz0; z <= t.bz1; z++ )
+            for ( int y = t.by0; y <= t.by1; y++ )
+                fn( t.bx0, t.bx1, y, z );
+    }
+    template<class F>
+    __device__ __forceinline__ void for_rest( const Stencil & t, F && fn )
+    {
+        for ( int z = t.z0; z <= t.z1; z++ )
+            for ( int y = t.y0; y <= t.y1; y++ )
             {
-                const uint32_t h0 = cell_index( M, x0, yy, zz );
-                const uint32_t a0 = b.cellStart[ h0 ], a1 = b.cellStart[ h0 + uint32_t( x1 - x0 ) + 1 ];
-                for ( uint32_t a = a0; a < a1; a++ )
+                if ( z >= t.bz0 && z <= t.bz1 && y >= t.by0 && y <= t.by1 )
                 {
-                    const float4 aa = b.atoms[ a ];
-                    const float  sd = sqrtf( len2( q - xyz( aa ) ) ) - ( aa.w + rp );
-                    if ( sd < g || ( sd == g && a < gA ) )
-                        g = sd, gA = a;
-                    if ( s.exterior && s.comp[ a ] == extC )
-                        gExt = fminf( gExt, sd );
-                    if ( sd > best || ( s.exterior && s.comp[ a ] != extC ) )
-                        continue; // every element of a lies on sphere a
-                    for ( uint32_t f = b.fullBase[ a ]; f < b.fullBase[ a + 1 ]; f++ )
-                    {
-                        const uint2 fc = b.fullCircles[ f ];
-                        f3          ctr, nrm;
-                        float       rad;
-                        circle_world( b, fc.x, fc.y, ctr, nrm, rad );
-                        const f3    v   = q - ctr;
-                        const float ax  = dot( v, nrm );
-                        const float rho = sqrtf( fmaxf( len2( v ) - ax * ax, 0.f ) );
-                        const float d   = sqrtf( ax * ax + ( rho - rad ) * ( rho - rad ) );
-                        if ( d < best )
-                            best = d, bc = code_of( SdfCircle, f );
-                    }
-                    const uint32_t s0 = b.segStart[ b.trimBase[ a ] ], s1 = b.segStart[ b.trimBase[ a + 1 ] ];
-                    for ( uint32_t sg = s0; sg < s1; sg++ )
-                    {
-                        const uint4 seg = b.segments[ sg ];
-                        f3          ctr, nrm;
-                        float       rad;
-                        circle_world( b, seg.x, seg.y, ctr, nrm, rad );
-                        const f3    v   = q - ctr;
-                        const float ax  = dot( v, nrm );
-                        const f3    w   = v - nrm * ax;
-                        const float rho = sqrtf( len2( w ) );
-                        if ( !( rho > 0.f ) )
-                            continue;
-                        const float d = sqrtf( ax * ax + ( rho - rad ) * ( rho - rad ) );
-                        if ( d >= best )
-                            continue;
-                        const f3 A = xyz( b.concavePos[ seg.z ] ) - ctr, B = xyz( b.concavePos[ seg.w ] ) - ctr;
-                        if ( angle_ccw( A, w, nrm ) <= angle_ccw( A, B, nrm ) )
-                            best = d, bc = code_of( SdfSegment, sg );
-                    }
-                    for ( int bucket = 0; bucket < 2; bucket++ )
-                    {
-                        const uint32_t v0 = bucket == 0 ? b.wStart[ a ] : Iw + b.nStart[ a ];
-                        const uint32_t v1 = bucket == 0 ? b.wStart[ a + 1 ] : Iw + b.nStart[ a + 1 ];
-                        for ( uint32_t v = v0; v < v1; v++ )
-                        {
-                            const float d = sqrtf( len2( q - xyz( b.concavePos[ v ] ) ) );
-                            if ( d < best )
-                                best = d, bc = code_of( SdfVertex, v );
-                        }
-                    }
+                    if ( t.x0 < t.bx0 )
+                        fn( t.x0, t.bx0 - 1, y, z );
+                    if ( t.bx1 < t.x1 )
+                        fn( t.bx1 + 1, t.x1, y, z );
                 }
+                else
+                    fn( t.x0, t.x1, y, z );
             }
-        // pass 2: faces closer than the best element so far
-        for ( int zz = z0; zz <= z1; zz++ )
-            for ( int yy = y0; yy <= y1; yy++ )
+    }
+
+    // ---- primitive element tests -------------------------------------------------------------
+    // Hardware square root and reciprocal (1 ulp): the field is compared to FP64 at 1e-4 A, and
+    // the correctly rounded forms cost over ten instructions each in these loops.
+    __device__ __forceinline__ float fsqrt( float x ) { return __builtin_amdgcn_sqrtf( x ); }
+    __device__ __forceinline__ float frcp( float x ) { return __builtin_amdgcn_rcpf( x ); }
+
+    __device__ __forceinline__ void full_circle_test( const Buffers & b, SdfLane & L, uint32_t f )
+    {
+        const uint2 fc = b.fullCircles[ f ];
+        f3          ctr, nrm;
+        float       rad;
+        circle_world( b, fc.x, fc.y, ctr, nrm, rad );
+        const f3    v   = L.q - ctr;
+        const float ax  = dot( v, nrm );
+        const float rho = fsqrt( fmaxf( len2( v ) - ax * ax, 0.f ) );
+        const float d   = fsqrt( ax * ax + ( rho - rad ) * ( rho - rad ) );
+        if ( d < L.best )
+            L.best = d, L.bc = code_of( SdfCircle, f );
+    }
+
+    // Segment geometry, 4 float4: (centre, radius), (normal n, 1 if the arc exceeds pi),
+    // (n x A', 0), (B' x n, 0) with A', B' the endpoints relative to the centre. The foot w of q
+    // in the circle plane lies on the arc from A' counter-clockwise to B' iff
+    //   arc <= pi: w . (n x A') >= 0 and w . (B' x n) >= 0,
+    //   arc >  pi: either holds (w is not strictly inside the complementary arc).
+    __device__ __forceinline__ void segment_test( SdfLane & L, float4 g0, float4 g1, float4 g2, float4 g3, uint32_t sg )
+    {
+        const f3    nrm = xyz( g1 );
+        const f3    v   = L.q - xyz( g0 );
+        const float ax  = dot( v, nrm );
+        const f3    w   = v - nrm * ax;
+        const float rho = fsqrt( len2( w ) );
+        const float d   = fsqrt( ax * ax + ( rho - g0.w ) * ( rho - g0.w ) );
+        if ( !( rho > 0.f ) || !( d < L.best ) )
+            return;
+        const bool a = dot( w, xyz( g2 ) ) >= 0.f, bb = dot( w, xyz( g3 ) ) >= 0.f;
+        if ( g1.w != 0.f ? ( a || bb ) : ( a && bb ) )
+            L.best = d, L.bc = code_of( SdfSegment, sg );
+    }
+
+    __device__ __forceinline__ void vertex_test( SdfLane & L, float4 p, uint32_t v )
+    {
+        const float d = fsqrt( len2( L.q - xyz( p ) ) );
+        if ( d < L.best )
+            L.best = d, L.bc = code_of( SdfVertex, v );
+    }
+
+    // ---- per-lane evaluation (scattered points): each lane walks its own cells ----------------
+    __device__ __forceinline__ void lane_nearest( const SdfArgs & s, SdfLane & L, uint32_t a0, uint32_t a1, uint32_t extC )
+    {
+        for ( uint32_t a = a0; a < a1; a++ )
+        {
+            const float4 aa = s.sAtoms[ a ];
+            const float  sd = fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w;
+            if ( sd < L.g || ( sd == L.g && a < L.gA ) )
+                L.g = sd, L.gA = a;
+            if ( s.exterior && s.comp[ a ] == extC )
+                L.gExt = fminf( L.gExt, sd );
+        }
+    }
+
+    // Arcs and vertices of the surface atoms [k0, k1) (each lies on its atom's sphere, at distance
+    // >= |sd| from q), then their faces: the radial projection of q is exposed unless it lies in
+    // the cap of a visible circle (hidden circles' caps lie inside visible ones).
+    __device__ __forceinline__ void lane_row( const Buffers & b, const SdfArgs & s, SdfLane & L, uint32_t k0, uint32_t k1, uint32_t extC,
+                                              uint32_t Iw, bool elems, bool faces )
+    {
+        for ( uint32_t k = k0; elems && k < k1; k++ )
+        {
+            const uint32_t a = s.surfIds[ k ] & 0x7fffffffu;
+            if ( s.exterior && s.comp[ a ] != extC )
+                continue;
+            const float4 aa = s.surfAtoms[ k ];
+            if ( !( fabsf( fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w ) < L.best ) )
+                continue;
+            const int4 r0 = s.surfRanges[ 2 * k ], r1 = s.surfRanges[ 2 * k + 1 ];
+            for ( int f = r0.x; f < r0.y; f++ )
+                full_circle_test( b, L, uint32_t( f ) );
+            for ( int sg = r0.z; sg < r0.w; sg++ )
+                segment_test( L, s.segGeo[ 4 * sg ], s.segGeo[ 4 * sg + 1 ], s.segGeo[ 4 * sg + 2 ], s.segGeo[ 4 * sg + 3 ], uint32_t( sg ) );
+            for ( int v = r1.x; v < r1.y; v++ )
+                vertex_test( L, b.concavePos[ v ], uint32_t( v ) );
+            for ( int v = int( Iw ) + r1.z; v < int( Iw ) + r1.w; v++ )
+                vertex_test( L, b.concavePos[ v ], uint32_t( v ) );
+        }
+        for ( uint32_t k = k0; faces && k < k1; k++ )
+        {
+            const uint32_t id = s.surfIds[ k ], a = id & 0x7fffffffu;
+            if ( !( id >> 31 ) || ( s.exterior && s.comp[ a ] != extC ) )
+                continue;
+            const float4 aa   = s.surfAtoms[ k ];
+            const f3     dq   = L.q - xyz( aa );
+            const float  dist = fsqrt( len2( dq ) );
+            const float  sd   = dist - aa.w;
+            if ( !( fabsf( sd ) < L.best ) || !( dist > 0.f ) )
+                continue;
+            const f3   u       = dq * frcp( dist );
+            const int2 cr      = s.surfCaps[ k ];
+            bool       exposed = true;
+            for ( int c = cr.x; c < cr.y && exposed; c++ )
             {
-                const uint32_t h0 = cell_index( M, x0, yy, zz );
-                const uint32_t a0 = b.cellStart[ h0 ], a1 = b.cellStart[ h0 + uint32_t( x1 - x0 ) + 1 ];
-                for ( uint32_t a = a0; a < a1; a++ )
+                const float4 cap = s.caps[ c ]; // axis, cos(half angle)
+                exposed          = !( dot( u, xyz( cap ) ) > cap.w );
+            }
+            if ( exposed )
+                L.best = fabsf( sd ), L.bc = code_of( SdfFace, a );
+        }
+    }
+
+    // ---- wave-cooperative evaluation: records are loaded 64 at a time, one per lane, and
+    // broadcast with readlane, or staged in the wave's LDS area and read at a uniform address --
+    __device__ __forceinline__ float    rl( float v, int j ) { return __int_as_float( __builtin_amdgcn_readlane( __float_as_int( v ), j ) ); }
+    __device__ __forceinline__ int      rl( int v, int j ) { return __builtin_amdgcn_readlane( v, j ); }
+    __device__ __forceinline__ uint32_t rl( uint32_t v, int j ) { return uint32_t( __builtin_amdgcn_readlane( int( v ), j ) ); }
+    __device__ __forceinline__ float4   rl( float4 v, int j ) { return make_float4( rl( v.x, j ), rl( v.y, j ), rl( v.z, j ), rl( v.w, j ) ); }
+    __device__ __forceinline__ int4     rl( int4 v, int j ) { return make_int4( rl( v.x, j ), rl( v.y, j ), rl( v.z, j ), rl( v.w, j ) ); }
+
+    constexpr int StageSize = 384; // float4 per wave: 64 segments (4 each) + 64 + 64 vertices, or caps
+
+    __device__ __forceinline__ void coop_nearest( const SdfArgs & s, SdfLane & L, uint32_t a0, uint32_t a1, uint32_t extC )
+    {
+        const uint32_t lane = uint32_t( lane_id() );
+        for ( uint32_t base = a0; base < a1; base += WaveSize )
+        {
+            const uint32_t a    = base + lane;
+            const bool     ok   = a < a1;
+            const float4   mine = ok ? s.sAtoms[ a ] : make_float4( 0.f, 0.f, 0.f, 0.f );
+            const uint32_t ext  = ok && s.exterior && s.comp[ a ] == extC ? 1u : 0u;
+            const int      cnt  = int( min( uint32_t( WaveSize ), a1 - base ) );
+            for ( int j = 0; j < cnt; j++ )
+            {
+                const float4   aa = rl( mine, j );
+                const float    sd = fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w;
+                const uint32_t ai = base + uint32_t( j );
+                if ( sd < L.g || ( sd == L.g && ai < L.gA ) )
+                    L.g = sd, L.gA = ai;
+                if ( rl( ext, j ) )
+                    L.gExt = fminf( L.gExt, sd );
+            }
+        }
+    }
+
+    __device__ __forceinline__ void coop_row( const Buffers & b, const SdfArgs & s, SdfLane & L, bool need, uint32_t k0, uint32_t k1, uint32_t extC,
+                                              uint32_t Iw, bool elems, bool faces, float4 * stage )
+    {
+        const uint32_t lane = uint32_t( lane_id() );
+        for ( uint32_t base = k0; base < k1; base += WaveSize )
+        {
+            const uint32_t k      = base + lane;
+            const bool     ok     = k < k1;
+            const float4   myAtom = ok ? s.surfAtoms[ k ] : make_float4( 0.f, 0.f, 0.f, 0.f );
+            const uint32_t myId   = ok ? s.surfIds[ k ] : 0u;
+            const uint32_t myExt  = ok && ( !s.exterior || s.comp[ myId & 0x7fffffffu ] == extC ) ? 1u : 0u;
+            const int      cnt    = int( min( uint32_t( WaveSize ), k1 - base ) );
+            const uint32_t last   = min( k, k1 - 1 );
+            if ( elems )
+            {
+                const int4 myR0 = s.surfRanges[ 2 * last ], myR1 = s.surfRanges[ 2 * last + 1 ];
+                // the chunk's atoms are consecutive in sorted order, so the records they own are
+                // contiguous; staged in LDS (up to 64 of each kind) on the first atom that needs them
+                const int  s0 = rl( myR0.z, 0 ), s1 = rl( myR0.w, cnt - 1 );
+                const int  w0 = rl( myR1.x, 0 ), w1 = rl( myR1.y, cnt - 1 );
+                const int  n0 = int( Iw ) + rl( myR1.z, 0 ), n1 = int( Iw ) + rl( myR1.w, cnt - 1 );
+                const bool segIn = s1 - s0 <= WaveSize, wIn = w1 - w0 <= WaveSize, nIn = n1 - n0 <= WaveSize;
+                bool       staged = false;
+                for ( int j = 0; j < cnt; j++ )
                 {
-                    const float4 aa   = b.atoms[ a ];
-                    const f3     dq   = q - xyz( aa );
-                    const float  dist = sqrtf( len2( dq ) );
-                    const float  Ra   = aa.w + rp;
-                    const float  sd   = dist - Ra;
-                    if ( !( fabsf( sd ) < best ) || !( dist > 0.f ) || ( s.exterior && s.comp[ a ] != extC ) )
+                    if ( !rl( myExt, j ) )
                         continue;
-                    const f3       p   = dq * ( Ra / dist ); // relative to c_a
-                    const uint32_t na  = b.nbCount[ a ];
-                    bool           exp = true;
-                    for ( uint32_t l = 0; l < na && exp; l++ )
+                    const float4 aa = rl( myAtom, j );
+                    if ( !any_lane<true>( need && fabsf( fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w ) < L.best ) )
+                        continue;
+                    if ( !staged )
                     {
-                        const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
-                        const float  Rl = al.w + rp;
-                        exp             = len2( p - ( xyz( al ) - xyz( aa ) ) ) - Rl * Rl >= 0.f;
+                        wave_sync();
+                        for ( int i = int( lane ); segIn && i < 4 * ( s1 - s0 ); i += WaveSize )
+                            stage[ i ] = s.segGeo[ 4 * s0 + i ];
+                        if ( wIn && int( lane ) < w1 - w0 )
+                            stage[ 256 + lane ] = b.concavePos[ w0 + int( lane ) ];
+                        if ( nIn && int( lane ) < n1 - n0 )
+                            stage[ 320 + lane ] = b.concavePos[ n0 + int( lane ) ];
+                        wave_sync();
+                        staged = true;
                     }
-                    if ( exp )
-                        best = fabsf( sd ), bc = code_of( SdfFace, a );
+                    const int4 r0 = rl( myR0, j ), r1 = rl( myR1, j );
+                    for ( int f = r0.x; f < r0.y; f++ )
+                        full_circle_test( b, L, uint32_t( f ) );
+                    for ( int sg = r0.z; sg < r0.w; sg++ )
+                    {
+                        const float4 * g = segIn ? stage + 4 * ( sg - s0 ) : s.segGeo + 4 * sg;
+                        segment_test( L, g[ 0 ], g[ 1 ], g[ 2 ], g[ 3 ], uint32_t( sg ) );
+                    }
+                    for ( int v = r1.x; v < r1.y; v++ )
+                        vertex_test( L, wIn ? stage[ 256 + v - w0 ] : b.concavePos[ v ], uint32_t( v ) );
+                    for ( int v = int( Iw ) + r1.z; v < int( Iw ) + r1.w; v++ )
+                        vertex_test( L, nIn ? stage[ 320 + v - n0 ] : b.concavePos[ v ], uint32_t( v ) );
                 }
             }
-        float f;
-        if ( gA == 0xffffffffu )
+            if ( faces )
+            {
+                const int2 myCaps = s.surfCaps[ last ];
+                const int  c0 = rl( myCaps.x, 0 ), c1 = rl( myCaps.y, cnt - 1 );
+                const bool capIn  = c1 - c0 <= StageSize;
+                bool       staged = false;
+                for ( int j = 0; j < cnt; j++ )
+                {
+                    const uint32_t id = rl( myId, j );
+                    if ( !( id >> 31 ) || !rl( myExt, j ) )
+                        continue;
+                    const float4 aa      = rl( myAtom, j );
+                    const f3     dq      = L.q - xyz( aa );
+                    const float  dist    = fsqrt( len2( dq ) );
+                    const float  sd      = dist - aa.w;
+                    bool         exposed = need && fabsf( sd ) < L.best && dist > 0.f;
+                    if ( !any_lane<true>( exposed ) )
+                        continue;
+                    if ( capIn && !staged )
+                    {
+                        wave_sync();
+                        for ( int i = int( lane ); i < c1 - c0; i += WaveSize )
+                            stage[ i ] = s.caps[ c0 + i ];
+                        wave_sync();
+                        staged = true;
+                    }
+                    const f3  u  = dq * frcp( dist );
+                    const int a0 = rl( myCaps.x, j ), a1 = rl( myCaps.y, j );
+                    // caps in groups of four (largest first): one exit test per group
+                    for ( int c = a0; c < a1 && any_lane<true>( exposed ); c += 4 )
+                    {
+#pragma unroll
+                        for ( int e = 0; e < 4; e++ )
+                            if ( c + e < a1 )
+                            {
+                                const float4 cap = capIn ? stage[ c + e - c0 ] : s.caps[ c + e ]; // axis, cos(half angle)
+                                exposed          = exposed && !( dot( u, xyz( cap ) ) > cap.w );
+                            }
+                    }
+                    if ( exposed )
+                        L.best = fabsf( sd ), L.bc = code_of( SdfFace, id & 0x7fffffffu );
+                }
+            }
+        }
+    }
+
+    // Field value and winner code at L.q (molecule `mol`, identical in every lane when COOP).
+    //   pass A (all atoms): nearest sphere g; with g >= rp + band nothing lies within the band and
+    //     the value is +band; outside A (g >= 0, exterior component for the exterior variant) the
+    //     value is rp + g;
+    //   pass B (surface atoms only, inside points): nearest arc, vertex or exposed face; in the
+    //     near block arcs and vertices come first, so that the costlier face tests are pruned by
+    //     their distance; further rows are nearly always culled or cheap.
+    // A row segment is skipped when the distance bound of its box cannot change the outcome.
+    template<bool COOP>
+    __device__ float sdf_eval_lanes( const Buffers & b, const SdfArgs & s, uint32_t mol, SdfLane & L, uint64_t & code, float4 * stage )
+    {
+        const float    rp    = b.probeRadius;
+        const float    Rmax  = __uint_as_float( b.ctl->radiusMaxBits ) + rp;
+        const float    D     = rp + s.band;
+        const Molecule M     = b.mols[ mol ];
+        const uint32_t Iw    = b.ctl->intersectionWithNbNb;
+        const uint32_t extC  = s.exterior ? *s.extComp : 0u;
+        const int      reach = max( int( ceilf( ( D + Rmax + 1e-3f ) / M.cell ) ), 1 );
+        int            gx, gy, gz;
+        cell_of( M, L.q.x, L.q.y, L.q.z, gx, gy, gz );
+        int cx0 = gx, cx1 = gx, cy0 = gy, cy1 = gy, cz0 = gz, cz1 = gz;
+        if constexpr ( COOP )
+        {
+            constexpr int Big = 1 << 30;
+            cx0 = uniform_int<COOP>( wave_min_int( L.active ? gx : Big ) ), cx1 = uniform_int<COOP>( wave_max_int( L.active ? gx : -Big ) );
+            cy0 = uniform_int<COOP>( wave_min_int( L.active ? gy : Big ) ), cy1 = uniform_int<COOP>( wave_max_int( L.active ? gy : -Big ) );
+            cz0 = uniform_int<COOP>( wave_min_int( L.active ? gz : Big ) ), cz1 = uniform_int<COOP>( wave_max_int( L.active ? gz : -Big ) );
+        }
+        Stencil t;
+        t.x0 = max( cx0 - reach, 0 ), t.x1 = min( cx1 + reach, M.dx - 1 );
+        t.y0 = max( cy0 - reach, 0 ), t.y1 = min( cy1 + reach, M.dy - 1 );
+        t.z0 = max( cz0 - reach, 0 ), t.z1 = min( cz1 + reach, M.dz - 1 );
+        t.bx0 = max( cx0 - 1, 0 ), t.bx1 = min( cx1 + 1, M.dx - 1 );
+        t.by0 = max( cy0 - 1, 0 ), t.by1 = min( cy1 + 1, M.dy - 1 );
+        t.bz0 = max( cz0 - 1, 0 ), t.bz1 = min( cz1 + 1, M.dz - 1 );
+        // lower bound of |q - c_a| - R_a over a row segment (1e-3 A covers the rounding of the
+        // cell assignment)
+        auto lower = [ & ]( int xa, int xb, int y, int z ) { return row_min_dist( M, L.q, xa, xb, y, z ) - Rmax - 1e-3f; };
+
+        // pass A. Once g < 0 the point is inside and g no longer matters. Otherwise a segment can
+        // change the outcome only below g + 1e-4 (exterior test) or, for the complete variant,
+        // below min(g, max(band - rp, 0)) (above, the value is clamped to +band).
+        L.g = D, L.gExt = 3.0e38f, L.gA = 0xffffffffu;
+        auto nearest = [ & ]( int xa, int xb, int y, int z ) {
+            const float lb   = lower( xa, xb, y, z );
+            const float gCut = s.exterior ? L.g + 1e-4f : fminf( L.g, fmaxf( s.band - rp, 0.f ) );
+            if ( !any_lane<COOP>( L.active && L.g >= 0.f && lb <= gCut ) )
+                return;
+            const uint32_t h0 = cell_index( M, xa, y, z );
+            const uint32_t a0 = b.cellStart[ h0 ], a1 = b.cellStart[ h0 + uint32_t( xb - xa ) + 1 ];
+            if constexpr ( COOP )
+                coop_nearest( s, L, a0, a1, extC );
+            else
+                lane_nearest( s, L, a0, a1, extC );
+        };
+        for_block( t, nearest );
+        for_rest( t, nearest );
+
+        // 0: +band (nothing within the band), 1: outside A, 2: inside (pass B)
+        int state = 0;
+        if ( L.gA != 0xffffffffu && L.g < D )
+            state = L.g >= 0.f && ( !s.exterior || L.gExt <= L.g + 1e-4f ) ? 1 : 2;
+        const bool needB = L.active && state == 2;
+        L.best = D, L.bc = code_of( SdfClamped, 0 );
+        if ( any_lane<COOP>( needB ) )
+        {
+            auto surface = [ & ]( int xa, int xb, int y, int z, bool elems, bool faces ) {
+                if ( !any_lane<COOP>( needB && lower( xa, xb, y, z ) < L.best ) )
+                    return;
+                const uint32_t h0 = cell_index( M, xa, y, z );
+                const uint32_t k0 = s.surfCellStart[ h0 ], k1 = s.surfCellStart[ h0 + uint32_t( xb - xa ) + 1 ];
+                if constexpr ( COOP )
+                    coop_row( b, s, L, needB, k0, k1, extC, Iw, elems, faces, stage );
+                else
+                    lane_row( b, s, L, k0, k1, extC, Iw, elems, faces );
+            };
+            for_block( t, [ & ]( int xa, int xb, int y, int z ) { surface( xa, xb, y, z, true, false ); } );
+            for_block( t, [ & ]( int xa, int xb, int y, int z ) { surface( xa, xb, y, z, false, true ); } );
+            for_rest( t, [ & ]( int xa, int xb, int y, int z ) { surface( xa, xb, y, z, true, true ); } );
+        }
+
+        if ( state == 0 )
         {
             code = code_of( SdfClamped, 0 );
             return s.band;
         }
-        // outside A: exterior unless the nearest sphere of the exterior component is farther than
-        // the nearest sphere by more than 1e-4 A (ties between components count as exterior)
-        if ( g >= 0.f && ( !s.exterior || gExt <= g + 1e-4f ) )
-        {
-            f    = g;
-            code = code_of( SdfOutside, gA );
-        }
-        else
-        {
-            f    = -best;
-            code = bc;
-        }
-        const float v = rp + f;
+        const float v = rp + ( state == 1 ? L.g : -L.best );
+        code          = state == 1 ? code_of( SdfOutside, L.gA ) : L.bc;
         if ( v >= s.band || v <= -s.band )
         {
             code = code_of( SdfClamped, 0 );
@@ -230,17 +525,170 @@ namespace sesdf
         return v;
     }
 
-    __global__ void __launch_bounds__( 256 ) sdf_kernel( Buffers b, SdfArgs s )
+    __device__ float sdf_eval( const Buffers & b, const SdfArgs & s, f3 q, uint32_t mol, uint64_t & code )
     {
-        for ( uint64_t q = uint64_t( blockIdx.x ) * blockDim.x + threadIdx.x; q < s.n; q += uint64_t( gridDim.x ) * blockDim.x )
+        SdfLane L;
+        L.q      = q;
+        L.active = true;
+        return sdf_eval_lanes<false>( b, s, mol, L, code, nullptr );
+    }
+
+    // Per sorted atom: centre and R; surface flag (owns an arc or vertex, or its face may be
+    // exposed) for the compaction scan. A face can be exposed only if the atom has a visible
+    // circle, or no neighbour cuts a cap off its sphere: otherwise the exposed part would be
+    // bounded by arcs of visible circles.
+    __global__ void sdf_pack_kernel( Buffers b, float4 * pa, uint32_t * surfFlag, uint32_t * info )
+    {
+        const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+        if ( a >= b.n )
+            return;
+        const float4 at = b.atoms[ a ];
+        const float  Ra = at.w + b.probeRadius;
+        pa[ a ]         = make_float4( at.x, at.y, at.z, Ra );
+        const bool elems = b.fullBase[ a + 1 ] > b.fullBase[ a ] || b.segStart[ b.trimBase[ a + 1 ] ] > b.segStart[ b.trimBase[ a ] ] ||
+                           b.wStart[ a + 1 ] > b.wStart[ a ] || b.nStart[ a + 1 ] > b.nStart[ a ];
+        // inside: the sphere lies in a neighbour's ball; cap: some neighbour robustly cuts a
+        // non-empty cap (the caps of visible circles then bound any exposed part)
+        bool           inside = false, cap = false;
+        const uint32_t na     = b.nbCount[ a ];
+        for ( uint32_t l = 0; l < na; l++ )
         {
+            const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
+            const float  Rl = al.w + b.probeRadius;
+            const float  d  = sqrtf( len2( xyz( al ) - xyz( at ) ) );
+            inside          = inside || d + Ra <= Rl;
+            cap             = cap || ( d < Ra + Rl - 1e-3f && d + Rl > Ra + 1e-3f );
+        }
+        const bool open = !inside && ( b.vStart[ a + 1 ] > b.vStart[ a ] || !cap );
+        surfFlag[ a ] = ( elems || open ) ? 1u : 0u;
+        info[ a ]     = open ? 1u : 0u;
+    }
+
+    // Compacted surface atoms in cell order and their per-cell offsets.
+    __global__ void sdf_surface_kernel( Buffers b, const float4 * pa, const uint32_t * surfFlag, const uint32_t * info, const uint32_t * prefix,
+                                        float4 * sa, int4 * sr, uint32_t * sid, int2 * sc, uint32_t * sCellStart )
+    {
+        const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+        if ( x <= b.cellNb )
+            sCellStart[ x ] = prefix[ b.cellStart[ x ] ];
+        if ( x >= b.n || !surfFlag[ x ] )
+            return;
+        const uint32_t a = x, k = prefix[ a ];
+        sa[ k ]          = pa[ a ];
+        sr[ 2 * k ]      = make_int4( int( b.fullBase[ a ] ), int( b.fullBase[ a + 1 ] ), int( b.segStart[ b.trimBase[ a ] ] ),
+                                      int( b.segStart[ b.trimBase[ a + 1 ] ] ) );
+        sr[ 2 * k + 1 ]  = make_int4( int( b.wStart[ a ] ), int( b.wStart[ a + 1 ] ), int( b.nStart[ a ] ), int( b.nStart[ a + 1 ] ) );
+        sid[ k ]         = a | ( info[ a ] << 31 );
+        sc[ k ]          = make_int2( int( b.vStart[ a ] ), int( b.vStart[ a + 1 ] ) );
+    }
+
+    // Cap of every visible circle (sector: axis, half angle) and the geometry of every segment.
+    __global__ void sdf_geometry_kernel( Buffers b, float4 * caps, float4 * segGeo )
+    {
+        const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+        if ( x < min( b.ctl->sectorNb, b.capSectors ) )
+        {
+            const float4 sc = b.sectors[ x ];
+            caps[ x ]       = make_float4( sc.x, sc.y, sc.z, cosf( sc.w ) );
+        }
+        if ( x < min( b.ctl->segmentNb, b.capSegments ) )
+        {
+            const uint4 seg = b.segments[ x ];
+            f3          ctr, nrm;
+            float       rad;
+            circle_world( b, seg.x, seg.y, ctr, nrm, rad );
+            const f3 A = xyz( b.concavePos[ seg.z ] ) - ctr, B = xyz( b.concavePos[ seg.w ] ) - ctr;
+            const f3 P = cross( nrm, A ), Q = cross( B, nrm );
+            segGeo[ 4 * x ]     = make_float4( ctr.x, ctr.y, ctr.z, rad );
+            segGeo[ 4 * x + 1 ] = make_float4( nrm.x, nrm.y, nrm.z, angle_ccw( A, B, nrm ) > 3.14159265f ? 1.f : 0.f );
+            segGeo[ 4 * x + 2 ] = make_float4( P.x, P.y, P.z, 0.f );
+            segGeo[ 4 * x + 3 ] = make_float4( Q.x, Q.y, Q.z, 0.f );
+        }
+    }
+
+    // Caps of each atom, largest first (a covered direction is then found early).
+    __global__ void sdf_sort_caps_kernel( Buffers b, float4 * caps )
+    {
+        const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+        if ( a >= b.n )
+            return;
+        const uint32_t c0 = b.vStart[ a ], c1 = min( b.vStart[ a + 1 ], b.capSectors );
+        for ( uint32_t i = c0 + 1; i < c1; i++ )
+        {
+            const float4 x = caps[ i ];
+            uint32_t     j = i;
+            for ( ; j > c0 && caps[ j - 1 ].w > x.w; j-- )
+                caps[ j ] = caps[ j - 1 ];
+            caps[ j ] = x;
+        }
+    }
+
+    // Scattered points: a wave whose points fall in a compact block of cells is evaluated
+    // cooperatively, any other wave lane by lane.
+    __global__ void __launch_bounds__( 256 ) sdf_points_kernel( Buffers b, SdfArgs s )
+    {
+        __shared__ float4 stage[ 256 / WaveSize ][ StageSize ];
+        const uint64_t stride = uint64_t( gridDim.x ) * blockDim.x;
+        for ( uint64_t base = uint64_t( blockIdx.x ) * blockDim.x; base < s.n; base += stride )
+        {
+            const uint64_t q = base + threadIdx.x;
+            SdfLane        L;
+            L.active = q < s.n;
+            L.q      = L.active ? xyz( s.points[ q ] ) : mk( 0.f, 0.f, 0.f );
+            int gx = 0, gy = 0, gz = 0;
+            const Molecule M = b.mols[ 0 ];
+            if ( L.active )
+                cell_of( M, L.q.x, L.q.y, L.q.z, gx, gy, gz );
+            constexpr int Big = 1 << 30;
+            const int     ex  = wave_max_int( L.active ? gx : -Big ) - wave_min_int( L.active ? gx : Big );
+            const int     ey  = wave_max_int( L.active ? gy : -Big ) - wave_min_int( L.active ? gy : Big );
+            const int     ez  = wave_max_int( L.active ? gz : -Big ) - wave_min_int( L.active ? gz : Big );
+            uint64_t      code;
+            float         v = 0.f;
+            if ( !any_lane<true>( L.active ) )
+                continue;
+            if ( ex <= 2 && ey <= 2 && ez <= 2 )
+                v = sdf_eval_lanes<true>( b, s, 0, L, code, stage[ threadIdx.x / WaveSize ] );
+            else if ( L.active )
+                v = sdf_eval_lanes<false>( b, s, 0, L, code, nullptr );
+            if ( L.active )
+            {
+                s.out[ q ] = v;
+                if ( s.codes )
+                    s.codes[ q ] = code;
+            }
+        }
+    }
+
+    // Regular grids: one wave per 4 x 4 x 4 brick of grid vertices (batches: per molecule grid).
+    __global__ void __launch_bounds__( 256, 4 ) sdf_grid_kernel( Buffers b, SdfArgs s, uint32_t molNb )
+    {
+        __shared__ float4 stage[ 256 / WaveSize ][ StageSize ];
+        const uint32_t nbx = ( s.dx + 3 ) / 4, nby = ( s.dy + 3 ) / 4, nbz = ( s.dz + 3 ) / 4;
+        const uint64_t bricks = uint64_t( nbx ) * nby * nbz, total = bricks * molNb;
+        const uint64_t per    = uint64_t( s.dx ) * s.dy * s.dz;
+        const int      lane   = lane_id();
+        const uint32_t wpb    = blockDim.x / WaveSize;
+        for ( uint64_t w = uint64_t( blockIdx.x ) * wpb + threadIdx.x / WaveSize; w < total; w += uint64_t( gridDim.x ) * wpb )
+        {
+            const uint32_t mol = uint32_t( w / bricks );
+            const uint64_t br  = w - uint64_t( mol ) * bricks;
+            const uint32_t ix  = uint32_t( br % nbx ) * 4 + ( lane & 3 );
+            const uint32_t iy  = uint32_t( ( br / nbx ) % nby ) * 4 + ( ( lane >> 2 ) & 3 );
+            const uint32_t iz  = uint32_t( br / ( uint64_t( nbx ) * nby ) ) * 4 + ( lane >> 4 );
+            SdfLane        L;
+            L.active         = ix < s.dx && iy < s.dy && iz < s.dz;
+            const uint64_t q = uint64_t( mol ) * per + ( uint64_t( iz ) * s.dy + iy ) * s.dx + ix;
+            uint32_t       m;
+            L.q = query_point( s, L.active ? q : uint64_t( mol ) * per, m );
             uint64_t    code;
-            uint32_t    mol;
-            const f3    qp = query_point( s, q, mol );
-            const float v  = sdf_eval( b, s, qp, mol, code );
-            s.out[ q ]    = v;
-            if ( s.codes )
-                s.codes[ q ] = code;
+            const float v = sdf_eval_lanes<true>( b, s, mol, L, code, stage[ threadIdx.x / WaveSize ] );
+            if ( L.active )
+            {
+                s.out[ q ] = v;
+                if ( s.codes )
+                    s.codes[ q ] = code;
+            }
         }
     }
 
@@ -495,6 +943,78 @@ namespace
         return SESDF_OK;
     }
 
+    // Packed atoms and the compacted surface atoms of the current build (once per build epoch).
+    sesdf_status prepare_surface( Handle & h )
+    {
+        SdfState &     d = h.sdf;
+        const uint32_t n = h.n;
+        if ( d.packAtoms && d.packEpoch == h.hctl->epoch )
+            return SESDF_OK;
+        if ( !d.packAtoms )
+        {
+            const size_t n1    = size_t( n ) + 1;
+            const size_t bytes = n1 * ( 16 + 16 + 32 + 8 ) + ( n1 * 4 ) * 4 + ( size_t( h.cellNb ) + 1 ) * 4 + 256;
+            void *       p     = nullptr;
+            if ( hipMalloc( &p, bytes ) != hipSuccess )
+                return SESDF_ERR_HIP;
+            h.sdfAllocs.push_back( p );
+            h.sdfBytes += bytes;
+            char * c        = static_cast<char *>( p );
+            d.packAtoms     = reinterpret_cast<float4 *>( c ), c += n1 * 16;
+            d.surfAtoms     = reinterpret_cast<float4 *>( c ), c += n1 * 16;
+            d.surfRanges    = reinterpret_cast<int4 *>( c ), c += n1 * 32;
+            d.surfCaps      = reinterpret_cast<int2 *>( c ), c += n1 * 8;
+            d.surfIds       = reinterpret_cast<uint32_t *>( c ), c += n1 * 4;
+            d.surfFlag      = reinterpret_cast<uint32_t *>( c ), c += n1 * 4;
+            d.surfInfo      = reinterpret_cast<uint32_t *>( c ), c += n1 * 4;
+            d.surfPrefix    = reinterpret_cast<uint32_t *>( c ), c += n1 * 4;
+            d.surfCellStart = reinterpret_cast<uint32_t *>( c ), c += ( size_t( h.cellNb ) + 1 ) * 4;
+            d.surfTotal     = reinterpret_cast<uint32_t *>( c );
+        }
+        // caps and segment geometry, sized by the build's capacities (which grow on overflow)
+        if ( !d.geo || d.geoSectors < h.cap.sectors || d.geoSegments < h.cap.segments )
+        {
+            if ( d.geo )
+            {
+                h.sdfAllocs.erase( std::find( h.sdfAllocs.begin(), h.sdfAllocs.end(), static_cast<void *>( d.geo ) ) );
+                (void)hipFree( d.geo );
+                d.geo = nullptr;
+            }
+            const size_t bytes = ( size_t( h.cap.sectors ) + 4 * size_t( h.cap.segments ) ) * 16;
+            void *       p     = nullptr;
+            if ( hipMalloc( &p, bytes ) != hipSuccess )
+                return SESDF_ERR_HIP;
+            h.sdfAllocs.push_back( p );
+            h.sdfBytes += bytes;
+            d.geo         = static_cast<float4 *>( p );
+            d.geoSectors  = h.cap.sectors;
+            d.geoSegments = h.cap.segments;
+        }
+        hipStream_t s = h.stream;
+        if ( n )
+            sdf_pack_kernel<<<( n + 255 ) / 256, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo );
+        ScanArgs a {};
+        a.in[ 0 ]     = d.surfFlag;
+        a.out[ 0 ]    = d.surfPrefix;
+        a.totals[ 0 ] = d.surfTotal;
+        a.n           = n;
+        a.ticket      = TicketSdf;
+        a.stage       = 6;
+        launch_scan<1>( h.b, a, s );
+        const uint32_t items = std::max( n, h.cellNb + 1 );
+        sdf_surface_kernel<<<( items + 255 ) / 256, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo, d.surfPrefix, d.surfAtoms,
+                                                                 d.surfRanges, d.surfIds, d.surfCaps, d.surfCellStart );
+        const uint32_t geoItems = std::max( std::min( h.hctl->sectorNb, h.cap.sectors ), std::min( h.hctl->segmentNb, h.cap.segments ) );
+        if ( geoItems )
+            sdf_geometry_kernel<<<( geoItems + 255 ) / 256, 256, 0, s>>>( h.b, d.geo, d.geo + d.geoSectors );
+        if ( n )
+            sdf_sort_caps_kernel<<<( n + 255 ) / 256, 256, 0, s>>>( h.b, d.geo );
+        if ( hipGetLastError() != hipSuccess )
+            return SESDF_ERR_HIP;
+        d.packEpoch = h.hctl->epoch;
+        return SESDF_OK;
+    }
+
     sesdf_status sdf_common( Handle & h, SdfArgs & s, int flags )
     {
         if ( !h.built )
@@ -507,6 +1027,17 @@ namespace
         if ( hipSetDevice( h.device ) != hipSuccess )
             return SESDF_ERR_HIP;
         s.exterior = flags & 1;
+        sesdf_status pst = prepare_surface( h );
+        if ( pst != SESDF_OK )
+            return pst;
+        s.sAtoms        = h.sdf.packAtoms;
+        s.surfAtoms     = h.sdf.surfAtoms;
+        s.surfRanges    = h.sdf.surfRanges;
+        s.surfIds       = h.sdf.surfIds;
+        s.surfCellStart = h.sdf.surfCellStart;
+        s.surfCaps      = h.sdf.surfCaps;
+        s.caps          = h.sdf.geo;
+        s.segGeo        = h.sdf.geo + h.sdf.geoSectors;
         if ( s.exterior )
         {
             sesdf_status st = prepare_components( h );
@@ -516,6 +1047,14 @@ namespace
             s.extComp = h.sdf.component + h.n;
         }
         return SESDF_OK;
+    }
+
+    // blocks of 4 waves for one wave per 4^3 brick of grid vertices
+    uint32_t brick_blocks( const SdfArgs & s, uint32_t molNb )
+    {
+        const uint64_t waves = uint64_t( ( s.dx + 3 ) / 4 ) * ( ( s.dy + 3 ) / 4 ) * ( ( s.dz + 3 ) / 4 ) * molNb;
+        const uint64_t cap   = uint64_t( device_cu_count() ) * 32;
+        return uint32_t( std::max<uint64_t>( 1, std::min( ( waves + 3 ) / 4, cap ) ) );
     }
 
     uint32_t grid_blocks( uint64_t n )
@@ -543,7 +1082,7 @@ extern "C"
         sesdf_status st = sdf_common( h, s, flags );
         if ( st != SESDF_OK || npoints == 0 )
             return st;
-        sdf_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+        sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -586,7 +1125,7 @@ extern "C"
         sesdf_status st = sdf_common( h, s, flags );
         if ( st != SESDF_OK || s.n == 0 )
             return st;
-        sdf_kernel<<<grid_blocks( s.n ), 256, 0, h.stream>>>( h.b, s );
+        sdf_grid_kernel<<<brick_blocks( s, 1 ), 256, 0, h.stream>>>( h.b, s, 1u );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -631,7 +1170,7 @@ extern "C"
         sesdf_status st = sdf_common( h, s, 0 );
         if ( st != SESDF_OK )
             return st;
-        sdf_kernel<<<grid_blocks( s.n ), 256, 0, h.stream>>>( h.b, s );
+        sdf_grid_kernel<<<brick_blocks( s, uint32_t( M ) ), 256, 0, h.stream>>>( h.b, s, uint32_t( M ) );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -675,15 +1214,15 @@ extern "C"
         if ( !hh )
             return SESDF_ERR_INVALID;
         Handle & h = *reinterpret_cast<Handle *>( hh );
-        if ( !h.sdf.arcs )
+        if ( !h.sdf.gradScratch )
         {
             void * p = nullptr;
             if ( hipMalloc( &p, std::max<size_t>( size_t( h.n ) * 16, 16 ) ) != hipSuccess )
                 return SESDF_ERR_HIP;
             h.sdfAllocs.push_back( p );
             h.sdfBytes += size_t( h.n ) * 16;
-            h.sdf.arcs = static_cast<float4 *>( p ); // scratch: gradient in sorted order
+            h.sdf.gradScratch = static_cast<float4 *>( p );
         }
-        return sesdf_sdf_backward_ex( hh, d_points, d_weights, npoints, band, 0, nullptr, reinterpret_cast<float *>( h.sdf.arcs ), d_grad );
+        return sesdf_sdf_backward_ex( hh, d_points, d_weights, npoints, band, 0, nullptr, reinterpret_cast<float *>( h.sdf.gradScratch ), d_grad );
     }
 }

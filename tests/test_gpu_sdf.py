@@ -64,13 +64,22 @@ def test_surface_components_match_oracle(gpu):
     assert ext_g == ext_o
 
 
-@pytest.mark.parametrize("exterior", [False, True])
-def test_sdf_points_match_fp64(gpu, exterior):
+def spatial_order(pts, edge=2.0):
+    """Points sorted by a coarse cell key: waves of consecutive points are spatially compact."""
+    key = np.floor((pts - pts.min(0)) / edge).astype(np.int64)
+    return pts[np.lexsort((key[:, 0], key[:, 1], key[:, 2]))]
+
+
+@pytest.mark.parametrize("exterior,order", [(False, "random"), (True, "random"), (False, "sorted"), (True, "sorted")])
+def test_sdf_points_match_fp64(gpu, exterior, order):
+    """Random order: lanes of a wave evaluate alone; sorted: waves evaluate cooperatively."""
     atoms = synth.globule(3000, synth.SEED_BASE + 60)
     surf = Sesdf(atoms)
     orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
     band = 2.0
     pts = near_surface_points(atoms, orc, 40000, band, 3)
+    if order == "sorted":
+        pts = spatial_order(pts)
     g = gpu_sdf(surf, pts, band, exterior)
     o = orc.sdf(pts.astype(np.float64), band=band, exterior=exterior, threads=8)
     err = np.abs(g.astype(np.float64) - o)
@@ -78,7 +87,8 @@ def test_sdf_points_match_fp64(gpu, exterior):
     assert err.max() <= SDF_TOL
 
 
-def test_sdf_grid_64_config0(gpu):
+@pytest.mark.parametrize("exterior", [False, True])
+def test_sdf_grid_64_config0(gpu, exterior):
     """The small workload: 2,000 atoms, SDF on a 64^3 vertex grid over the bounding box + 3 A."""
     torch = _torch()
     atoms = synth.globule(2000, synth.SEED_BASE)
@@ -89,7 +99,7 @@ def test_sdf_grid_64_config0(gpu):
     spacing = (hi - lo) / 63
     out = torch.empty(64 ** 3, device="cuda")
     band = 3.0
-    surf.sample_grid_ptr(lo, spacing, dims, out.data_ptr(), band=band)
+    surf.sample_grid_ptr(lo, spacing, dims, out.data_ptr(), band=band, exterior=exterior)
     torch.cuda.synchronize()
     g = out.cpu().numpy()
     ii = np.arange(64 ** 3)
@@ -98,7 +108,7 @@ def test_sdf_grid_64_config0(gpu):
                     np.float32(lo[1]) + np.float32(spacing[1]) * ((ii // 64) % 64).astype(np.float32),
                     np.float32(lo[2]) + np.float32(spacing[2]) * (ii // 4096).astype(np.float32)], 1)
     orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
-    o = orc.sdf(pts.astype(np.float64), band=band, threads=8)
+    o = orc.sdf(pts.astype(np.float64), band=band, exterior=exterior, threads=8)
     assert np.abs(g - o).max() <= SDF_TOL
     # invariants of the field: 1-Lipschitz along the grid axes, vdW lower bound
     f = g.reshape(64, 64, 64)
