@@ -2,6 +2,7 @@
 
 * ``libsesdf_synth.so`` -- host-only synthetic atom-cloud generator (g++).
 * ``libsesdf_hip.so``   -- the HIP kernels and the C-ABI (hipcc, --offload-arch=gfx950).
+* ``tests/cpp/host_api`` -- a C++ program over the header-only host classes (include/bcs), g++.
 
 Everything is built in-tree so the shared objects travel with the source tree.
 """
@@ -76,6 +77,15 @@ def build_hip(jobs=4):
     return out
 
 
+def build_host_api():
+    """The C++ host-class example (bcs::Sesdf / bcs::Ssesdf over the C ABI), used by the tests."""
+    src = os.path.join(ROOT, "tests", "cpp", "host_api.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "host_api")
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+          "-I", os.path.join(ROOT, "include"), src, "-o", out, "-L", PKG, "-lsesdf_hip", "-Wl,-rpath,$ORIGIN/../../sesdf_amd"])
+    return out
+
+
 def build_oracle():
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
 
@@ -83,9 +93,10 @@ def build_oracle():
 def build_all():
     build_synth()
     build_hip()
+    build_host_api()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["synth", "hip"]
     for w in what:
-        {"synth": build_synth, "hip": build_hip, "oracle": build_oracle}[w]()
+        {"synth": build_synth, "hip": build_hip, "oracle": build_oracle, "host_api": build_host_api}[w]()

@@ -62,7 +62,7 @@ namespace sesdf
 
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t i    = blockIdx.x * WAVES + w;
+        const uint32_t i    = xcd_block() * WAVES + w;
         if ( i >= b.n )
             return;
         const float    rp   = b.probeRadius;
@@ -236,7 +236,7 @@ namespace sesdf
     {
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t i    = blockIdx.x * CirclesWaves + w;
+        const uint32_t i    = xcd_block() * CirclesWaves + w;
         if ( i >= b.n )
             return;
         const uint32_t K    = b.K;

@@ -210,6 +210,17 @@ namespace sesdf
     __device__ __forceinline__ uint32_t stage_tag( const Control * ctl, uint32_t stage ) { return ctl->epoch * 8u + stage; }
 
     // Orders this wave's LDS writes before its later LDS reads of other lanes' values.
+    // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), each XCD with
+    // its own L2. Remapping so that each XCD takes a contiguous range of blocks keeps spatially
+    // close atoms (sorted by cell) and their neighbours in one L2. Blocks past the last full round
+    // keep their index; the mapping is a bijection either way.
+    constexpr uint32_t XcdCount = 8;
+    __device__ __forceinline__ uint32_t xcd_block()
+    {
+        const uint32_t b = blockIdx.x, g = gridDim.x, full = ( g / XcdCount ) * XcdCount;
+        return b < full ? ( b % XcdCount ) * ( g / XcdCount ) + b / XcdCount : b;
+    }
+
     __device__ __forceinline__ void wave_sync()
     {
         __builtin_amdgcn_fence( __ATOMIC_SEQ_CST, "wavefront" );

@@ -108,13 +108,13 @@ namespace sesdf
     {
         __shared__ uint32_t sIdx[ WAVES ][ KMAX ];
         __shared__ float4   sAtom[ WAVES ][ KMAX ];   // (c_l - c_i, R_l^2)
-        __shared__ float    sRad[ WAVES ][ KMAX ];    // vdW radius r_l
+        __shared__ float4   sAbs[ WAVES ][ KMAX ];    // (c_l, r_l) as stored
         __shared__ uint16_t sCand[ WAVES ][ KMAX ];
         __shared__ uint4    sTouch[ WAVES ][ TouchStride ];
 
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t i    = blockIdx.x * WAVES + w;
+        const uint32_t i    = xcd_block() * WAVES + w;
         if ( i >= b.n )
             return;
         const float    rp   = b.probeRadius;
@@ -125,6 +125,7 @@ namespace sesdf
         const float    Ri2  = Ri * Ri;
         const uint32_t ni   = b.nbCount[ i ];
         uint32_t * const stage = b.stage + size_t( i ) * b.stageStride;
+        const bool       truncated = b.ctl->nbOverflowAtoms != 0;
 
         uint32_t m = 0;
         for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
@@ -138,7 +139,7 @@ namespace sesdf
                 const float    Rj = aj.w + rp;
                 sIdx[ w ][ l ]    = j;
                 sAtom[ w ][ l ]   = make_float4( aj.x - ci.x, aj.y - ci.y, aj.z - ci.z, Rj * Rj );
-                sRad[ w ][ l ]    = aj.w;
+                sAbs[ w ][ l ]    = aj;
                 cand              = j > i && b.status[ size_t( i ) * K + l ] == 1;
             }
             const uint64_t mask = ballot( cand );
@@ -175,9 +176,20 @@ namespace sesdf
                 const uint32_t lk = sCand[ w ][ cc ];
                 const uint32_t j  = sIdx[ w ][ la ];
                 const uint32_t k  = sIdx[ w ][ lk ];
-                // k must be a neighbour of j (operations.inl:330-376): binary search in j's ascending list
-                const int jk = find_in_list( b.nbIds + size_t( j ) * K, b.nbCount[ j ], k );
-                if ( jk >= 0 )
+                // k must be a neighbour of j (operations.inl:330-376). Unless some list was truncated
+                // at K, membership is exactly the predicate that built j's list, evaluated here with
+                // the same operands and operations; the slot of k in j's list is found at emission.
+                bool member;
+                if ( truncated )
+                    member = find_in_list( b.nbIds + size_t( j ) * K, b.nbCount[ j ], k ) >= 0;
+                else
+                {
+                    const float4 Aj = sAbs[ w ][ la ], Ak = sAbs[ w ][ lk ];
+                    const f3     d  = xyz( Ak ) - xyz( Aj );
+                    const float  Rs = ( Aj.w + rp ) + ( Ak.w + rp );
+                    member          = len2( d ) - Rs * Rs <= PredicateEps;
+                }
+                if ( member )
                 {
                     const float4    aj  = sAtom[ w ][ la ];
                     const float4    ak  = sAtom[ w ][ lk ];
@@ -196,8 +208,8 @@ namespace sesdf
                         // FP64 re-solve of xc, t and the two probe positions
                         fp64             = true;
                         const double RiD = double( ai.w ) + double( rp );
-                        const double RjD = double( sRad[ w ][ la ] ) + double( rp );
-                        const double RkD = double( sRad[ w ][ lk ] ) + double( rp );
+                        const double RjD = double( sAbs[ w ][ la ].w ) + double( rp );
+                        const double RkD = double( sAbs[ w ][ lk ].w ) + double( rp );
                         const d3     Dj = mkd( dj ), Dk = mkd( dk );
                         const double dj2 = dot( Dj, Dj ), dk2 = dot( Dk, Dk );
                         const d3     cij = Dj * ( ( RiD * RiD - RjD * RjD + dj2 ) / ( 2.0 * dj2 ) );
@@ -266,7 +278,7 @@ namespace sesdf
                             }
                         // hasNeighbors: the probe is closer than rp to the plane of the centres (:447-463)
                         const f3       pn     = unit( cross( ij.normal, unit( dj - dk ) ) );
-                        const uint32_t packed = la | ( lk << 10 ) | ( uint32_t( jk ) << 20 );
+                        const uint32_t packed = la | ( lk << 10 );
                         for ( int s = 0; s < 2; s++ )
                         {
                             if ( !( s == 0 ? vis0 : vis1 ) )
@@ -316,7 +328,7 @@ namespace sesdf
     {
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t i    = blockIdx.x * InterWaves + w;
+        const uint32_t i    = xcd_block() * InterWaves + w;
         if ( i >= b.n )
             return;
         const uint32_t nW   = b.wStart[ i + 1 ] - b.wStart[ i ];
@@ -331,14 +343,20 @@ namespace sesdf
         {
             const uint32_t r    = rb + lane;
             const bool     live = r < nRec;
-            const uint32_t rec  = live ? stage[ r ] : 0u;
+            uint32_t       rec  = live ? stage[ r ] : 0u;
             const bool     isW  = live && ( rec & RecHasNb );
             const uint64_t mW = ballot( isW ), mN = ballot( live && !isW );
             if ( live )
             {
-                const uint32_t la = rec & 1023u, lk = ( rec >> 10 ) & 1023u, jk = ( rec >> 20 ) & 1023u;
+                const uint32_t la = rec & 1023u, lk = ( rec >> 10 ) & 1023u;
                 const uint32_t j  = b.nbIds[ size_t( i ) * K + la ];
                 const uint32_t k  = b.nbIds[ size_t( i ) * K + lk ];
+                // slot of k in j's ascending neighbour list (k is a member: tested at counting)
+                const int      found = find_in_list( b.nbIds + size_t( j ) * K, b.nbCount[ j ], k );
+                const uint32_t jk    = found >= 0 ? uint32_t( found ) : 0u;
+                if ( found < 0 )
+                    atomicAdd( &b.ctl->jkAnomalyNb, 1u );
+                rec |= jk << 20;
                 const uint32_t idx = isW ? runW + rank_below( mW ) : runN + rank_below( mN );
                 if ( idx < b.capRecords )
                     b.records[ isW ? idx : b.capRecords - 1 - idx ] = make_uint4( i, j, k, rec );
@@ -594,7 +612,7 @@ namespace sesdf
     {
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t i    = blockIdx.x * 4 + w;
+        const uint32_t i    = xcd_block() * 4 + w;
         if ( i >= b.n )
             return;
         const uint32_t ni  = b.nbCount[ i ];
@@ -613,7 +631,7 @@ namespace sesdf
     {
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        const uint32_t i    = blockIdx.x * 4 + w;
+        const uint32_t i    = xcd_block() * 4 + w;
         if ( i >= b.n )
             return;
         const uint32_t cnt = b.torusBase[ i + 1 ] - b.torusBase[ i ];
