@@ -84,6 +84,19 @@ def stage_times(surf, reps=10):
     return dict(zip(STAGES, (acc / reps).tolist()))
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (copy-calibrated), or None when the summary is absent."""
+    path = os.path.join(ROOT, "profiles", "topology_kernels_100k_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_dispatch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def intersections_alg_bytes(c, n):
     """Algorithmic HBM bytes of one launch of the intersection kernel: atom i + one gather of its
     neighbours (32 N), neighbour ids + statuses (5 B per ordered circle), the touch masks of the
@@ -106,7 +119,8 @@ def leg_main(args, atoms, out):
         "value": n / (ms * 1e-3), "ms_per_step": ms, "stage_ms": stages,
         "counts": {k: int(v) for k, v in counts.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "intersections_count_kernel",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic("intersections_count_kernel"),
+                     "kernel": "intersections_count_kernel",
                      "alg_bytes_per_launch": int(alg), "kernel_ms": kms},
     })
     log(f"main: {ms:.3f} ms/build, {n / (ms * 1e-3) / 1e6:.1f} M atoms/s")
