@@ -15,7 +15,7 @@
 //                  cases and counted.
 // All indices are in the reference's grid-sorted atom order; sortedToInitial maps them back.
 //
-// Parity pins (tests/test_oracle_*.py): brute-force O(N^2)/O(N^3) enumeration on small inputs,
+// Parity pins (tests/test_oracle.py, tests/test_oracle_sdf.py): brute-force O(N^2)/O(N^3) enumeration on small inputs,
 // closed forms for 1/2/3/4-atom configurations, and the invariants sum(per-circle counts) = 3I,
 // even per-circle counts, G = 3I/2, status(ij) = status(ji), i<j<k in every concave id.
 #include <algorithm>

@@ -2,11 +2,12 @@
 //
 //   intersections_count_kernel <- findIntersectionsBetweenCircles (sesdf/include/bcs/sesdf/operations.inl:230-486;
 //                            ssesdf/include/bcs/ssesdf/operation.inl:100-351): one wave per atom i,
-//                            candidate pairs (ij, ik) flattened over the lanes, N(j) membership by
-//                            binary search in j's ascending list, occluders N(i) from LDS; records
-//                            staged per atom (4 bytes each).
-//   intersections_emit_kernel <- the |w| sort of findIntersections (operations.inl:128-141) and
-//                            the per-circle counts; offsets from a scan of per-atom counts.
+//                            candidate pairs (ij, ik) flattened over the lanes, k in N(j) by the
+//                            predicate that built j's list (binary search only after a truncation),
+//                            occluders N(i) from LDS; records staged per atom (4 bytes each).
+//   intersections_emit_kernel <- the |w| sort of findIntersections (operations.inl:128-141), the
+//                            slot of k in j's list and the per-circle counts; offsets from a scan of
+//                            per-atom counts.
 //   launch_circle_scan    <- the per-circle offset scans (operations.inl:149-156, 500-517)
 //   fill_kernel           <- fillIntersections (sesdf/src/sesdf/operations.cu:206-269;
 //                            ssesdf/src/ssesdf/operations.cu:287-329)
