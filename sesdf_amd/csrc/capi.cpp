@@ -236,6 +236,7 @@ namespace sesdf
     sesdf_status enqueue( Handle & h )
     {
         HIP_TRY( hipSetDevice( h.device ) );
+        h.buildSerial++; // outputs of the previous build are stale from here on
         if ( h.n == 0 )
             return SESDF_OK;
         if ( h.useGraph )
@@ -644,6 +645,7 @@ extern "C"
                 HIP_TRY( hipEventCreate( &e ) );
         if ( h.n == 0 )
             return SESDF_OK;
+        h.buildSerial++;
         enqueue_stages( h, h.stream, h.events );
         HIP_TRY( hipGetLastError() );
         sesdf_status st = readback( h );

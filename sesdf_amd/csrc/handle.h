@@ -13,13 +13,13 @@ namespace sesdf
         uint32_t sectors = 0, full = 0, trimmed = 0, records = 0, lists = 0, segments = 0, probeNb = 0, torus = 0;
     };
 
-    // Device state of the SDF evaluator (allocated on first use, refreshed per build epoch).
+    // Device state of the SDF evaluator (allocated on first use, refreshed once per build).
     struct SdfState
     {
-        uint32_t   builtEpoch  = 0;
+        uint64_t   builtSerial = ~0ull;   // build the component labels belong to
         uint32_t * component   = nullptr;  // N: surface component label (exterior classification)
         float4 *   gradScratch = nullptr;  // N: gradient in sorted order (backward without a caller buffer)
-        float4 *   packAtoms     = nullptr; // N: centre, R (rebuilt once per build epoch)
+        float4 *   packAtoms     = nullptr; // N: centre, R
         float4 *   surfAtoms     = nullptr; // surface atoms, compacted in cell order
         int4 *     surfRanges    = nullptr; // 2 per surface atom: element ranges
         uint32_t * surfIds       = nullptr; // per surface atom: sorted index | face flag << 31
@@ -32,7 +32,7 @@ namespace sesdf
         float4 *   geo           = nullptr; // caps (cap.sectors) then segment geometry (3 per segment)
         uint32_t   geoSectors    = 0;
         uint32_t   geoSegments   = 0;
-        uint32_t   packEpoch     = 0xffffffffu;
+        uint64_t   packSerial    = ~0ull; // build the packed / surface arrays belong to
         float4 *   molGrids    = nullptr;  // batches: per-molecule grid origin / spacing
         uint32_t   molGridDim  = 0;
         float      molGridMargin = 0.f;
@@ -54,6 +54,7 @@ namespace sesdf
         bool        built     = false;
         bool        inputValid = false;
         uint32_t    rebuilds  = 0;
+        uint64_t    buildSerial = 0;     // stage sequences enqueued so far (keys the SDF caches)
         uint32_t    stageStride = 32;    // staged concave records per atom (grows on overflow)
         uint32_t    wavesCircles = 4;    // block shapes (autotuned)
         uint32_t    wavesInter   = 4;

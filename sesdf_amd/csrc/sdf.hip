@@ -100,18 +100,6 @@ namespace sesdf
         return t < 0.f ? t + TwoPi : t;
     }
 
-    // Distance from q to the box of cells [x0, x1] x {y} x {z} of molecule brick M.
-    __device__ __forceinline__ float row_min_dist( const Molecule & M, f3 q, int x0, int x1, int y, int z )
-    {
-        const float bx0 = M.ox + M.cell * float( x0 ), bx1 = M.ox + M.cell * float( x1 + 1 );
-        const float by0 = M.oy + M.cell * float( y ), by1 = by0 + M.cell;
-        const float bz0 = M.oz + M.cell * float( z ), bz1 = bz0 + M.cell;
-        const float dx  = fmaxf( fmaxf( bx0 - q.x, q.x - bx1 ), 0.f );
-        const float dy  = fmaxf( fmaxf( by0 - q.y, q.y - by1 ), 0.f );
-        const float dz  = fmaxf( fmaxf( bz0 - q.z, q.z - bz1 ), 0.f );
-        return sqrtf( dx * dx + dy * dy + dz * dz );
-    }
-
     // In a wave-cooperative evaluation (COOP) all lanes walk the same cells and atoms, so a test
     // decides for the whole wave; otherwise every lane walks its own stencil.
     template<bool COOP>
@@ -143,6 +131,25 @@ namespace sesdf
         return v;
     }
 
+    __device__ __forceinline__ float wave_min_float( float v )
+    {
+        for ( int o = 32; o > 0; o >>= 1 )
+            v = fminf( v, __shfl_xor( v, o, WaveSize ) );
+        return v;
+    }
+    __device__ __forceinline__ float wave_max_float( float v )
+    {
+        for ( int o = 32; o > 0; o >>= 1 )
+            v = fmaxf( v, __shfl_xor( v, o, WaveSize ) );
+        return v;
+    }
+
+    // Bounding box of the wave's lanes that search surface elements (pass B).
+    struct LaneBox
+    {
+        f3 lo, hi;
+    };
+
     struct SdfLane
     {
         f3       q;
@@ -160,29 +167,47 @@ namespace sesdf
         int bx0, bx1, by0, by1, bz0, bz1; // near block
     };
 
+    // Gap between coordinate v and the cell slab [lo, lo + cell] along one axis.
+    __device__ __forceinline__ float axis_gap( float lo, float cell, float v ) { return fmaxf( fmaxf( lo - v, v - ( lo + cell ) ), 0.f ); }
+
+    // Row segments of the stencil as fn(xa, xb, y, z, ord, dyz2): the near block first (ord = the
+    // row's ordinal within the block), then the rest (ord = -1). dyz2 is the squared y/z gap from q
+    // to the row, computed once per row for the caller's distance bound.
     template<class F>
-    __device__ __forceinline__ void for_block( const Stencil & t, F && fn )
+    __device__ __forceinline__ void for_block( const Stencil & t, const Molecule & M, f3 q, F && fn )
     {
+        int ord = 0;
         for ( int z = t.bz0; z <= t.bz1; z++ )
-            for ( int y = t.by0; y <= t.by1; y++ )
-                fn( t.bx0, t.bx1, y, z );
+        {
+            const float gz = axis_gap( M.oz + M.cell * float( z ), M.cell, q.z );
+            for ( int y = t.by0; y <= t.by1; y++, ord++ )
+            {
+                const float gy = axis_gap( M.oy + M.cell * float( y ), M.cell, q.y );
+                fn( t.bx0, t.bx1, y, z, ord, gz * gz + gy * gy );
+            }
+        }
     }
     template<class F>
-    __device__ __forceinline__ void for_rest( const Stencil & t, F && fn )
+    __device__ __forceinline__ void for_rest( const Stencil & t, const Molecule & M, f3 q, F && fn )
     {
         for ( int z = t.z0; z <= t.z1; z++ )
+        {
+            const float gz = axis_gap( M.oz + M.cell * float( z ), M.cell, q.z );
             for ( int y = t.y0; y <= t.y1; y++ )
             {
+                const float gy   = axis_gap( M.oy + M.cell * float( y ), M.cell, q.y );
+                const float dyz2 = gz * gz + gy * gy;
                 if ( z >= t.bz0 && z <= t.bz1 && y >= t.by0 && y <= t.by1 )
                 {
                     if ( t.x0 < t.bx0 )
-                        fn( t.x0, t.bx0 - 1, y, z );
+                        fn( t.x0, t.bx0 - 1, y, z, -1, dyz2 );
                     if ( t.bx1 < t.x1 )
-                        fn( t.bx1 + 1, t.x1, y, z );
+                        fn( t.bx1 + 1, t.x1, y, z, -1, dyz2 );
                 }
                 else
-                    fn( t.x0, t.x1, y, z );
+                    fn( t.x0, t.x1, y, z, -1, dyz2 );
             }
+        }
     }
 
     // ---- primitive element tests -------------------------------------------------------------
@@ -303,6 +328,7 @@ namespace sesdf
     __device__ __forceinline__ int4     rl( int4 v, int j ) { return make_int4( rl( v.x, j ), rl( v.y, j ), rl( v.z, j ), rl( v.w, j ) ); }
 
     constexpr int StageSize = 384; // float4 per wave: 64 segments (4 each) + 64 + 64 vertices, or caps
+    constexpr int MaskRows  = 32;  // near-block rows whose element-gate masks are kept (2 chunks each)
 
     __device__ __forceinline__ void coop_nearest( const SdfArgs & s, SdfLane & L, uint32_t a0, uint32_t a1, uint32_t extC )
     {
@@ -327,12 +353,21 @@ namespace sesdf
         }
     }
 
+    // Surface atoms [k0, k1) of one row segment, 64 per chunk: arcs and vertices (elems), then faces
+    // (faces). A face can win only where the element gate passed (|sd| < best, and best only
+    // decreases), so faces are tested for the atoms of the chunk's element-gate mask: from the
+    // element loop just run, or, for near-block rows (ord >= 0) whose elements ran in an earlier
+    // pass, from the wave's mask store.
     __device__ __forceinline__ void coop_row( const Buffers & b, const SdfArgs & s, SdfLane & L, bool need, uint32_t k0, uint32_t k1, uint32_t extC,
-                                              uint32_t Iw, bool elems, bool faces, float4 * stage )
+                                              uint32_t Iw, bool elems, bool faces, int ord, float4 * stage, uint64_t * masks,
+                                              const LaneBox & box )
     {
         const uint32_t lane = uint32_t( lane_id() );
         for ( uint32_t base = k0; base < k1; base += WaveSize )
         {
+            const uint32_t chunk    = ( base - k0 ) / WaveSize;
+            const bool     keepMask = ord >= 0 && ord < MaskRows && chunk < 2;
+            uint64_t       passMask = 0;
             const uint32_t k      = base + lane;
             const bool     ok     = k < k1;
             const float4   myAtom = ok ? s.surfAtoms[ k ] : make_float4( 0.f, 0.f, 0.f, 0.f );
@@ -342,6 +377,17 @@ namespace sesdf
             const uint32_t last   = min( k, k1 - 1 );
             if ( elems )
             {
+                // Chunk prefilter, one atom per lane: the shell |d - R| < best of some searching lane
+                // must meet the box of those lanes (largest best, nearest / farthest box point).
+                const float bestMax = wave_max_float( need ? L.best : 0.f );
+                const f3    c       = xyz( myAtom );
+                const float nx = fmaxf( fmaxf( box.lo.x - c.x, c.x - box.hi.x ), 0.f ), ny = fmaxf( fmaxf( box.lo.y - c.y, c.y - box.hi.y ), 0.f ),
+                            nz = fmaxf( fmaxf( box.lo.z - c.z, c.z - box.hi.z ), 0.f );
+                const float fx = fmaxf( fabsf( c.x - box.lo.x ), fabsf( c.x - box.hi.x ) ), fy = fmaxf( fabsf( c.y - box.lo.y ), fabsf( c.y - box.hi.y ) ),
+                            fz = fmaxf( fabsf( c.z - box.lo.z ), fabsf( c.z - box.hi.z ) );
+                const float dNear = fsqrt( nx * nx + ny * ny + nz * nz ), dFar = fsqrt( fx * fx + fy * fy + fz * fz );
+                // 1e-4 A of slack covers the rounding of the two bounds
+                uint64_t pre = ballot( ok && myExt && dNear - myAtom.w < bestMax + 1e-4f && myAtom.w - dFar < bestMax + 1e-4f );
                 const int4 myR0 = s.surfRanges[ 2 * last ], myR1 = s.surfRanges[ 2 * last + 1 ];
                 // the chunk's atoms are consecutive in sorted order, so the records they own are
                 // contiguous; staged in LDS (up to 64 of each kind) on the first atom that needs them
@@ -350,13 +396,14 @@ namespace sesdf
                 const int  n0 = int( Iw ) + rl( myR1.z, 0 ), n1 = int( Iw ) + rl( myR1.w, cnt - 1 );
                 const bool segIn = s1 - s0 <= WaveSize, wIn = w1 - w0 <= WaveSize, nIn = n1 - n0 <= WaveSize;
                 bool       staged = false;
-                for ( int j = 0; j < cnt; j++ )
+                while ( pre )
                 {
-                    if ( !rl( myExt, j ) )
-                        continue;
+                    const int j = __ffsll( (unsigned long long)pre ) - 1;
+                    pre &= pre - 1;
                     const float4 aa = rl( myAtom, j );
                     if ( !any_lane<true>( need && fabsf( fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w ) < L.best ) )
                         continue;
+                    passMask |= uint64_t( 1 ) << j;
                     if ( !staged )
                     {
                         wave_sync();
@@ -383,22 +430,38 @@ namespace sesdf
                         vertex_test( L, nIn ? stage[ 320 + v - n0 ] : b.concavePos[ v ], uint32_t( v ) );
                 }
             }
+            if ( elems && !faces && keepMask )
+            {
+                if ( lane == 0 )
+                    masks[ 2 * ord + chunk ] = passMask;
+                wave_sync();
+            }
             if ( faces )
             {
+                uint64_t cand = ~uint64_t( 0 );
+                if ( elems )
+                    cand = passMask;
+                else if ( keepMask )
+                {
+                    const uint64_t m = masks[ 2 * ord + chunk ];
+                    cand             = uint64_t( uint32_t( __builtin_amdgcn_readfirstlane( int( uint32_t( m ) ) ) ) ) |
+                           ( uint64_t( uint32_t( __builtin_amdgcn_readfirstlane( int( uint32_t( m >> 32 ) ) ) ) ) << 32 );
+                }
+                cand &= ballot( ok && ( myId >> 31 ) && myExt );
                 const int2 myCaps = s.surfCaps[ last ];
                 const int  c0 = rl( myCaps.x, 0 ), c1 = rl( myCaps.y, cnt - 1 );
                 const bool capIn  = c1 - c0 <= StageSize;
                 bool       staged = false;
-                for ( int j = 0; j < cnt; j++ )
+                while ( cand )
                 {
-                    const uint32_t id = rl( myId, j );
-                    if ( !( id >> 31 ) || !rl( myExt, j ) )
-                        continue;
-                    const float4 aa      = rl( myAtom, j );
-                    const f3     dq      = L.q - xyz( aa );
-                    const float  dist    = fsqrt( len2( dq ) );
-                    const float  sd      = dist - aa.w;
-                    bool         exposed = need && fabsf( sd ) < L.best && dist > 0.f;
+                    const int j = __ffsll( (unsigned long long)cand ) - 1;
+                    cand &= cand - 1;
+                    const uint32_t id      = rl( myId, j );
+                    const float4   aa      = rl( myAtom, j );
+                    const f3       dq      = L.q - xyz( aa );
+                    const float    dist    = fsqrt( len2( dq ) );
+                    const float    sd      = dist - aa.w;
+                    bool           exposed = need && fabsf( sd ) < L.best && dist > 0.f;
                     if ( !any_lane<true>( exposed ) )
                         continue;
                     if ( capIn && !staged )
@@ -438,7 +501,8 @@ namespace sesdf
     //     their distance; further rows are nearly always culled or cheap.
     // A row segment is skipped when the distance bound of its box cannot change the outcome.
     template<bool COOP>
-    __device__ float sdf_eval_lanes( const Buffers & b, const SdfArgs & s, uint32_t mol, SdfLane & L, uint64_t & code, float4 * stage )
+    __device__ float sdf_eval_lanes( const Buffers & b, const SdfArgs & s, uint32_t mol, SdfLane & L, uint64_t & code, float4 * stage,
+                                     uint64_t * masks )
     {
         const float    rp    = b.probeRadius;
         const float    Rmax  = __uint_as_float( b.ctl->radiusMaxBits ) + rp;
@@ -464,16 +528,20 @@ namespace sesdf
         t.bx0 = max( cx0 - 1, 0 ), t.bx1 = min( cx1 + 1, M.dx - 1 );
         t.by0 = max( cy0 - 1, 0 ), t.by1 = min( cy1 + 1, M.dy - 1 );
         t.bz0 = max( cz0 - 1, 0 ), t.bz1 = min( cz1 + 1, M.dz - 1 );
-        // lower bound of |q - c_a| - R_a over a row segment (1e-3 A covers the rounding of the
-        // cell assignment)
-        auto lower = [ & ]( int xa, int xb, int y, int z ) { return row_min_dist( M, L.q, xa, xb, y, z ) - Rmax - 1e-3f; };
+        // lower bound of |q - c_a| - R_a over a row segment [xa, xb] (1e-3 A covers the rounding of
+        // the cell assignment)
+        auto lower = [ & ]( int xa, int xb, float dyz2 ) {
+            const float bx0 = M.ox + M.cell * float( xa ), bx1 = M.ox + M.cell * float( xb + 1 );
+            const float gx  = fmaxf( fmaxf( bx0 - L.q.x, L.q.x - bx1 ), 0.f );
+            return fsqrt( gx * gx + dyz2 ) - Rmax - 1e-3f;
+        };
 
         // pass A. Once g < 0 the point is inside and g no longer matters. Otherwise a segment can
         // change the outcome only below g + 1e-4 (exterior test) or, for the complete variant,
         // below min(g, max(band - rp, 0)) (above, the value is clamped to +band).
         L.g = D, L.gExt = 3.0e38f, L.gA = 0xffffffffu;
-        auto nearest = [ & ]( int xa, int xb, int y, int z ) {
-            const float lb   = lower( xa, xb, y, z );
+        auto nearest = [ & ]( int xa, int xb, int y, int z, int, float dyz2 ) {
+            const float lb   = lower( xa, xb, dyz2 );
             const float gCut = s.exterior ? L.g + 1e-4f : fminf( L.g, fmaxf( s.band - rp, 0.f ) );
             if ( !any_lane<COOP>( L.active && L.g >= 0.f && lb <= gCut ) )
                 return;
@@ -484,8 +552,8 @@ namespace sesdf
             else
                 lane_nearest( s, L, a0, a1, extC );
         };
-        for_block( t, nearest );
-        for_rest( t, nearest );
+        for_block( t, M, L.q, nearest );
+        for_rest( t, M, L.q, nearest );
 
         // 0: +band (nothing within the band), 1: outside A, 2: inside (pass B)
         int state = 0;
@@ -495,19 +563,27 @@ namespace sesdf
         L.best = D, L.bc = code_of( SdfClamped, 0 );
         if ( any_lane<COOP>( needB ) )
         {
-            auto surface = [ & ]( int xa, int xb, int y, int z, bool elems, bool faces ) {
-                if ( !any_lane<COOP>( needB && lower( xa, xb, y, z ) < L.best ) )
+            LaneBox box {};
+            if constexpr ( COOP )
+            {
+                box.lo = mk( wave_min_float( needB ? L.q.x : 3.0e38f ), wave_min_float( needB ? L.q.y : 3.0e38f ),
+                             wave_min_float( needB ? L.q.z : 3.0e38f ) );
+                box.hi = mk( wave_max_float( needB ? L.q.x : -3.0e38f ), wave_max_float( needB ? L.q.y : -3.0e38f ),
+                             wave_max_float( needB ? L.q.z : -3.0e38f ) );
+            }
+            auto surface = [ & ]( int xa, int xb, int y, int z, int ord, float dyz2, bool elems, bool faces ) {
+                if ( !any_lane<COOP>( needB && lower( xa, xb, dyz2 ) < L.best ) )
                     return;
                 const uint32_t h0 = cell_index( M, xa, y, z );
                 const uint32_t k0 = s.surfCellStart[ h0 ], k1 = s.surfCellStart[ h0 + uint32_t( xb - xa ) + 1 ];
                 if constexpr ( COOP )
-                    coop_row( b, s, L, needB, k0, k1, extC, Iw, elems, faces, stage );
+                    coop_row( b, s, L, needB, k0, k1, extC, Iw, elems, faces, ord, stage, masks, box );
                 else
                     lane_row( b, s, L, k0, k1, extC, Iw, elems, faces );
             };
-            for_block( t, [ & ]( int xa, int xb, int y, int z ) { surface( xa, xb, y, z, true, false ); } );
-            for_block( t, [ & ]( int xa, int xb, int y, int z ) { surface( xa, xb, y, z, false, true ); } );
-            for_rest( t, [ & ]( int xa, int xb, int y, int z ) { surface( xa, xb, y, z, true, true ); } );
+            for_block( t, M, L.q, [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, true, false ); } );
+            for_block( t, M, L.q, [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, false, true ); } );
+            for_rest( t, M, L.q, [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, true, true ); } );
         }
 
         if ( state == 0 )
@@ -530,7 +606,7 @@ namespace sesdf
         SdfLane L;
         L.q      = q;
         L.active = true;
-        return sdf_eval_lanes<false>( b, s, mol, L, code, nullptr );
+        return sdf_eval_lanes<false>( b, s, mol, L, code, nullptr, nullptr );
     }
 
     // Per sorted atom: centre and R; surface flag (owns an arc or vertex, or its face may be
@@ -606,18 +682,42 @@ namespace sesdf
         }
     }
 
-    // Caps of each atom, largest first (a covered direction is then found early).
-    __global__ void sdf_sort_caps_kernel( Buffers b, float4 * caps )
+    // Caps of each atom, largest first (smallest cosine; a covered direction is then found early).
+    // One wave per atom: up to 64 caps are ranked in registers (key, then index: a total order, so
+    // ranks are distinct) and written to their places; longer lists are insertion-sorted by lane 0.
+    // Caps of non-existent circles (contained spheres) carry a NaN cosine and are ranked last.
+    __global__ void __launch_bounds__( 256 ) sdf_sort_caps_kernel( Buffers b, float4 * caps )
     {
-        const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t a    = xcd_block() * 4 + threadIdx.x / WaveSize;
+        const int      lane = lane_id();
         if ( a >= b.n )
             return;
         const uint32_t c0 = b.vStart[ a ], c1 = min( b.vStart[ a + 1 ], b.capSectors );
+        const int      n  = int( c1 > c0 ? c1 - c0 : 0u );
+        auto           key = []( float w ) { return w == w ? w : 3.0f; };
+        if ( n <= WaveSize )
+        {
+            const bool   live = lane < n;
+            const float4 mine = live ? caps[ c0 + lane ] : make_float4( 0.f, 0.f, 0.f, 0.f );
+            const float  k    = key( mine.w );
+            int          rank = 0;
+            for ( int m = 0; m < n; m++ )
+            {
+                const float km = __shfl( k, m, WaveSize );
+                rank += ( km < k || ( km == k && m < lane ) ) ? 1 : 0;
+            }
+            wave_sync(); // every lane has read its cap before any is overwritten
+            if ( live )
+                caps[ c0 + rank ] = mine;
+            return;
+        }
+        if ( lane != 0 )
+            return;
         for ( uint32_t i = c0 + 1; i < c1; i++ )
         {
             const float4 x = caps[ i ];
             uint32_t     j = i;
-            for ( ; j > c0 && caps[ j - 1 ].w > x.w; j-- )
+            for ( ; j > c0 && key( caps[ j - 1 ].w ) > key( x.w ); j-- )
                 caps[ j ] = caps[ j - 1 ];
             caps[ j ] = x;
         }
@@ -627,7 +727,8 @@ namespace sesdf
     // cooperatively, any other wave lane by lane.
     __global__ void __launch_bounds__( 256 ) sdf_points_kernel( Buffers b, SdfArgs s )
     {
-        __shared__ float4 stage[ 256 / WaveSize ][ StageSize ];
+        __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
+        __shared__ uint64_t masks[ 256 / WaveSize ][ 2 * MaskRows ];
         const uint64_t stride = uint64_t( gridDim.x ) * blockDim.x;
         for ( uint64_t base = uint64_t( blockIdx.x ) * blockDim.x; base < s.n; base += stride )
         {
@@ -648,9 +749,9 @@ namespace sesdf
             if ( !any_lane<true>( L.active ) )
                 continue;
             if ( ex <= 2 && ey <= 2 && ez <= 2 )
-                v = sdf_eval_lanes<true>( b, s, 0, L, code, stage[ threadIdx.x / WaveSize ] );
+                v = sdf_eval_lanes<true>( b, s, 0, L, code, stage[ threadIdx.x / WaveSize ], masks[ threadIdx.x / WaveSize ] );
             else if ( L.active )
-                v = sdf_eval_lanes<false>( b, s, 0, L, code, nullptr );
+                v = sdf_eval_lanes<false>( b, s, 0, L, code, nullptr, nullptr );
             if ( L.active )
             {
                 s.out[ q ] = v;
@@ -663,7 +764,8 @@ namespace sesdf
     // Regular grids: one wave per 4 x 4 x 4 brick of grid vertices (batches: per molecule grid).
     __global__ void __launch_bounds__( 256, 4 ) sdf_grid_kernel( Buffers b, SdfArgs s, uint32_t molNb )
     {
-        __shared__ float4 stage[ 256 / WaveSize ][ StageSize ];
+        __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
+        __shared__ uint64_t masks[ 256 / WaveSize ][ 2 * MaskRows ];
         const uint32_t nbx = ( s.dx + 3 ) / 4, nby = ( s.dy + 3 ) / 4, nbz = ( s.dz + 3 ) / 4;
         const uint64_t bricks = uint64_t( nbx ) * nby * nbz, total = bricks * molNb;
         const uint64_t per    = uint64_t( s.dx ) * s.dy * s.dz;
@@ -682,7 +784,7 @@ namespace sesdf
             uint32_t       m;
             L.q = query_point( s, L.active ? q : uint64_t( mol ) * per, m );
             uint64_t    code;
-            const float v = sdf_eval_lanes<true>( b, s, mol, L, code, stage[ threadIdx.x / WaveSize ] );
+            const float v = sdf_eval_lanes<true>( b, s, mol, L, code, stage[ threadIdx.x / WaveSize ], masks[ threadIdx.x / WaveSize ] );
             if ( L.active )
             {
                 s.out[ q ] = v;
@@ -902,7 +1004,7 @@ namespace
 {
     sesdf_status prepare_components( Handle & h )
     {
-        if ( h.sdf.component && h.sdf.builtEpoch == h.hctl->epoch )
+        if ( h.sdf.component && h.sdf.builtSerial == h.buildSerial )
             return SESDF_OK;
         const uint32_t n    = h.n;
         const size_t   nPad = ( size_t( n ) + 2 + 1 ) & ~size_t( 1 ); // 8-byte aligned slot for the argmax key
@@ -939,16 +1041,16 @@ namespace
         comp_compress_kernel<<<( n + 255 ) / 256, 256, 0, s>>>( parent, n, best, extComp );
         if ( hipGetLastError() != hipSuccess )
             return SESDF_ERR_HIP;
-        h.sdf.builtEpoch = h.hctl->epoch;
+        h.sdf.builtSerial = h.buildSerial;
         return SESDF_OK;
     }
 
-    // Packed atoms and the compacted surface atoms of the current build (once per build epoch).
+    // Packed atoms and the compacted surface atoms of the current build (once per build).
     sesdf_status prepare_surface( Handle & h )
     {
         SdfState &     d = h.sdf;
         const uint32_t n = h.n;
-        if ( d.packAtoms && d.packEpoch == h.hctl->epoch )
+        if ( d.packAtoms && d.packSerial == h.buildSerial )
             return SESDF_OK;
         if ( !d.packAtoms )
         {
@@ -1004,14 +1106,15 @@ namespace
         const uint32_t items = std::max( n, h.cellNb + 1 );
         sdf_surface_kernel<<<( items + 255 ) / 256, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo, d.surfPrefix, d.surfAtoms,
                                                                  d.surfRanges, d.surfIds, d.surfCaps, d.surfCellStart );
-        const uint32_t geoItems = std::max( std::min( h.hctl->sectorNb, h.cap.sectors ), std::min( h.hctl->segmentNb, h.cap.segments ) );
+        // sized by the capacities: the kernel reads the counts of the build from the device
+        const uint32_t geoItems = std::max( h.cap.sectors, h.cap.segments );
         if ( geoItems )
             sdf_geometry_kernel<<<( geoItems + 255 ) / 256, 256, 0, s>>>( h.b, d.geo, d.geo + d.geoSectors );
         if ( n )
-            sdf_sort_caps_kernel<<<( n + 255 ) / 256, 256, 0, s>>>( h.b, d.geo );
+            sdf_sort_caps_kernel<<<( n + 3 ) / 4, 256, 0, s>>>( h.b, d.geo );
         if ( hipGetLastError() != hipSuccess )
             return SESDF_ERR_HIP;
-        d.packEpoch = h.hctl->epoch;
+        d.packSerial = h.buildSerial;
         return SESDF_OK;
     }
 

@@ -173,3 +173,36 @@ def test_torch_autograd_function(gpu):
     fd, stable = orc.sdf_fd(pts.astype(np.float64), w.cpu().numpy().astype(np.float64), ids, h=1e-4, band=2.0)
     rel = np.abs(fd - g_store[ids]).max(1) / np.abs(ref).max()
     assert np.all(rel[stable] <= GRAD_TOL)
+
+
+def test_sampling_after_enqueue_uses_the_new_build(gpu):
+    """A build launched without readback (sesdf_enqueue) on new atoms in the input buffer: the next
+    sampling sees the new surface, not the per-build data prepared for the previous one."""
+    import ctypes
+
+    import sesdf_amd._lib as L
+
+    torch = _torch()
+    atoms = synth.globule(1500, synth.SEED_BASE + 62)
+    moved = atoms.copy()
+    moved[:, :3] += np.array([0.37, -0.21, 0.55], np.float32)
+    surf = Sesdf(atoms)
+    lo = atoms[:, :3].min(0) - 6
+    hi = atoms[:, :3].max(0) + 6
+    dims = (40, 40, 40)
+    sp = (hi - lo) / 39
+    out = torch.empty(40 ** 3, device="cuda")
+    surf.sample_grid_ptr(lo, sp, dims, out.data_ptr(), band=2.0)  # prepares data for the first build
+    buf = ctypes.c_void_p()
+    L.check(L.lib().sesdf_input_buffer(surf._h, ctypes.byref(buf)), "input_buffer")
+    L.memcpy(buf, moved.ctypes.data, moved.nbytes, 1)  # host to device
+    L.check(L.lib().sesdf_enqueue(surf._h), "enqueue")
+    surf.sample_grid_ptr(lo, sp, dims, out.data_ptr(), band=2.0)
+    torch.cuda.synchronize()
+    L.check(L.lib().sesdf_sync(surf._h), "sync")
+    fresh = Sesdf(moved)
+    ref = torch.empty(40 ** 3, device="cuda")
+    fresh.sample_grid_ptr(lo, sp, dims, ref.data_ptr(), band=2.0)
+    torch.cuda.synchronize()
+    # same surface; the two handles differ only in grid layout and sorted order (rounding)
+    assert (out - ref).abs().max().item() <= 1e-5
