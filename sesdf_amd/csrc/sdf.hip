@@ -613,21 +613,20 @@ namespace sesdf
     // exposed) for the compaction scan. A face can be exposed only if the atom has a visible
     // circle, or no neighbour cuts a cap off its sphere: otherwise the exposed part would be
     // bounded by arcs of visible circles.
-    __global__ void sdf_pack_kernel( Buffers b, float4 * pa, uint32_t * surfFlag, uint32_t * info )
+    // One wave per atom, lanes over its neighbours.
+    __global__ void __launch_bounds__( 256 ) sdf_pack_kernel( Buffers b, float4 * pa, uint32_t * surfFlag, uint32_t * info )
     {
-        const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+        const uint32_t a    = xcd_block() * 4 + threadIdx.x / WaveSize;
+        const uint32_t lane = uint32_t( lane_id() );
         if ( a >= b.n )
             return;
         const float4 at = b.atoms[ a ];
         const float  Ra = at.w + b.probeRadius;
-        pa[ a ]         = make_float4( at.x, at.y, at.z, Ra );
-        const bool elems = b.fullBase[ a + 1 ] > b.fullBase[ a ] || b.segStart[ b.trimBase[ a + 1 ] ] > b.segStart[ b.trimBase[ a ] ] ||
-                           b.wStart[ a + 1 ] > b.wStart[ a ] || b.nStart[ a + 1 ] > b.nStart[ a ];
         // inside: the sphere lies in a neighbour's ball; cap: some neighbour robustly cuts a
         // non-empty cap (the caps of visible circles then bound any exposed part)
         bool           inside = false, cap = false;
         const uint32_t na     = b.nbCount[ a ];
-        for ( uint32_t l = 0; l < na; l++ )
+        for ( uint32_t l = lane; l < na; l += WaveSize )
         {
             const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
             const float  Rl = al.w + b.probeRadius;
@@ -635,9 +634,16 @@ namespace sesdf
             inside          = inside || d + Ra <= Rl;
             cap             = cap || ( d < Ra + Rl - 1e-3f && d + Rl > Ra + 1e-3f );
         }
-        const bool open = !inside && ( b.vStart[ a + 1 ] > b.vStart[ a ] || !cap );
-        surfFlag[ a ] = ( elems || open ) ? 1u : 0u;
-        info[ a ]     = open ? 1u : 0u;
+        inside = ballot( inside ) != 0;
+        cap    = ballot( cap ) != 0;
+        if ( lane != 0 )
+            return;
+        pa[ a ]          = make_float4( at.x, at.y, at.z, Ra );
+        const bool elems = b.fullBase[ a + 1 ] > b.fullBase[ a ] || b.segStart[ b.trimBase[ a + 1 ] ] > b.segStart[ b.trimBase[ a ] ] ||
+                           b.wStart[ a + 1 ] > b.wStart[ a ] || b.nStart[ a + 1 ] > b.nStart[ a ];
+        const bool open  = !inside && ( b.vStart[ a + 1 ] > b.vStart[ a ] || !cap );
+        surfFlag[ a ]    = ( elems || open ) ? 1u : 0u;
+        info[ a ]        = open ? 1u : 0u;
     }
 
     // Compacted surface atoms in cell order and their per-cell offsets.
@@ -659,16 +665,18 @@ namespace sesdf
     }
 
     // Cap of every visible circle (sector: axis, half angle) and the geometry of every segment.
-    __global__ void sdf_geometry_kernel( Buffers b, float4 * caps, float4 * segGeo )
+    __global__ void __launch_bounds__( 256 ) sdf_geometry_kernel( Buffers b, float4 * caps, float4 * segGeo )
     {
-        const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-        if ( x < min( b.ctl->sectorNb, b.capSectors ) )
+        const uint32_t nCaps = min( b.ctl->sectorNb, b.capSectors ), nSeg = min( b.ctl->segmentNb, b.capSegments );
+        for ( uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < max( nCaps, nSeg ); x += gridDim.x * blockDim.x )
         {
-            const float4 sc = b.sectors[ x ];
-            caps[ x ]       = make_float4( sc.x, sc.y, sc.z, cosf( sc.w ) );
-        }
-        if ( x < min( b.ctl->segmentNb, b.capSegments ) )
-        {
+            if ( x < nCaps )
+            {
+                const float4 sc = b.sectors[ x ];
+                caps[ x ]       = make_float4( sc.x, sc.y, sc.z, cosf( sc.w ) );
+            }
+            if ( x >= nSeg )
+                continue;
             const uint4 seg = b.segments[ x ];
             f3          ctr, nrm;
             float       rad;
@@ -686,12 +694,12 @@ namespace sesdf
     // One wave per atom: up to 64 caps are ranked in registers (key, then index: a total order, so
     // ranks are distinct) and written to their places; longer lists are insertion-sorted by lane 0.
     // Caps of non-existent circles (contained spheres) carry a NaN cosine and are ranked last.
-    __global__ void __launch_bounds__( 256 ) sdf_sort_caps_kernel( Buffers b, float4 * caps )
+    __global__ void __launch_bounds__( 256 ) sdf_sort_caps_kernel( Buffers b, float4 * caps, const uint32_t * open )
     {
         const uint32_t a    = xcd_block() * 4 + threadIdx.x / WaveSize;
         const int      lane = lane_id();
-        if ( a >= b.n )
-            return;
+        if ( a >= b.n || !open[ a ] )
+            return; // only faces that may be exposed are ever tested against caps
         const uint32_t c0 = b.vStart[ a ], c1 = min( b.vStart[ a + 1 ], b.capSectors );
         const int      n  = int( c1 > c0 ? c1 - c0 : 0u );
         auto           key = []( float w ) { return w == w ? w : 3.0f; };
@@ -1094,7 +1102,7 @@ namespace
         }
         hipStream_t s = h.stream;
         if ( n )
-            sdf_pack_kernel<<<( n + 255 ) / 256, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo );
+            sdf_pack_kernel<<<( n + 3 ) / 4, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo );
         ScanArgs a {};
         a.in[ 0 ]     = d.surfFlag;
         a.out[ 0 ]    = d.surfPrefix;
@@ -1109,9 +1117,10 @@ namespace
         // sized by the capacities: the kernel reads the counts of the build from the device
         const uint32_t geoItems = std::max( h.cap.sectors, h.cap.segments );
         if ( geoItems )
-            sdf_geometry_kernel<<<( geoItems + 255 ) / 256, 256, 0, s>>>( h.b, d.geo, d.geo + d.geoSectors );
+            sdf_geometry_kernel<<<std::min<uint32_t>( ( geoItems + 255 ) / 256, uint32_t( device_cu_count() ) * 8 ), 256, 0, s>>>(
+                h.b, d.geo, d.geo + d.geoSectors );
         if ( n )
-            sdf_sort_caps_kernel<<<( n + 3 ) / 4, 256, 0, s>>>( h.b, d.geo );
+            sdf_sort_caps_kernel<<<( n + 3 ) / 4, 256, 0, s>>>( h.b, d.geo, d.surfInfo );
         if ( hipGetLastError() != hipSuccess )
             return SESDF_ERR_HIP;
         d.packSerial = h.buildSerial;
