@@ -541,6 +541,8 @@ namespace sesdf
         // below min(g, max(band - rp, 0)) (above, the value is clamped to +band).
         L.g = D, L.gExt = 3.0e38f, L.gA = 0xffffffffu;
         auto nearest = [ & ]( int xa, int xb, int y, int z, int, float dyz2 ) {
+            if ( !any_lane<COOP>( L.active && L.g >= 0.f ) )
+                return; // every point is inside: g no longer matters
             const float lb   = lower( xa, xb, dyz2 );
             const float gCut = s.exterior ? L.g + 1e-4f : fminf( L.g, fmaxf( s.band - rp, 0.f ) );
             if ( !any_lane<COOP>( L.active && L.g >= 0.f && lb <= gCut ) )
@@ -576,6 +578,10 @@ namespace sesdf
                     return;
                 const uint32_t h0 = cell_index( M, xa, y, z );
                 const uint32_t k0 = s.surfCellStart[ h0 ], k1 = s.surfCellStart[ h0 + uint32_t( xb - xa ) + 1 ];
+                // faces of a near-block row whose atoms all failed the element gate: nothing to test
+                if constexpr ( COOP )
+                    if ( !elems && ord >= 0 && ord < MaskRows && k1 - k0 <= 2 * WaveSize && ( masks[ 2 * ord ] | masks[ 2 * ord + 1 ] ) == 0 )
+                        return;
                 if constexpr ( COOP )
                     coop_row( b, s, L, needB, k0, k1, extC, Iw, elems, faces, ord, stage, masks, box );
                 else
@@ -1162,19 +1168,17 @@ namespace
     }
 
     // blocks of 4 waves for one wave per 4^3 brick of grid vertices
+    // One wave per brick: brick costs differ by two orders of magnitude (outside vs surface), so
+    // the hardware dispatcher, not a static stride, balances them. Grids beyond 2^31 blocks stride.
     uint32_t brick_blocks( const SdfArgs & s, uint32_t molNb )
     {
         const uint64_t waves = uint64_t( ( s.dx + 3 ) / 4 ) * ( ( s.dy + 3 ) / 4 ) * ( ( s.dz + 3 ) / 4 ) * molNb;
-        const uint64_t cap   = uint64_t( device_cu_count() ) * 32;
-        return uint32_t( std::max<uint64_t>( 1, std::min( ( waves + 3 ) / 4, cap ) ) );
+        return uint32_t( std::max<uint64_t>( 1, std::min<uint64_t>( ( waves + 3 ) / 4, 0x7fffffffull ) ) );
     }
 
-    uint32_t grid_blocks( uint64_t n )
-    {
-        const uint64_t want = ( n + 255 ) / 256;
-        const uint64_t cap  = uint64_t( device_cu_count() ) * 32;
-        return uint32_t( std::max<uint64_t>( 1, std::min( want, cap ) ) );
-    }
+    // One thread per item (costs vary per query): dynamic dispatch balances; beyond 2^31 blocks the
+    // kernels stride.
+    uint32_t grid_blocks( uint64_t n ) { return uint32_t( std::max<uint64_t>( 1, std::min<uint64_t>( ( n + 255 ) / 256, 0x7fffffffull ) ) ); }
 } // namespace
 
 extern "C"
