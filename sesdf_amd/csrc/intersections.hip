@@ -23,6 +23,7 @@ namespace sesdf
     constexpr int      InterWaves  = 4;
     constexpr uint32_t RecMinus    = 1u << 31;
     constexpr uint32_t RecHasNb    = 1u << 30;
+    constexpr uint32_t PairRound   = 512; // candidate pairs per compaction round (LDS list per wave)
 
     struct CircleGeo
     {
@@ -112,6 +113,7 @@ namespace sesdf
         __shared__ float4   sAbs[ WAVES ][ KMAX ];    // (c_l, r_l) as stored
         __shared__ uint16_t sCand[ WAVES ][ KMAX ];
         __shared__ uint4    sTouch[ WAVES ][ TouchStride ];
+        __shared__ uint32_t sPair[ WAVES ][ PairRound ];
 
         const int      w    = threadIdx.x / WaveSize;
         const int      lane = lane_id();
@@ -154,35 +156,39 @@ namespace sesdf
         wave_sync();
 
         // ---- candidate pairs (a < c over the intersected circles of i), flattened over the lanes ----
+        // Pairs are taken in rounds of PairRound: first the pairs with k in N(j) are compacted into
+        // LDS in pair order (operations.inl:330-376: k must be a neighbour of j), then the geometry
+        // runs on them densely, 64 at a time. Records keep pair order either way.
         const uint32_t P     = m * ( m - 1 ) / 2;
         uint32_t       nRec  = 0, nW = 0, nFp64 = 0;
-        for ( uint32_t pb = 0; pb < P; pb += WaveSize )
+        for ( uint32_t r0 = 0; r0 < P; r0 += PairRound )
         {
-            const uint32_t p    = pb + lane;
-            int            nOut = 0;
-            uint32_t       out0 = 0, out1 = 0;
-            bool           fp64 = false;
-            if ( p < P )
+        const uint32_t r1 = min( P, r0 + PairRound );
+        uint32_t       nMember = 0;
+        for ( uint32_t pb = r0; pb < r1; pb += WaveSize )
+        {
+            const uint32_t p      = pb + lane;
+            bool           member = false;
+            uint32_t       a = 0, cc = 0;
+            if ( p < r1 )
             {
                 // row-major triangular index -> (a, c) with a < c
                 const float fm = float( 2 * m - 1 );
-                int         a  = int( ( fm - sqrtf( fm * fm - 8.f * float( p ) ) ) * 0.5f );
-                a              = max( 0, min( a, int( m ) - 2 ) );
-                while ( a > 0 && uint32_t( a ) * ( 2 * m - a - 1 ) / 2 > p )
-                    a--;
-                while ( uint32_t( a + 1 ) * ( 2 * m - a - 2 ) / 2 <= p )
-                    a++;
-                const uint32_t cc = p - uint32_t( a ) * ( 2 * m - a - 1 ) / 2 + a + 1;
+                int         ai = int( ( fm - sqrtf( fm * fm - 8.f * float( p ) ) ) * 0.5f );
+                ai             = max( 0, min( ai, int( m ) - 2 ) );
+                while ( ai > 0 && uint32_t( ai ) * ( 2 * m - ai - 1 ) / 2 > p )
+                    ai--;
+                while ( uint32_t( ai + 1 ) * ( 2 * m - ai - 2 ) / 2 <= p )
+                    ai++;
+                a                 = uint32_t( ai );
+                cc                = p - a * ( 2 * m - a - 1 ) / 2 + a + 1;
                 const uint32_t la = sCand[ w ][ a ];
                 const uint32_t lk = sCand[ w ][ cc ];
-                const uint32_t j  = sIdx[ w ][ la ];
-                const uint32_t k  = sIdx[ w ][ lk ];
-                // k must be a neighbour of j (operations.inl:330-376). Unless some list was truncated
-                // at K, membership is exactly the predicate that built j's list, evaluated here with
-                // the same operands and operations; the slot of k in j's list is found at emission.
-                bool member;
+                // Unless some list was truncated at K, membership is exactly the predicate that built
+                // j's list, evaluated with the same operands and operations; the slot of k in j's
+                // list is found at emission.
                 if ( truncated )
-                    member = find_in_list( b.nbIds + size_t( j ) * K, b.nbCount[ j ], k ) >= 0;
+                    member = find_in_list( b.nbIds + size_t( sIdx[ w ][ la ] ) * K, b.nbCount[ sIdx[ w ][ la ] ], sIdx[ w ][ lk ] ) >= 0;
                 else
                 {
                     const float4 Aj = sAbs[ w ][ la ], Ak = sAbs[ w ][ lk ];
@@ -190,7 +196,25 @@ namespace sesdf
                     const float  Rs = ( Aj.w + rp ) + ( Ak.w + rp );
                     member          = len2( d ) - Rs * Rs <= PredicateEps;
                 }
-                if ( member )
+            }
+            const uint64_t mk = ballot( member );
+            if ( member )
+                sPair[ w ][ nMember + rank_below( mk ) ] = a | ( cc << 16 );
+            nMember += uint32_t( __popcll( mk ) );
+        }
+        wave_sync();
+        for ( uint32_t mb = 0; mb < nMember; mb += WaveSize )
+        {
+            const uint32_t e    = mb + lane;
+            int            nOut = 0;
+            uint32_t       out0 = 0, out1 = 0;
+            bool           fp64 = false;
+            if ( e < nMember )
+            {
+                const uint32_t pk = sPair[ w ][ e ];
+                const uint32_t a = pk & 0xffffu, cc = pk >> 16;
+                const uint32_t la = sCand[ w ][ a ];
+                const uint32_t lk = sCand[ w ][ cc ];
                 {
                     const float4    aj  = sAtom[ w ][ la ];
                     const float4    ak  = sAtom[ w ][ lk ];
@@ -305,6 +329,8 @@ namespace sesdf
             nRec += uint32_t( __popcll( has1 ) + __popcll( has2 ) );
             nW += uint32_t( __popcll( ballot( nOut >= 1 && ( out0 & RecHasNb ) ) ) + __popcll( ballot( nOut == 2 && ( out1 & RecHasNb ) ) ) );
             nFp64 += uint32_t( __popcll( ballot( fp64 ) ) );
+        }
+        wave_sync(); // the next round overwrites the pair list
         }
         if ( nRec > b.stageStride )
         {
