@@ -57,6 +57,7 @@ namespace sesdf
     {
         __shared__ uint32_t sIdx[ WAVES ][ KMAX ];
         __shared__ float4   sAtom[ WAVES ][ KMAX ];   // (c_k - c_i, Rk^2)
+        __shared__ float2   sAux[ WAVES ][ KMAX ];    // (|c_k - c_i|^2, Rk^2 + eps - |c_k - c_i|^2 - Ri^2)
         __shared__ uint32_t sRowStart[ WAVES ][ WaveSize ];
         __shared__ uint32_t sRowPrefix[ WAVES ][ WaveSize + 1 ];
 
@@ -150,9 +151,21 @@ namespace sesdf
         if ( lane == 0 )
             b.nbCount[ i ] = ni;
         for ( uint32_t l = lane; l < ni; l += WaveSize )
+        {
             b.nbIds[ size_t( i ) * K + l ] = sIdx[ w ][ l ];
+            const float4 ak  = sAtom[ w ][ l ];
+            const float  K2  = len2( xyz( ak ) );
+            sAux[ w ][ l ]   = make_float2( K2, ( ak.w + PredicateEps ) - K2 - Ri2 );
+        }
+        wave_sync();
 
         // ---- visibility of circles (i, j): lanes over j, sequential over k in LDS -----------------
+        // The circle centre lies on the axis n (centre = rho n) and its points on sphere i
+        // (rho^2 + R^2 = Ri^2), so with e = n . c_k (c_k relative to c_i):
+        //   h^2 = |(centre - c_k) x n|^2 = |c_k|^2 - e^2,
+        //   A   = Rk^2 + eps - |centre - c_k|^2 - R^2 = (Rk^2 + eps - |c_k|^2 - Ri^2) + 2 rho e,
+        // one dot product and three fused operations per (circle, sphere) instead of a cross product
+        // and two squared lengths.
         uint32_t nV = 0, nT = 0, nF = 0;
         for ( uint32_t lb = 0; lb < K; lb += WaveSize )
         {
@@ -167,16 +180,17 @@ namespace sesdf
                 {
                     const float4     aj      = sAtom[ w ][ l ];
                     const CircleTest c       = make_circle( xyz( aj ), Ri2, aj.w );
+                    const float      twoRho  = 2.f * dot( c.centre, c.normal );
                     uint32_t         visible = 1, complete = 1;
                     for ( uint32_t k = 0; k < ni; k++ )
                     {
                         if ( k == l )
                             continue;
                         const float4 ak  = sAtom[ w ][ k ];
-                        const f3     v   = c.centre - xyz( ak );
-                        const f3     vx  = cross( v, c.normal );
-                        const float  h2  = len2( vx );
-                        const float  A   = ( ak.w + PredicateEps ) - len2( v ) - c.radius2;
+                        const float2 ax  = sAux[ w ][ k ];
+                        const float  e   = fmaf( c.normal.z, ak.z, fmaf( c.normal.y, ak.y, c.normal.x * ak.x ) );
+                        const float  h2  = fmaf( -e, e, ax.x );
+                        const float  A   = fmaf( twoRho, e, ax.y );
                         const float  lhs = c.fourR2 * h2;
                         const float  A2  = A * A;
                         if ( A > 0.f && lhs < A2 )
