@@ -93,10 +93,23 @@ namespace sesdf
             uint32_t  st = 0, len = 0;
             if ( r < rows )
             {
-                const int      yy = y0 + r % ny, zz = z0 + r / ny;
-                const uint32_t h0 = cell_index( M, x0, yy, zz );
-                st                = b.cellStart[ h0 ];
-                len               = b.cellStart[ h0 + uint32_t( x1 - x0 ) + 1 ] - st;
+                // the row's cells within `range` of c_i: its y/z gap leaves a half-width along x
+                const int   yy  = y0 + r % ny, zz = z0 + r / ny;
+                const float gyv = slab_gap( M.oy + M.cell * float( yy ), M.cell, ci.y );
+                const float gzv = slab_gap( M.oz + M.cell * float( zz ), M.cell, ci.z );
+                const float rem = range * range - gyv * gyv - gzv * gzv;
+                if ( rem >= 0.f )
+                {
+                    const float half = sqrtf( rem );
+                    const int   xa   = max( x0, int( floorf( ( ci.x - half - M.ox ) / M.cell ) ) );
+                    const int   xb   = min( x1, int( floorf( ( ci.x + half - M.ox ) / M.cell ) ) );
+                    if ( xa <= xb )
+                    {
+                        const uint32_t h0 = cell_index( M, xa, yy, zz );
+                        st                = b.cellStart[ h0 ];
+                        len               = b.cellStart[ h0 + uint32_t( xb - xa ) + 1 ] - st;
+                    }
+                }
             }
             const uint32_t incl  = wave_incl_scan( len );
             const uint32_t total = wave_bcast( incl, WaveSize - 1 );

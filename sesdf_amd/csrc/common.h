@@ -265,6 +265,9 @@ namespace sesdf
         gy = clampi( int( floorf( ( y - m.oy ) / m.cell ) ), 0, m.dy - 1 );
         gz = clampi( int( floorf( ( z - m.oz ) / m.cell ) ), 0, m.dz - 1 );
     }
+    // Gap between coordinate v and the cell slab [lo, lo + cell] along one axis (0 inside).
+    __device__ __forceinline__ float slab_gap( float lo, float cell, float v ) { return fmaxf( fmaxf( lo - v, v - ( lo + cell ) ), 0.f ); }
+
     __device__ __forceinline__ uint32_t cell_index( const Molecule & m, int gx, int gy, int gz )
     {
         return m.cellBase + uint32_t( ( gz * m.dy + gy ) * m.dx + gx );

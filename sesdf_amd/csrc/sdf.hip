@@ -167,9 +167,6 @@ namespace sesdf
         int bx0, bx1, by0, by1, bz0, bz1; // near block
     };
 
-    // Gap between coordinate v and the cell slab [lo, lo + cell] along one axis.
-    __device__ __forceinline__ float axis_gap( float lo, float cell, float v ) { return fmaxf( fmaxf( lo - v, v - ( lo + cell ) ), 0.f ); }
-
     // Row segments of the stencil as fn(xa, xb, y, z, ord, dyz2): the near block first (ord = the
     // row's ordinal within the block), then the rest (ord = -1). dyz2 is the squared y/z gap from q
     // to the row, computed once per row for the caller's distance bound.
@@ -179,10 +176,10 @@ namespace sesdf
         int ord = 0;
         for ( int z = t.bz0; z <= t.bz1; z++ )
         {
-            const float gz = axis_gap( M.oz + M.cell * float( z ), M.cell, q.z );
+            const float gz = slab_gap( M.oz + M.cell * float( z ), M.cell, q.z );
             for ( int y = t.by0; y <= t.by1; y++, ord++ )
             {
-                const float gy = axis_gap( M.oy + M.cell * float( y ), M.cell, q.y );
+                const float gy = slab_gap( M.oy + M.cell * float( y ), M.cell, q.y );
                 fn( t.bx0, t.bx1, y, z, ord, gz * gz + gy * gy );
             }
         }
@@ -192,10 +189,10 @@ namespace sesdf
     {
         for ( int z = t.z0; z <= t.z1; z++ )
         {
-            const float gz = axis_gap( M.oz + M.cell * float( z ), M.cell, q.z );
+            const float gz = slab_gap( M.oz + M.cell * float( z ), M.cell, q.z );
             for ( int y = t.y0; y <= t.y1; y++ )
             {
-                const float gy   = axis_gap( M.oy + M.cell * float( y ), M.cell, q.y );
+                const float gy   = slab_gap( M.oy + M.cell * float( y ), M.cell, q.y );
                 const float dyz2 = gz * gz + gy * gy;
                 if ( z >= t.bz0 && z <= t.bz1 && y >= t.by0 && y <= t.by1 )
                 {
