@@ -579,58 +579,99 @@ namespace sesdf
         }
     }
 
+    // One wave per with-neighbour probe q: the rows of the cell stencil around q are flattened over
+    // the lanes (the with-neighbour probes of a row are the W records of its atoms, contiguous in
+    // records), every candidate is tested by one lane, and hits are appended with a ballot in
+    // (row, record) order, the first maxNb kept.
     template<int MODE>
     __global__ void __launch_bounds__( 256 ) probe_neighbors_kernel( Buffers b )
     {
-        const uint32_t Iw = b.ctl->intersectionWithNbNb;
+        __shared__ uint32_t sRowStart[ 4 ][ WaveSize ];
+        __shared__ uint32_t sRowPrefix[ 4 ][ WaveSize + 1 ];
+        const uint32_t      Iw = b.ctl->intersectionWithNbNb;
         if ( Iw + b.ctl->intersectionNoNbNb > b.capRecords || Iw > b.capProbeNb || ( b.ctl->overflow & OverflowIntersections ) )
             return;
+        const int      w     = threadIdx.x / WaveSize;
+        const int      lane  = lane_id();
         const float    rp    = b.probeRadius;
         const float    rmax  = __uint_as_float( b.ctl->radiusMaxBits );
         const float    range = 2.f * rp;
         const uint32_t maxNb = b.maxProbeNb;
-        for ( uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < Iw; q += gridDim.x * blockDim.x )
+        for ( uint32_t q = blockIdx.x * 4 + w; q < Iw; q += gridDim.x * 4 )
         {
-            const uint32_t owner = b.records[ q ].x;
-            const float4   pq    = b.concavePos[ q ];
-            const Molecule M     = b.mols[ find_molecule( b.mols, b.molNb, owner ) ];
-            // a probe within 2 rp of pq lies on the SAS sphere of its owner atom, so that atom is
-            // within 2 rp + (r_max + rp) of pq
-            const float search = range + rmax + rp + 1e-3f;
-            const int   reach  = max( int( ceilf( search / M.cell ) ), 1 );
-            int         gx, gy, gz;
-            cell_of( M, pq.x, pq.y, pq.z, gx, gy, gz );
-            const int x0 = max( gx - reach, 0 ), x1 = min( gx + reach, M.dx - 1 );
-            const int y0 = max( gy - reach, 0 ), y1 = min( gy + reach, M.dy - 1 );
-            const int z0 = max( gz - reach, 0 ), z1 = min( gz + reach, M.dz - 1 );
-            const float wv = MODE == 0 ? pq.x * pq.y * pq.z : 0.f;
-            uint32_t    count = 0;
-            for ( int zz = z0; zz <= z1 && count < maxNb; zz++ )
-                for ( int yy = y0; yy <= y1 && count < maxNb; yy++ )
+        const uint32_t owner = b.records[ q ].x;
+        const float4   pq    = b.concavePos[ q ];
+        const Molecule M     = b.mols[ find_molecule( b.mols, b.molNb, owner ) ];
+        // a probe within 2 rp of pq lies on the SAS sphere of its owner atom, so that atom is within
+        // 2 rp + (r_max + rp) of pq
+        const float search = range + rmax + rp + 1e-3f;
+        const int   reach  = max( int( ceilf( search / M.cell ) ), 1 );
+        int         gx, gy, gz;
+        cell_of( M, pq.x, pq.y, pq.z, gx, gy, gz );
+        const int   x0 = max( gx - reach, 0 ), x1 = min( gx + reach, M.dx - 1 );
+        const int   y0 = max( gy - reach, 0 ), y1 = min( gy + reach, M.dy - 1 );
+        const int   z0 = max( gz - reach, 0 ), z1 = min( gz + reach, M.dz - 1 );
+        const int   ny = y1 - y0 + 1, rows = ny * ( z1 - z0 + 1 );
+        const float wv = MODE == 0 ? pq.x * pq.y * pq.z : 0.f;
+        uint32_t    count = 0;
+        for ( int rowBase = 0; rowBase < rows; rowBase += WaveSize )
+        {
+            const int r  = rowBase + lane;
+            uint32_t  st = 0, len = 0;
+            if ( r < rows )
+            {
+                const uint32_t h0 = cell_index( M, x0, y0 + r % ny, z0 + r / ny );
+                const uint32_t a0 = b.cellStart[ h0 ], a1 = b.cellStart[ h0 + uint32_t( x1 - x0 ) + 1 ];
+                st                = b.wStart[ a0 ];
+                len               = b.wStart[ a1 ] - st;
+            }
+            const uint32_t incl  = wave_incl_scan( len );
+            const uint32_t total = wave_bcast( incl, WaveSize - 1 );
+            sRowStart[ w ][ lane ]      = st;
+            sRowPrefix[ w ][ lane + 1 ] = incl;
+            if ( lane == 0 )
+                sRowPrefix[ w ][ 0 ] = 0;
+            wave_sync();
+            const int rowsHere = min( rows - rowBase, WaveSize );
+            for ( uint32_t base = 0; base < total; base += WaveSize )
+            {
+                const uint32_t e   = base + lane;
+                bool           hit = false;
+                float4         pr  = make_float4( 0.f, 0.f, 0.f, 0.f );
+                if ( e < total )
                 {
-                    const uint32_t h0 = cell_index( M, x0, yy, zz );
-                    const uint32_t a0 = b.cellStart[ h0 ], a1 = b.cellStart[ h0 + uint32_t( x1 - x0 ) + 1 ];
-                    if ( a0 == a1 )
-                        continue;
-                    const uint32_t r0 = b.wStart[ a0 ], r1 = b.wStart[ a1 ];
-                    for ( uint32_t r = r0; r < r1 && count < maxNb; r++ )
+                    int lo = 0, hi = rowsHere; // row: largest with prefix <= e
+                    while ( hi - lo > 1 )
                     {
-                        if ( r == q )
-                            continue;
-                        const float4 pr = b.concavePos[ r ];
-                        const f3     d  = xyz( pq ) - xyz( pr );
-                        if ( sqrtf( len2( d ) ) < range )
-                        {
-                            b.concaveNb[ size_t( q ) * maxNb + count ] = make_float4( pr.x, pr.y, pr.z, wv );
-                            count++;
-                        }
+                        const int mid = ( lo + hi ) >> 1;
+                        if ( sRowPrefix[ w ][ mid ] <= e )
+                            lo = mid;
+                        else
+                            hi = mid;
+                    }
+                    const uint32_t rec = sRowStart[ w ][ lo ] + ( e - sRowPrefix[ w ][ lo ] );
+                    if ( rec != q )
+                    {
+                        pr  = b.concavePos[ rec ];
+                        hit = sqrtf( len2( xyz( pq ) - xyz( pr ) ) ) < range;
                     }
                 }
+                const uint64_t mask = ballot( hit );
+                const uint32_t slot = count + rank_below( mask );
+                if ( hit && slot < maxNb )
+                    b.concaveNb[ size_t( q ) * maxNb + slot ] = make_float4( pr.x, pr.y, pr.z, wv );
+                count += uint32_t( __popcll( mask ) );
+            }
+            wave_sync();
+        }
+        if ( lane == 0 )
+        {
             if ( count >= maxNb )
                 atomicAdd( &b.ctl->probeNbOverflowNb, 1u );
-            b.concaveIds[ q ].w = int( count );
+            b.concaveIds[ q ].w = int( min( count, maxNb ) );
             if ( MODE == 1 )
                 b.concavePos[ q ].w = float( q * maxNb );
+        }
         }
     }
 
@@ -803,7 +844,8 @@ namespace sesdf
     {
         if ( b.n == 0 )
             return;
-        const uint32_t blocks = persistent_blocks( b.capProbeNb, 256, 8 );
+        // one wave per with-neighbour probe, striding beyond 64 workgroups per CU
+        const uint32_t blocks = persistent_blocks( uint64_t( b.capProbeNb ) * WaveSize, 256, 64 );
         if ( b.mode == 0 )
             probe_neighbors_kernel<0><<<blocks, 256, 0, s>>>( b );
         else
