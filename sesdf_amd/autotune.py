@@ -25,17 +25,22 @@ def signature(surf):
     return f"n2^{int(math.log2(n))}-m2^{int(math.log2(max(mols, 1)))}-mode{surf.MODE}-rp{surf.probe_radius:.2f}-K{surf.get_max_neighbor_per_atom()}"
 
 
-def time_build(surf, reps=5, warmup=2):
-    """Mean device time (ms) of one graph-replayed build."""
+def time_build(surf, reps=5, warmup=2, after=None):
+    """Mean device time (ms) of one graph-replayed build, plus the work `after()` enqueues on the
+    build stream after each build (e.g. SDF sampling) when given."""
     lib = L.lib()
     stream = surf.stream()
     L.check(lib.sesdf_build(surf._h), "build")  # grows capacities if needed, captures the graph
     for _ in range(warmup):
         L.check(lib.sesdf_enqueue(surf._h), "enqueue")
+        if after:
+            after()
     e0, e1 = L.Event(), L.Event()
     e0.record(stream)
     for _ in range(reps):
         L.check(lib.sesdf_enqueue(surf._h), "enqueue")
+        if after:
+            after()
     e1.record(stream)
     e1.sync()
     L.check(lib.sesdf_sync(surf._h), "sync")
@@ -50,9 +55,10 @@ def load_cache():
         return {}
 
 
-def autotune(surf, reps=5, use_cache=True, log=None):
-    """Pick (cell_size, circle_waves, intersection_waves) for this handle, apply it, return it."""
-    sig = signature(surf)
+def autotune(surf, reps=5, use_cache=True, log=None, after=None, tag=""):
+    """Pick (cell_size, circle_waves, intersection_waves) for this handle, apply it, return it.
+    With `after`, the objective is the build plus the work it enqueues (`tag` names it in the cache)."""
+    sig = signature(surf) + tag
     cache = load_cache()
     if use_cache and sig in cache:
         best = cache[sig]
@@ -63,7 +69,7 @@ def autotune(surf, reps=5, use_cache=True, log=None):
 
     def trial(cfg):
         set_tuning(surf, cfg["cell_size"], cfg["circle_waves"], cfg["intersection_waves"])
-        ms = time_build(surf, reps)
+        ms = time_build(surf, reps, after=after)
         trials.append(dict(cfg, ms=ms))
         if log:
             log(f"autotune {cfg} -> {ms:.3f} ms")
