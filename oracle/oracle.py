@@ -35,6 +35,8 @@ def lib():
         L.oracle_build.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_uint32, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.oracle_build.restype = ctypes.c_void_p
+        L.oracle_build_nb.argtypes = L.oracle_build.argtypes + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_build_nb.restype = ctypes.c_void_p
         L.oracle_free.argtypes = [ctypes.c_void_p]
         L.oracle_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
         L.oracle_get.restype = ctypes.c_uint64
@@ -131,9 +133,27 @@ class OracleResult:
 
 
 def build(atoms, probe_radius=1.4, mode=0, precision=64, max_neighbors=128, track=True, delta=1e-3,
-          delta_theta=1e-4, threads=1):
-    """Run the oracle build. mode 0 = complete surface (Sesdf), 1 = exterior path (Ssesdf)."""
+          delta_theta=1e-4, threads=1, neighbors=None, order=None):
+    """Run the oracle build. mode 0 = complete surface (Sesdf), 1 = exterior path (Ssesdf).
+
+    neighbors: optional (count[n], ids[n, max_neighbors]) per input atom, in input ids, used in place
+    of the grid query's lists; order: optional sorted order (sorted index -> input index) used in place
+    of the grid's. Together they check a build whose lists were truncated at K in another grid order."""
     a = np.ascontiguousarray(atoms, np.float32)
-    h = lib().oracle_build(ctypes.c_void_p(a.ctypes.data), a.shape[0], probe_radius, mode, precision,
-                           max_neighbors, int(track), delta, delta_theta, threads)
+    if order is not None:
+        order = np.ascontiguousarray(order, np.uint32)
+        assert order.shape == (a.shape[0],) and np.array_equal(np.sort(order), np.arange(a.shape[0]))
+    if neighbors is None and order is None:
+        h = lib().oracle_build(ctypes.c_void_p(a.ctypes.data), a.shape[0], probe_radius, mode, precision,
+                               max_neighbors, int(track), delta, delta_theta, threads)
+    else:
+        cnt = ids = None
+        if neighbors is not None:
+            cnt = np.ascontiguousarray(neighbors[0], np.uint32)
+            ids = np.ascontiguousarray(neighbors[1], np.uint32).reshape(-1)
+            assert cnt.shape == (a.shape[0],) and ids.size == a.shape[0] * max_neighbors
+            assert (cnt <= max_neighbors).all() and (ids < max(a.shape[0], 1)).all()
+        ptr = lambda v: ctypes.c_void_p(v.ctypes.data) if v is not None else None  # noqa: E731
+        h = lib().oracle_build_nb(ctypes.c_void_p(a.ctypes.data), a.shape[0], probe_radius, mode, precision,
+                                  max_neighbors, int(track), delta, delta_theta, threads, ptr(cnt), ptr(ids), ptr(order))
     return OracleResult(h, mode, a.shape[0], max_neighbors, probe_radius)

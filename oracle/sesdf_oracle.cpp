@@ -295,6 +295,57 @@ namespace orc
         double             delta, deltaTheta;
         int                threads;
         Result &           res;
+        // Optional neighbour lists to use in place of the grid query's (per input atom: a count and
+        // K input ids). Test infrastructure: lets a build whose lists were truncated at K in another
+        // grid order be checked stage by stage on exactly those lists.
+        const uint32_t *   nbOverrideCount = nullptr;
+        const uint32_t *   nbOverrideIds   = nullptr;
+        // Optional sorted order (sorted index -> input index) to use in place of the grid's: the atom
+        // that heads a triple (lowest sorted index) is the one whose list bounds its probe tests, so
+        // builds with truncated lists are comparable only in the same order.
+        const uint32_t *   orderOverride   = nullptr;
+
+        void fillSorted()
+        {
+            res.atoms.resize( size_t( n ) * 4 );
+            c.resize( n ), r.resize( n ), R.resize( n );
+            for ( uint32_t s = 0; s < n; s++ )
+            {
+                const AtomF & a = input[ res.sortedToInitial[ s ] ];
+                res.atoms[ 4 * s + 0 ] = a.x, res.atoms[ 4 * s + 1 ] = a.y, res.atoms[ 4 * s + 2 ] = a.z, res.atoms[ 4 * s + 3 ] = a.r;
+                c[ s ] = { T( a.x ), T( a.y ), T( a.z ) };
+                r[ s ] = T( a.r );
+                R[ s ] = T( a.r ) + rp;
+            }
+        }
+
+        // Renames every sorted index produced by the grid query into the order `orderOverride`.
+        void reorder()
+        {
+            std::vector<uint32_t> newOfInput( n ), oldToNew( n );
+            for ( uint32_t s = 0; s < n; s++ )
+                newOfInput[ orderOverride[ s ] ] = s;
+            for ( uint32_t s = 0; s < n; s++ )
+                oldToNew[ s ] = newOfInput[ res.sortedToInitial[ s ] ];
+            std::vector<uint32_t> cnt( n ), ids( size_t( n ) * K, 0 );
+            for ( uint32_t s = 0; s < n; s++ )
+            {
+                const uint32_t t = oldToNew[ s ];
+                cnt[ t ]         = res.nbCount[ s ];
+                for ( uint32_t l = 0; l < cnt[ t ]; l++ )
+                    ids[ size_t( t ) * K + l ] = oldToNew[ res.nbIds[ size_t( s ) * K + l ] ];
+                std::sort( ids.begin() + size_t( t ) * K, ids.begin() + size_t( t ) * K + cnt[ t ] );
+            }
+            res.nbCount.swap( cnt );
+            res.nbIds.swap( ids );
+            for ( size_t q = 0; q + 1 < res.ambPairs.size(); q += 2 )
+            {
+                const uint32_t a = oldToNew[ res.ambPairs[ q ] ], b = oldToNew[ res.ambPairs[ q + 1 ] ];
+                res.ambPairs[ q ] = std::min( a, b ), res.ambPairs[ q + 1 ] = std::max( a, b );
+            }
+            res.sortedToInitial.assign( orderOverride, orderOverride + n );
+            fillSorted();
+        }
 
         std::vector<V3<T>> c;   // sorted centres
         std::vector<T>     r;   // sorted vdW radii
@@ -316,16 +367,7 @@ namespace orc
             res.sortedToInitial = sg.sortedIdx;
             cellStart_          = sg.cellStart;
             cellEnd_            = sg.cellEnd;
-            res.atoms.resize( size_t( n ) * 4 );
-            c.resize( n ), r.resize( n ), R.resize( n );
-            for ( uint32_t s = 0; s < n; s++ )
-            {
-                const AtomF & a = input[ sg.sortedIdx[ s ] ];
-                res.atoms[ 4 * s + 0 ] = a.x, res.atoms[ 4 * s + 1 ] = a.y, res.atoms[ 4 * s + 2 ] = a.z, res.atoms[ 4 * s + 3 ] = a.r;
-                c[ s ] = { T( a.x ), T( a.y ), T( a.z ) };
-                r[ s ] = T( a.r );
-                R[ s ] = T( a.r ) + rp;
-            }
+            fillSorted();
             res.nbCount.assign( n, 0 );
             res.nbIds.assign( size_t( n ) * K, 0 );
             std::vector<std::vector<uint32_t>> amb( threads > 1 ? n : 0 );
@@ -383,6 +425,24 @@ namespace orc
                 res.nbCount[ i ] = count;
             } );
             res.overflowNb = overflow;
+            if ( orderOverride )
+                reorder(); // the atom grid is not used past this point
+            if ( nbOverrideCount )
+            {
+                std::vector<uint32_t> initialToSorted( n );
+                for ( uint32_t s = 0; s < n; s++ )
+                    initialToSorted[ res.sortedToInitial[ s ] ] = s;
+                for ( uint32_t i = 0; i < n; i++ )
+                {
+                    const uint32_t a   = res.sortedToInitial[ i ];
+                    const uint32_t cnt = std::min( nbOverrideCount[ a ], K );
+                    uint32_t *     out = res.nbIds.data() + size_t( i ) * K;
+                    for ( uint32_t l = 0; l < cnt; l++ )
+                        out[ l ] = initialToSorted[ nbOverrideIds[ size_t( a ) * K + l ] ];
+                    std::sort( out, out + cnt ); // ascending, as the grid traversal writes them
+                    res.nbCount[ i ] = cnt;
+                }
+            }
             if ( threads > 1 )
             {
                 for ( uint32_t i = 0; i < n; i++ )
@@ -424,8 +484,13 @@ namespace orc
                         const V3<T> kToCircle      = g.center - c[ k ];
                         const T     distToCircle   = len( kToCircle );
                         const T     secondDistance = dot( kToCircle, g.normal );
-                        const T     cosTheta       = secondDistance / distToCircle;
-                        const T     sinTheta       = std::sqrt( T( 1 ) - cosTheta * cosTheta );
+                        // The reference divides by distToCircle and takes sqrt(1 - cos^2) as is: a
+                        // sphere centred on the circle's centre (0/0) or on its axis (|cos| rounding
+                        // above 1) gives NaN there, and NaN compares as "neither hides nor touches".
+                        // Here both take their continuous limit (far = near = radius^2 at the centre),
+                        // as the HIP kernels' h^2 / A form does.
+                        const T     cosTheta       = distToCircle > T( 0 ) ? secondDistance / distToCircle : T( 1 );
+                        const T     sinTheta       = std::sqrt( std::max( T( 0 ), T( 1 ) - cosTheta * cosTheta ) );
                         const T     firstFar       = sinTheta * distToCircle + g.radius;
                         const T     far            = firstFar * firstFar + secondDistance * secondDistance;
                         const T     mFar           = far - Rk * Rk - T( Eps );
@@ -1442,8 +1507,12 @@ extern "C"
     //   precision 32 or 64
     //   track     classify predicates against the epsilon band delta (A^2 / A) and deltaTheta (rad)
     //   threads   host threads for the per-atom stages
-    void * oracle_build( const float * xyzr, uint32_t n, float probeRadius, int mode, int precision, uint32_t maxNeighbors,
-                         int track, double delta, double deltaTheta, int threads )
+    // As oracle_build, with the neighbour lists given per input atom (nbCount[n], nbIds[n * K] of
+    // input ids) instead of queried (both null: the grid query), and optionally the sorted order
+    // (order[n]: sorted index -> input index; null: the grid's).
+    void * oracle_build_nb( const float * xyzr, uint32_t n, float probeRadius, int mode, int precision, uint32_t maxNeighbors,
+                            int track, double delta, double deltaTheta, int threads, const uint32_t * nbCount, const uint32_t * nbIds,
+                            const uint32_t * order )
     {
         auto * res = new Result();
         res->mode  = mode;
@@ -1456,14 +1525,23 @@ extern "C"
         if ( precision == 64 )
         {
             Builder<double> b( atoms, n, probeRadius, maxNeighbors, mode, track != 0, delta, deltaTheta, threads, *res );
+            b.nbOverrideCount = nbCount, b.nbOverrideIds = nbIds, b.orderOverride = order;
             b.run();
         }
         else
         {
             Builder<float> b( atoms, n, probeRadius, maxNeighbors, mode, false, delta, deltaTheta, threads, *res );
+            b.nbOverrideCount = nbCount, b.nbOverrideIds = nbIds, b.orderOverride = order;
             b.run();
         }
         return res;
+    }
+
+    void * oracle_build( const float * xyzr, uint32_t n, float probeRadius, int mode, int precision, uint32_t maxNeighbors,
+                         int track, double delta, double deltaTheta, int threads )
+    {
+        return oracle_build_nb( xyzr, n, probeRadius, mode, precision, maxNeighbors, track, delta, deltaTheta, threads, nullptr, nullptr,
+                                nullptr );
     }
 
     void oracle_free( void * h ) { delete static_cast<Result *>( h ); }

@@ -61,7 +61,7 @@ namespace sesdf
         __shared__ uint32_t sRowStart[ WAVES ][ WaveSize ];
         __shared__ uint32_t sRowPrefix[ WAVES ][ WaveSize + 1 ];
 
-        const int      w    = threadIdx.x / WaveSize;
+        const int      w    = wave_in_block();
         const int      lane = lane_id();
         const uint32_t i    = xcd_block() * WAVES + w;
         if ( i >= b.n )
@@ -158,7 +158,7 @@ namespace sesdf
         }
         if ( count > K && lane == 0 )
             atomicAdd( &b.ctl->nbOverflowAtoms, 1u ); // rare: truncated list, as the reference
-        const uint32_t ni = min( count, K );
+        const uint32_t ni = uint32_t( __builtin_amdgcn_readfirstlane( int( min( count, K ) ) ) ); // uniform
         wave_sync();
 
         if ( lane == 0 )
@@ -169,6 +169,13 @@ namespace sesdf
             const float4 ak  = sAtom[ w ][ l ];
             const float  K2  = len2( xyz( ak ) );
             sAux[ w ][ l ]   = make_float2( K2, ( ak.w + PredicateEps ) - K2 - Ri2 );
+        }
+        // up to 3 padding slots past ni that neither hide, touch nor come near any circle: e = 0,
+        // h^2 = 1e30, A = -1e30 (so A |A| = -inf)
+        if ( uint32_t( lane ) < ( ( 4u - ( ni & 3u ) ) & 3u ) )
+        {
+            sAtom[ w ][ ni + lane ] = make_float4( 0.f, 0.f, 0.f, 0.f );
+            sAux[ w ][ ni + lane ]  = make_float2( 1e30f, -1e30f );
         }
         wave_sync();
 
@@ -182,48 +189,113 @@ namespace sesdf
         uint32_t nV = 0, nT = 0, nF = 0;
         for ( uint32_t lb = 0; lb < K; lb += WaveSize )
         {
-            const uint32_t l  = lb + lane;
-            uint32_t       st = 0;
-            uint32_t       j  = 0;
+            const uint32_t l      = lb + lane;
+            const uint32_t j      = l < ni ? sIdx[ w ][ l ] : 0u;
+            const bool     active = l < ni && ( MODE == 0 || i < j );
+            uint32_t       st     = 0;
             uint64_t       touch0 = 0, touch1 = 0; // spheres touching circle (i, j), by slot (K <= 128)
-            if ( l < ni )
+            if ( __builtin_expect( ballot( active ) != 0, 1 ) )
             {
-                j = sIdx[ w ][ l ];
-                if ( MODE == 0 || i < j )
+                const float4     aj     = sAtom[ w ][ min( l, K - 1 ) ];
+                const CircleTest c      = make_circle( xyz( aj ), Ri2, aj.w );
+                const float      twoRho = 2.f * dot( c.centre, c.normal );
+                const float      fourR2 = fmaxf( c.fourR2, 0.f );
+                // Lanes run the k loop in lockstep (k is wave-uniform). With P = A |A| and lhs >= 0
+                // (h^2 >= 0 wherever the test below is not redone):
+                //   hidden  <=> A > 0 and lhs < A^2   <=> lhs <  P
+                //   touched <=> A > 0 or  lhs > A^2   <=> lhs > -P
+                // Hidden / near-axis states are wave masks (scalar); a lane's own slot and the
+                // padding slots past ni are masked out of them, and out of the touch words at the
+                // end of each 32-slot block. The wave leaves once every active lane is hidden.
+                const uint64_t activeM = ballot( active );
+                uint64_t       hidM = 0, nearM = 0;
+                uint32_t       w0 = 0, w1 = 0, w2 = 0, w3 = 0, anyT = 0, word = 0;
+                const uint32_t nPad = ( ni + 3u ) & ~3u; // <= KMAX (a multiple of 4); padding is inert
+                auto           test = [ & ]( uint32_t k, uint64_t self ) {
+                    const float4 ak  = sAtom[ w ][ k ];
+                    const float2 ax  = sAux[ w ][ k ];
+                    const float  e   = fmaf( c.normal.z, ak.z, fmaf( c.normal.y, ak.y, c.normal.x * ak.x ) );
+                    const float  h2  = fmaf( -e, e, ax.x );
+                    const float  A   = fmaf( twoRho, e, ax.y );
+                    const float  lhs = fourR2 * h2;
+                    const float  P   = A * fabsf( A );
+                    nearM |= ballot( h2 < 1e-3f ) & ~self;
+                    hidM |= ballot( lhs < P ) & ~self;
+                    word |= lhs > -P ? 1u << ( k & 31u ) : 0u;
+                };
+                for ( uint32_t k0 = 0; k0 < nPad; k0 += 4 )
                 {
-                    const float4     aj      = sAtom[ w ][ l ];
-                    const CircleTest c       = make_circle( xyz( aj ), Ri2, aj.w );
-                    const float      twoRho  = 2.f * dot( c.centre, c.normal );
-                    uint32_t         visible = 1, complete = 1;
-                    for ( uint32_t k = 0; k < ni; k++ )
+                    if ( k0 - lb < uint32_t( WaveSize ) ) // slots of this pass's lanes: skip each own
                     {
-                        if ( k == l )
-                            continue;
-                        const float4 ak  = sAtom[ w ][ k ];
-                        const float2 ax  = sAux[ w ][ k ];
-                        const float  e   = fmaf( c.normal.z, ak.z, fmaf( c.normal.y, ak.y, c.normal.x * ak.x ) );
-                        const float  h2  = fmaf( -e, e, ax.x );
-                        const float  A   = fmaf( twoRho, e, ax.y );
-                        const float  lhs = c.fourR2 * h2;
-                        const float  A2  = A * A;
-                        if ( A > 0.f && lhs < A2 )
+#pragma unroll
+                        for ( uint32_t u = 0; u < 4; u++ )
+                            test( k0 + u, uint64_t( 1 ) << ( k0 + u - lb ) );
+                    }
+                    else
+                    {
+#pragma unroll
+                        for ( uint32_t u = 0; u < 4; u++ )
+                            test( k0 + u, 0ull );
+                    }
+                    if ( ( k0 & 31u ) == 28u || k0 + 4 >= nPad )
+                    {
+                        // block q of slots [32q, 32q + 32): keep real slots other than the lane's own
+                        const uint32_t q     = k0 >> 5;
+                        const uint32_t valid = ni - 32u * q >= 32u ? ~0u : ( 1u << ( ni - 32u * q ) ) - 1u;
+                        word &= valid & ~( ( l >> 5 ) == q ? 1u << ( l & 31u ) : 0u );
+                        w0 = q == 0 ? word : w0;
+                        w1 = q == 1 ? word : w1;
+                        w2 = q == 2 ? word : w2;
+                        w3 = q == 3 ? word : w3;
+                        anyT |= word;
+                        word = 0u;
+                    }
+                    if ( ( activeM & ~hidM ) == 0 )
+                        break; // every active circle is hidden: the rest no longer matters
+                }
+                bool hidden  = ( hidM >> lane ) & 1u;
+                bool near    = ( nearM >> lane ) & 1u;
+                bool touched = anyT != 0u;
+                if ( __builtin_expect( ballot( active && near ) != 0, 0 ) )
+                {
+                    // Some sphere centre lies within 0.03 A of a circle's axis. |c_k|^2 - e^2 cancels
+                    // there: its rounding (a few 1e-6 for |c_k| up to 6 A) moves 2 R h, the side
+                    // the predicates compare against A, by about R 4e-6 / h, which reaches their
+                    // 1e-3 scale as h shrinks (above 0.03 A it stays under 4e-4). Those circles
+                    // are tested again with h^2 = |c_k x n|^2.
+                    if ( active && near )
+                    {
+                        hidden = false, touched = false;
+                        w0 = w1 = w2 = w3 = 0u;
+                        for ( uint32_t k = 0; k < ni && !hidden; k++ )
                         {
-                            visible = 0;
-                            break;
-                        }
-                        if ( A > 0.f || lhs > A2 )
-                        {
-                            complete = 0;
-                            if ( k < 64 )
-                                touch0 |= uint64_t( 1 ) << k;
-                            else
-                                touch1 |= uint64_t( 1 ) << ( k - 64 );
+                            if ( k == l )
+                                continue;
+                            const float4 ak  = sAtom[ w ][ k ];
+                            const float2 ax  = sAux[ w ][ k ];
+                            const float  e   = dot( c.normal, xyz( ak ) );
+                            const float  h2  = len2( cross( xyz( ak ), c.normal ) );
+                            const float  A   = fmaf( twoRho, e, ax.y );
+                            const float  lhs = c.fourR2 * h2;
+                            const float  A2  = A * A;
+                            hidden           = A > 0.f && lhs < A2;
+                            if ( A > 0.f || lhs > A2 )
+                            {
+                                touched = true;
+                                w0 |= k < 32 ? 1u << k : 0u;
+                                w1 |= k >= 32 && k < 64 ? 1u << ( k - 32 ) : 0u;
+                                w2 |= k >= 64 && k < 96 ? 1u << ( k - 64 ) : 0u;
+                                w3 |= k >= 96 && k < 128 ? 1u << ( k - 96 ) : 0u;
+                            }
                         }
                     }
-                    st = visible * ( 1 + complete );
                 }
-                b.status[ size_t( i ) * K + l ] = uint8_t( st );
+                touch0 = uint64_t( w0 ) | ( uint64_t( w1 ) << 32 );
+                touch1 = uint64_t( w2 ) | ( uint64_t( w3 ) << 32 );
+                st     = active ? ( hidden ? 0u : ( touched ? 1u : 2u ) ) : 0u;
             }
+            if ( l < ni )
+                b.status[ size_t( i ) * K + l ] = uint8_t( st );
             const uint64_t mT = ballot( l < ni && st == 1 && j > i );
             if ( l < ni && st == 1 && j > i )
             {

@@ -46,15 +46,20 @@ namespace sesdf
     // Wave64 primitives
     // ------------------------------------------------------------------------------------------
     __device__ __forceinline__ int      lane_id() { return __lane_id(); }
-    __device__ __forceinline__ uint64_t ballot( bool p ) { return __ballot( p ); }
+    __device__ __forceinline__ uint64_t ballot( bool p ) { return __builtin_amdgcn_ballot_w64( p ); }
     __device__ __forceinline__ uint64_t lanes_below() { return ( uint64_t( 1 ) << lane_id() ) - 1; }
     __device__ __forceinline__ uint32_t rank_below( uint64_t mask ) { return uint32_t( __popcll( mask & lanes_below() ) ); }
 
+    // Value of lane `src` (uniform) in every lane, as a scalar: the compiler then knows it is
+    // wave-uniform, so loops bounded by it stay uniform (no exec-mask bookkeeping).
     template<class T>
     __device__ __forceinline__ T wave_bcast( T v, int src )
     {
-        return __shfl( v, src, WaveSize );
+        static_assert( sizeof( T ) == 4, "32-bit values" );
+        return __builtin_bit_cast( T, __builtin_amdgcn_readlane( __builtin_bit_cast( int, v ), src ) );
     }
+    // Wave index of the calling lane within its workgroup, as a uniform value.
+    __device__ __forceinline__ int wave_in_block() { return __builtin_amdgcn_readfirstlane( int( threadIdx.x ) / WaveSize ); }
     __device__ __forceinline__ uint32_t wave_sum( uint32_t v )
     {
         for ( int o = 32; o > 0; o >>= 1 )

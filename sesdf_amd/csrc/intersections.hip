@@ -46,13 +46,50 @@ namespace sesdf
     // FP64: the circle-circle intersection point xc and discriminant t of the probe trilateration.
     __device__ __forceinline__ bool ill_conditioned( float den, float t, float Ri2 ) { return den < 1e-3f || fabsf( t ) < 1e-3f * Ri2; }
 
+    // c_b - c_a in FP64 from the stored coordinates: exact, where the FP32 difference rounds (its
+    // operands differ by more than a factor of two) and an ill-conditioned solve would amplify that.
+    __device__ __forceinline__ d3 rel64( float4 b, float4 a ) { return mkd( xyz( b ) ) - mkd( xyz( a ) ); }
+
+    struct Resolved
+    {
+        f3   x0, x1;
+        bool ok;
+    };
+
+    // FP64 re-solve of xc, t and the two probe positions of triplet (i, j, k), in i's local frame,
+    // from the stored (absolute) coordinates. Out of line: its double temporaries would otherwise
+    // set the register budget, and so the occupancy, of the whole pair loop it is rarely taken in.
+    __device__ __noinline__ Resolved resolve_fp64( float4 ai, float4 aj, float4 ak, float rp )
+    {
+        Resolved     r {};
+        const double RiD = double( ai.w ) + double( rp );
+        const double RjD = double( aj.w ) + double( rp );
+        const double RkD = double( ak.w ) + double( rp );
+        const d3     Dj = rel64( aj, ai ), Dk = rel64( ak, ai );
+        const double dj2 = dot( Dj, Dj ), dk2 = dot( Dk, Dk );
+        const d3     cij = Dj * ( ( RiD * RiD - RjD * RjD + dj2 ) / ( 2.0 * dj2 ) );
+        const d3     cik = Dk * ( ( RiD * RiD - RkD * RkD + dk2 ) / ( 2.0 * dk2 ) );
+        const d3     nij = Dj * ( 1.0 / sqrt( dj2 ) ), nik = Dk * ( 1.0 / sqrt( dk2 ) );
+        const d3     uu  = nik - nij * dot( nik, nij );
+        const d3     xcD = cij + uu * ( dot( cik - cij, nik ) / dot( uu, nik ) );
+        const double tD  = RiD * RiD - dot( xcD, xcD );
+        r.ok             = tD >= 0.0;
+        if ( r.ok )
+        {
+            const d3 nn = unit( cross( nik, nij ) ) * sqrt( tD );
+            const d3 x1 = xcD - nn, x2 = xcD + nn;
+            r.x0        = mk( float( x1.x ), float( x1.y ), float( x1.z ) );
+            r.x1        = mk( float( x2.x ), float( x2.y ), float( x2.z ) );
+        }
+        return r;
+    }
+
     // Probe position of triplet (i, j, k) in i's local frame, mirroring the arithmetic of
     // fillIntersections: t = (c_ik - c_ij).n_ik / (u.n_ik), x1 = c_ij + t u,
-    // x = x1 + side sqrt(max(R_i^2 - |x1|^2, 0)) n(n_ij x n_ik).
-    __device__ d3 probe_fp64( f3 dj, f3 dk, float ri, float rj, float rk, float rp, int side )
+    // x = x1 + side sqrt(max(R_i^2 - |x1|^2, 0)) n(n_ij x n_ik). Dj, Dk: c_j - c_i, c_k - c_i.
+    __device__ d3 probe_fp64( d3 Dj, d3 Dk, float ri, float rj, float rk, float rp, int side )
     {
         const double Ri = double( ri ) + double( rp ), Rj = double( rj ) + double( rp ), Rk = double( rk ) + double( rp );
-        const d3     Dj = mkd( dj ), Dk = mkd( dk );
         const double dj2 = dot( Dj, Dj ), dk2 = dot( Dk, Dk );
         const d3     cij = Dj * ( ( Ri * Ri - Rj * Rj + dj2 ) / ( 2.0 * dj2 ) );
         const d3     cik = Dk * ( ( Ri * Ri - Rk * Rk + dk2 ) / ( 2.0 * dk2 ) );
@@ -230,27 +267,11 @@ namespace sesdf
                     bool            ok  = false;
                     if ( ill_conditioned( den, t, Ri2 ) )
                     {
-                        // FP64 re-solve of xc, t and the two probe positions
-                        fp64             = true;
-                        const double RiD = double( ai.w ) + double( rp );
-                        const double RjD = double( sAbs[ w ][ la ].w ) + double( rp );
-                        const double RkD = double( sAbs[ w ][ lk ].w ) + double( rp );
-                        const d3     Dj = mkd( dj ), Dk = mkd( dk );
-                        const double dj2 = dot( Dj, Dj ), dk2 = dot( Dk, Dk );
-                        const d3     cij = Dj * ( ( RiD * RiD - RjD * RjD + dj2 ) / ( 2.0 * dj2 ) );
-                        const d3     cik = Dk * ( ( RiD * RiD - RkD * RkD + dk2 ) / ( 2.0 * dk2 ) );
-                        const d3     nij = Dj * ( 1.0 / sqrt( dj2 ) ), nik = Dk * ( 1.0 / sqrt( dk2 ) );
-                        const d3     uu  = nik - nij * dot( nik, nij );
-                        const d3     xcD = cij + uu * ( dot( cik - cij, nik ) / dot( uu, nik ) );
-                        const double tD  = RiD * RiD - dot( xcD, xcD );
-                        if ( tD >= 0.0 )
-                        {
-                            const d3 nn = unit( cross( nik, nij ) ) * sqrt( tD );
-                            const d3 x1 = xcD - nn, x2 = xcD + nn;
-                            x[ 0 ]      = mk( float( x1.x ), float( x1.y ), float( x1.z ) );
-                            x[ 1 ]      = mk( float( x2.x ), float( x2.y ), float( x2.z ) );
-                            ok          = true;
-                        }
+                        fp64               = true;
+                        const Resolved r64 = resolve_fp64( b.atoms[ i ], sAbs[ w ][ la ], sAbs[ w ][ lk ], rp );
+                        x[ 0 ]             = r64.x0;
+                        x[ 1 ]             = r64.x1;
+                        ok                 = r64.ok;
                     }
                     else if ( t >= 0.f )
                     {
@@ -447,12 +468,14 @@ namespace sesdf
             const float    Ri = ai.w + rp, Rj = aj.w + rp, Rk = ak.w + rp;
             bool           ill;
             f3             x = probe_fp32( dj, dk, Ri * Ri, Rj * Rj, Rk * Rk, side, ill );
+            float4 pos = make_float4( ci.x + x.x, ci.y + x.y, ci.z + x.z, float( side * ( 1 + hasNb ) ) );
             if ( ill )
             {
-                const d3 xd = probe_fp64( dj, dk, ai.w, aj.w, ak.w, rp, side );
-                x           = mk( float( xd.x ), float( xd.y ), float( xd.z ) );
+                // absolute position rounded once
+                const d3 xd = mkd( ci ) + probe_fp64( rel64( aj, ai ), rel64( ak, ai ), ai.w, aj.w, ak.w, rp, side );
+                pos.x = float( xd.x ), pos.y = float( xd.y ), pos.z = float( xd.z );
             }
-            b.concavePos[ q ] = make_float4( ci.x + x.x, ci.y + x.y, ci.z + x.z, float( side * ( 1 + hasNb ) ) );
+            b.concavePos[ q ] = pos;
             b.concaveIds[ q ] = make_int4( int( i ), int( j ), int( k ), 0 );
             if ( MODE == 0 )
             {

@@ -59,6 +59,7 @@ namespace sesdf
         const int2 *     surfCaps;      // per surface atom: range of its visible circles' caps
         const float4 *   caps;          // per visible circle (sector): axis, cos(half angle)
         const float4 *   segGeo;        // per segment: (centre, radius), (normal, arc > pi), n x A', B' x n
+        int              empty;         // host side: no atoms, the field is +band everywhere
     };
 
     // Query q and its molecule (batches sample one dx*dy*dz grid per molecule, molecule-major).
@@ -932,6 +933,13 @@ namespace sesdf
         }
     }
 
+    __global__ void zero_grad_kernel( float4 * g, uint32_t n )
+    {
+        const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
+        if ( a < n )
+            g[ a ] = make_float4( 0.f, 0.f, 0.f, 0.f );
+    }
+
     __global__ void gather_grad_kernel( const float4 * gradSorted, const uint32_t * s2i, uint32_t n, float4 * gradInput )
     {
         const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1142,6 +1150,11 @@ namespace
         if ( hipSetDevice( h.device ) != hipSuccess )
             return SESDF_ERR_HIP;
         s.exterior = flags & 1;
+        if ( h.n == 0 )
+        {
+            s.empty = 1; // no surface: nothing to prepare, every sample reads +band
+            return SESDF_OK;
+        }
         sesdf_status pst = prepare_surface( h );
         if ( pst != SESDF_OK )
             return pst;
@@ -1164,7 +1177,17 @@ namespace
         return SESDF_OK;
     }
 
-    // blocks of 4 waves for one wave per 4^3 brick of grid vertices
+    // +band (code: clamped) for every sample of an empty surface.
+    __global__ void sdf_fill_empty_kernel( float * out, uint64_t * codes, uint64_t n, float band )
+    {
+        for ( uint64_t q = uint64_t( blockIdx.x ) * blockDim.x + threadIdx.x; q < n; q += uint64_t( gridDim.x ) * blockDim.x )
+        {
+            out[ q ] = band;
+            if ( codes )
+                codes[ q ] = uint64_t( SdfClamped ) << 40;
+        }
+    }
+
     // One wave per brick: brick costs differ by two orders of magnitude (outside vs surface), so
     // the hardware dispatcher, not a static stride, balances them. Grids beyond 2^31 blocks stride.
     uint32_t brick_blocks( const SdfArgs & s, uint32_t molNb )
@@ -1195,7 +1218,10 @@ extern "C"
         sesdf_status st = sdf_common( h, s, flags );
         if ( st != SESDF_OK || npoints == 0 )
             return st;
-        sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+        if ( s.empty )
+            sdf_fill_empty_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( d_out, d_codes, npoints, band );
+        else
+            sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -1211,6 +1237,11 @@ extern "C"
         sesdf_status st = sdf_common( h, s, 1 );
         if ( st != SESDF_OK )
             return st;
+        if ( s.empty )
+        {
+            *exterior_label = 0;
+            return SESDF_OK;
+        }
         if ( hipMemcpyAsync( host_labels, h.sdf.component, size_t( h.n ) * 4, hipMemcpyDeviceToHost, h.stream ) != hipSuccess ||
              hipMemcpyAsync( exterior_label, h.sdf.component + h.n, 4, hipMemcpyDeviceToHost, h.stream ) != hipSuccess ||
              hipStreamSynchronize( h.stream ) != hipSuccess )
@@ -1238,7 +1269,10 @@ extern "C"
         sesdf_status st = sdf_common( h, s, flags );
         if ( st != SESDF_OK || s.n == 0 )
             return st;
-        sdf_grid_kernel<<<brick_blocks( s, 1 ), 256, 0, h.stream>>>( h.b, s, 1u );
+        if ( s.empty )
+            sdf_fill_empty_kernel<<<grid_blocks( s.n ), 256, 0, h.stream>>>( d_out, nullptr, s.n, band );
+        else
+            sdf_grid_kernel<<<brick_blocks( s, 1 ), 256, 0, h.stream>>>( h.b, s, 1u );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -1283,7 +1317,10 @@ extern "C"
         sesdf_status st = sdf_common( h, s, 0 );
         if ( st != SESDF_OK )
             return st;
-        sdf_grid_kernel<<<brick_blocks( s, uint32_t( M ) ), 256, 0, h.stream>>>( h.b, s, uint32_t( M ) );
+        if ( s.empty )
+            sdf_fill_empty_kernel<<<grid_blocks( s.n ), 256, 0, h.stream>>>( d_out, nullptr, s.n, band );
+        else
+            sdf_grid_kernel<<<brick_blocks( s, uint32_t( M ) ), 256, 0, h.stream>>>( h.b, s, uint32_t( M ) );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -1308,10 +1345,10 @@ extern "C"
         s.recompute  = d_codes ? 0 : 1;
         s.gradSorted = reinterpret_cast<float4 *>( d_grad_sorted );
         sesdf_status st = sdf_common( h, s, flags );
-        if ( st != SESDF_OK )
-            return st;
-        if ( hipMemsetAsync( d_grad_sorted, 0, size_t( h.n ) * 16, h.stream ) != hipSuccess )
-            return SESDF_ERR_HIP;
+        if ( st != SESDF_OK || s.empty )
+            return st; // no atoms: no gradient
+        // cleared by a kernel, not a memset node, so that callers may capture this call in a graph
+        zero_grad_kernel<<<( h.n + 255 ) / 256, 256, 0, h.stream>>>( s.gradSorted, h.n );
         if ( npoints )
             sdf_backward_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
         if ( h.n )

@@ -74,10 +74,22 @@ def compare(gpu, orc, atoms_in, mode):
                 u[k] = max(u.get(k, 0), v)
             return u
         sg, so = und(sg), und(so)
+    # exterior path: a partially visible circle that holds a concave patch is promoted to status 2
+    # (ssesdf operations.cu:301-303), so a 1/2 difference on a circle of an ambiguous patch is in band
+    promo_amb = set()
+    if mode == 1:
+        co_ids, co_pos = orc["concave_ids"], orc["concave_pos64"]
+        amb_q = np.nonzero(orc["amb_concave"])[0]
+        tris = [tuple(concave_key(co_ids[q], co_pos[q], s2i_o, atoms_in)[:3]) for q in amb_q]
+        tris += [tuple(sorted((int(s2i_o[i]), int(s2i_o[j]), int(s2i_o[k])))) for i, j, k, _ in orc["amb_rejected"]]
+        for t in tris:
+            promo_amb.update({(t[0], t[1]), (t[0], t[2]), (t[1], t[2])})
     nbad = nband = 0
     for k in set(sg) & set(so):
         if sg[k] != so[k]:
             if tuple(sorted(k)) in amb_circles:
+                nband += 1
+            elif {sg[k], so[k]} == {1, 2} and tuple(sorted(k)) in promo_amb:
                 nband += 1
             else:
                 nbad += 1

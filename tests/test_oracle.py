@@ -284,3 +284,33 @@ def test_comparator_detects_planted_mistake():
     arr["segments"] = arr["segments"][:0]
     rep = parity.compare(arr, f64, atoms, 0)
     assert rep["mismatch"]["concave"] >= 1
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_neighbor_lists_given_back_reproduce_the_query(mode):
+    """The oracle's neighbour-list override (used to check builds truncated at K in another grid
+    order) reproduces the grid query's build when given its own lists, and has teeth: lists that
+    lost an entry change the result."""
+    atoms = synth.globule(800, synth.SEED_BASE + 5)
+    a = O.build(atoms, mode=mode, precision=64, track=False)
+    s2i, K = a["sorted_to_initial"].astype(np.int64), a.K
+    nb = a["nb_ids"].reshape(-1, K)
+    cnt = np.zeros(len(atoms), np.uint32)
+    ids = np.zeros((len(atoms), K), np.uint32)
+    cnt[s2i] = a["nb_count"]
+    for i in range(len(atoms)):
+        ids[s2i[i], : a["nb_count"][i]] = s2i[nb[i, : a["nb_count"][i]]]
+    b = O.build(atoms, mode=mode, precision=64, track=False, neighbors=(cnt, ids))
+    for name in ("nb_count", "nb_ids", "status", "concave_ids", "concave_pos"):
+        np.testing.assert_array_equal(a[name], b[name], err_msg=name)
+    short = cnt.copy()
+    short[short > 0] -= 1
+    c = O.build(atoms, mode=mode, precision=64, track=False, neighbors=(short, ids))
+    assert not np.array_equal(a["concave_ids"], c["concave_ids"])
+    # another sorted order (untruncated lists): the same surface, in other indices
+    order = np.random.default_rng(2).permutation(len(atoms))
+    d = O.build(atoms, mode=mode, precision=64, track=True, order=order)
+    assert np.array_equal(d["sorted_to_initial"], order)
+    rep = parity.compare(as_build_arrays(d), O.build(atoms, mode=mode, precision=64, track=True), atoms, mode)
+    parity.assert_parity(rep)
+    assert sum(rep["band"].values()) == 0
