@@ -333,7 +333,7 @@ namespace sesdf
     // Sectors, intersected circles and full circles of atom i at the offsets of the scan.
     __global__ void __launch_bounds__( CirclesWaves * WaveSize ) circles_emit_kernel( Buffers b )
     {
-        const int      w    = threadIdx.x / WaveSize;
+        const int      w    = wave_in_block();
         const int      lane = lane_id();
         const uint32_t i    = xcd_block() * CirclesWaves + w;
         if ( i >= b.n )

@@ -34,6 +34,13 @@ namespace sesdf
     __device__ __forceinline__ f3    cross( f3 a, f3 b ) { return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
     __device__ __forceinline__ f3    unit( f3 v ) { return v * ( 1.0f / sqrtf( dot( v, v ) ) ); }
 
+    // Hardware square root, reciprocal and reciprocal square root (about 1 ulp), where a result
+    // only feeds comparisons far wider than that: the correctly rounded forms expand to over ten
+    // instructions each.
+    __device__ __forceinline__ float fsqrt( float x ) { return __builtin_amdgcn_sqrtf( x ); }
+    __device__ __forceinline__ float frcp( float x ) { return __builtin_amdgcn_rcpf( x ); }
+    __device__ __forceinline__ float frsq( float x ) { return __builtin_amdgcn_rsqf( x ); }
+
     __device__ __forceinline__ d3     mkd( f3 a ) { return { a.x, a.y, a.z }; }
     __device__ __forceinline__ d3     operator+( d3 a, d3 b ) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
     __device__ __forceinline__ d3     operator-( d3 a, d3 b ) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }

@@ -23,7 +23,8 @@ namespace sesdf
     constexpr int      InterWaves  = 4;
     constexpr uint32_t RecMinus    = 1u << 31;
     constexpr uint32_t RecHasNb    = 1u << 30;
-    constexpr uint32_t PairRound   = 512; // candidate pairs per compaction round (LDS list per wave)
+    constexpr uint32_t PairRound   = 192; // candidate pairs per compaction round (LDS list per wave;
+                                          // 192 keeps a 4-wave block under 22.8 KB: 7 blocks per CU)
 
     struct CircleGeo
     {
@@ -39,6 +40,18 @@ namespace sesdf
         c.centre       = dj * t;
         c.radius2      = Ri2 - len2( c.centre );
         c.normal       = dj * ( 1.0f / sqrtf( d2 ) );
+        return c;
+    }
+
+    // circle_geo with hardware reciprocal square root (about 1 ulp), for decisions only.
+    __device__ __forceinline__ CircleGeo circle_geo_fast( f3 dj, float Ri2, float Rj2 )
+    {
+        CircleGeo   c;
+        const float d2 = len2( dj );
+        const float r  = frsq( d2 );
+        c.normal       = dj * r;
+        c.centre       = dj * ( ( Ri2 - Rj2 + d2 ) * ( 0.5f * r * r ) );
+        c.radius2      = Ri2 - len2( c.centre );
         return c;
     }
 
@@ -145,14 +158,13 @@ namespace sesdf
     template<int KMAX, int WAVES>
     __global__ void __launch_bounds__( WAVES * WaveSize ) intersections_count_kernel( Buffers b )
     {
-        __shared__ uint32_t sIdx[ WAVES ][ KMAX ];
         __shared__ float4   sAtom[ WAVES ][ KMAX ];   // (c_l - c_i, R_l^2)
         __shared__ float4   sAbs[ WAVES ][ KMAX ];    // (c_l, r_l) as stored
         __shared__ uint16_t sCand[ WAVES ][ KMAX ];
         __shared__ uint4    sTouch[ WAVES ][ TouchStride ];
         __shared__ uint32_t sPair[ WAVES ][ PairRound ];
 
-        const int      w    = threadIdx.x / WaveSize;
+        const int      w    = wave_in_block();
         const int      lane = lane_id();
         const uint32_t i    = xcd_block() * WAVES + w;
         if ( i >= b.n )
@@ -177,7 +189,6 @@ namespace sesdf
                 const uint32_t j  = b.nbIds[ size_t( i ) * K + l ];
                 const float4   aj = b.atoms[ j ];
                 const float    Rj = aj.w + rp;
-                sIdx[ w ][ l ]    = j;
                 sAtom[ w ][ l ]   = make_float4( aj.x - ci.x, aj.y - ci.y, aj.z - ci.z, Rj * Rj );
                 sAbs[ w ][ l ]    = aj;
                 cand              = j > i && b.status[ size_t( i ) * K + l ] == 1;
@@ -211,7 +222,7 @@ namespace sesdf
             {
                 // row-major triangular index -> (a, c) with a < c
                 const float fm = float( 2 * m - 1 );
-                int         ai = int( ( fm - sqrtf( fm * fm - 8.f * float( p ) ) ) * 0.5f );
+                int         ai = int( ( fm - fsqrt( fm * fm - 8.f * float( p ) ) ) * 0.5f ); // corrected below
                 ai             = max( 0, min( ai, int( m ) - 2 ) );
                 while ( ai > 0 && uint32_t( ai ) * ( 2 * m - ai - 1 ) / 2 > p )
                     ai--;
@@ -225,7 +236,10 @@ namespace sesdf
                 // j's list, evaluated with the same operands and operations; the slot of k in j's
                 // list is found at emission.
                 if ( truncated )
-                    member = find_in_list( b.nbIds + size_t( sIdx[ w ][ la ] ) * K, b.nbCount[ sIdx[ w ][ la ] ], sIdx[ w ][ lk ] ) >= 0;
+                {
+                    const uint32_t j = b.nbIds[ size_t( i ) * K + la ], k = b.nbIds[ size_t( i ) * K + lk ];
+                    member           = find_in_list( b.nbIds + size_t( j ) * K, b.nbCount[ j ], k ) >= 0;
+                }
                 else
                 {
                     const float4 Aj = sAbs[ w ][ la ], Ak = sAbs[ w ][ lk ];
@@ -253,15 +267,17 @@ namespace sesdf
                 const uint32_t la = sCand[ w ][ a ];
                 const uint32_t lk = sCand[ w ][ cc ];
                 {
+                    // Geometry for the decisions only (hardware rcp / rsq, about 1 ulp, far inside
+                    // the predicates' epsilon): the stored probe positions are solved again at fill.
                     const float4    aj  = sAtom[ w ][ la ];
                     const float4    ak  = sAtom[ w ][ lk ];
                     const f3        dj  = xyz( aj ), dk = xyz( ak );
-                    const CircleGeo ij  = circle_geo( dj, Ri2, aj.w );
-                    const CircleGeo ik  = circle_geo( dk, Ri2, ak.w );
+                    const CircleGeo ij  = circle_geo_fast( dj, Ri2, aj.w );
+                    const CircleGeo ik  = circle_geo_fast( dk, Ri2, ak.w );
                     const f3        u   = ik.normal - ij.normal * dot( ik.normal, ij.normal );
                     const float     den = dot( u, ik.normal );
                     const float     q   = dot( ik.centre - ij.centre, ik.normal );
-                    const f3        xc  = mk( ( u.x * q ) / den, ( u.y * q ) / den, ( u.z * q ) / den ) + ij.centre;
+                    const f3        xc  = u * ( q * frcp( den ) ) + ij.centre;
                     const float     t   = Ri2 - len2( xc );
                     f3              x[ 2 ];
                     bool            ok  = false;
@@ -275,7 +291,8 @@ namespace sesdf
                     }
                     else if ( t >= 0.f )
                     {
-                        const f3 nn = unit( cross( ik.normal, ij.normal ) ) * sqrtf( t );
+                        const f3 cr = cross( ik.normal, ij.normal );
+                        const f3 nn = cr * ( frsq( len2( cr ) ) * fsqrt( t ) );
                         x[ 0 ]      = xc - nn;
                         x[ 1 ]      = xc + nn;
                         ok          = true;
@@ -285,45 +302,41 @@ namespace sesdf
                         // visible iff outside every other SAS sphere of N(i) (operations.inl:415-430).
                         // A sphere containing a point of both circles touches both, so only the
                         // spheres in touch(ij) & touch(ik) can occlude (exact pruning).
-                        bool vis0 = true, vis1 = true;
+                        // per probe, the least of |c_l - x|^2 - R_l^2 over the occluder candidates l
+                        // (visible iff positive); integer / float state keeps the loops light
+                        float mn0 = 1.f, mn1 = 1.f;
+                        auto  occlude = [ & ]( uint32_t l ) {
+                            const float4 al = sAtom[ w ][ l ];
+                            const f3     v0 = xyz( al ) - x[ 0 ], v1 = xyz( al ) - x[ 1 ];
+                            mn0 = fminf( mn0, fmaf( v0.x, v0.x, fmaf( v0.y, v0.y, fmaf( v0.z, v0.z, -al.w ) ) ) );
+                            mn1 = fminf( mn1, fmaf( v1.x, v1.x, fmaf( v1.y, v1.y, fmaf( v1.z, v1.z, -al.w ) ) ) );
+                        };
                         if ( haveTouch && uint32_t( a ) < TouchStride && cc < TouchStride )
                         {
+                            // a touch set leaves out its circle's own sphere, so j and k are not in
+                            // touch(ij) & touch(ik)
                             const uint4 ta = sTouch[ w ][ a ], tb = sTouch[ w ][ cc ];
-                            uint64_t    o0 = ( uint64_t( ta.y & tb.y ) << 32 ) | ( ta.x & tb.x );
-                            uint64_t    o1 = ( uint64_t( ta.w & tb.w ) << 32 ) | ( ta.z & tb.z );
-                            while ( ( o0 | o1 ) && ( vis0 || vis1 ) )
+                            uint64_t    o  = ( uint64_t( ta.y & tb.y ) << 32 ) | ( ta.x & tb.x );
+                            while ( o != 0 && fmaxf( mn0, mn1 ) > 0.f )
                             {
-                                uint32_t l;
-                                if ( o0 )
-                                {
-                                    l = uint32_t( __ffsll( (unsigned long long)o0 ) - 1 );
-                                    o0 &= o0 - 1;
-                                }
-                                else
-                                {
-                                    l = 64u + uint32_t( __ffsll( (unsigned long long)o1 ) - 1 );
-                                    o1 &= o1 - 1;
-                                }
-                                if ( l == la || l == lk )
-                                    continue;
-                                const float4 al = sAtom[ w ][ l ];
-                                const f3     cl = xyz( al );
-                                vis0            = vis0 && len2( cl - x[ 0 ] ) - al.w > 0.f;
-                                vis1            = vis1 && len2( cl - x[ 1 ] ) - al.w > 0.f;
+                                occlude( uint32_t( __builtin_ctzll( o ) ) );
+                                o &= o - 1;
+                            }
+                            o = ( uint64_t( ta.w & tb.w ) << 32 ) | ( ta.z & tb.z );
+                            while ( o != 0 && fmaxf( mn0, mn1 ) > 0.f )
+                            {
+                                occlude( 64u + uint32_t( __builtin_ctzll( o ) ) );
+                                o &= o - 1;
                             }
                         }
                         else
-                            for ( uint32_t l = 0; l < ni && ( vis0 || vis1 ); l++ )
-                            {
-                                if ( l == la || l == lk )
-                                    continue;
-                                const float4 al = sAtom[ w ][ l ];
-                                const f3     cl = xyz( al );
-                                vis0            = vis0 && len2( cl - x[ 0 ] ) - al.w > 0.f;
-                                vis1            = vis1 && len2( cl - x[ 1 ] ) - al.w > 0.f;
-                            }
+                            for ( uint32_t l = 0; l < ni && fmaxf( mn0, mn1 ) > 0.f; l++ )
+                                if ( l != la && l != lk )
+                                    occlude( l );
+                        const bool vis0 = mn0 > 0.f, vis1 = mn1 > 0.f;
                         // hasNeighbors: the probe is closer than rp to the plane of the centres (:447-463)
-                        const f3       pn     = unit( cross( ij.normal, unit( dj - dk ) ) );
+                        const f3       pc     = cross( ij.normal, dj - dk );
+                        const f3       pn     = pc * frsq( len2( pc ) );
                         const uint32_t packed = la | ( lk << 10 );
                         for ( int s = 0; s < 2; s++ )
                         {
@@ -374,7 +387,7 @@ namespace sesdf
     template<int MODE>
     __global__ void __launch_bounds__( InterWaves * WaveSize ) intersections_emit_kernel( Buffers b )
     {
-        const int      w    = threadIdx.x / WaveSize;
+        const int      w    = wave_in_block();
         const int      lane = lane_id();
         const uint32_t i    = xcd_block() * InterWaves + w;
         if ( i >= b.n )
@@ -614,7 +627,7 @@ namespace sesdf
         const uint32_t      Iw = b.ctl->intersectionWithNbNb;
         if ( Iw + b.ctl->intersectionNoNbNb > b.capRecords || Iw > b.capProbeNb || ( b.ctl->overflow & OverflowIntersections ) )
             return;
-        const int      w     = threadIdx.x / WaveSize;
+        const int      w     = wave_in_block();
         const int      lane  = lane_id();
         const float    rp    = b.probeRadius;
         const float    rmax  = __uint_as_float( b.ctl->radiusMaxBits );
@@ -701,7 +714,7 @@ namespace sesdf
     // Exterior path: status-2 circles per atom (full, or promoted by a visible intersection).
     __global__ void __launch_bounds__( 256 ) torus_count_kernel( Buffers b )
     {
-        const int      w    = threadIdx.x / WaveSize;
+        const int      w    = wave_in_block();
         const int      lane = lane_id();
         const uint32_t i    = xcd_block() * 4 + w;
         if ( i >= b.n )
@@ -720,7 +733,7 @@ namespace sesdf
     // Torus records (MegaMol layout) of atom i's status-2 circles, in slot order.
     __global__ void __launch_bounds__( 256 ) torus_emit_kernel( Buffers b )
     {
-        const int      w    = threadIdx.x / WaveSize;
+        const int      w    = wave_in_block();
         const int      lane = lane_id();
         const uint32_t i    = xcd_block() * 4 + w;
         if ( i >= b.n )

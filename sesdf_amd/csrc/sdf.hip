@@ -209,10 +209,8 @@ namespace sesdf
     }
 
     // ---- primitive element tests -------------------------------------------------------------
-    // Hardware square root and reciprocal (1 ulp): the field is compared to FP64 at 1e-4 A, and
-    // the correctly rounded forms cost over ten instructions each in these loops.
-    __device__ __forceinline__ float fsqrt( float x ) { return __builtin_amdgcn_sqrtf( x ); }
-    __device__ __forceinline__ float frcp( float x ) { return __builtin_amdgcn_rcpf( x ); }
+    // (fsqrt / frcp: hardware forms, 1 ulp: the field is compared to FP64 at 1e-4 A, and the
+    // correctly rounded forms cost over ten instructions each in these loops.)
 
     __device__ __forceinline__ void full_circle_test( const Buffers & b, SdfLane & L, uint32_t f )
     {
