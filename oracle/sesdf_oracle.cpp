@@ -51,7 +51,8 @@ namespace orc
     // helper_math normalize: v * rsqrt(dot(v, v))
     template<class T> inline V3<T> normalize( V3<T> v ) { return v * ( T( 1 ) / std::sqrt( dot( v, v ) ) ); }
     template<class T> inline T     sgn( T f ) { return std::copysign( T( 1 ), f ); } // math.cuh:13-16
-    template<class T> inline T     clampv( T v, T lo, T hi ) { return std::min( std::max( v, lo ), hi ); }
+    // helper_math.h clamp: fmaxf(lo, fminf(v, hi)) -- a NaN argument yields hi
+    template<class T> inline T     clampv( T v, T lo, T hi ) { return std::fmax( lo, std::fmin( v, hi ) ); }
     template<class T> inline T     det3( V3<T> a, V3<T> b, V3<T> c ) { return dot( a, cross( b, c ) ); } // math.cuh:27-30
 
     struct AtomF

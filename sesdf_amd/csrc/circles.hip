@@ -366,11 +366,14 @@ namespace sesdf
                 {
                     const float4     aj = b.atoms[ j ];
                     const float      Rj = aj.w + rp;
+                    // half angle of the cap: acos(n . x / Ri) for a point x of the circle
+                    // (buildConvexPatchesSectors, operations.inl:175-228, takes x = centre +
+                    // radius * orthogonalVector(n)); n . x is the plane offset n . centre. For
+                    // spheres tangent within the predicate's epsilon (radius^2 < 0) the
+                    // reference's radius is NaN and its clamp (fmaxf(-1, fminf(v, 1))) makes the
+                    // angle 0; the plane offset is then Ri or just above it: the same 0.
                     const CircleTest c  = make_circle( xyz( aj ) - ci, Ri2, Rj * Rj );
-                    const f3         o  = orthogonal_vector( c.normal );
-                    const f3         x  = c.centre + o * sqrtf( c.radius2 );
-                    const f3         xr = mk( x.x / Ri, x.y / Ri, x.z / Ri );
-                    const float      ca = fminf( fmaxf( dot( c.normal, xr ), -1.f ), 1.f );
+                    const float      ca = fmaxf( -1.f, fminf( dot( c.normal, c.centre ) / Ri, 1.f ) );
                     b.sectors[ dst ]    = make_float4( c.normal.x, c.normal.y, c.normal.z, acosf( ca ) );
                 }
                 else

@@ -184,3 +184,31 @@ def test_large_probe_radius_wide_lists(gpu, mode):
     err = np.abs(out.cpu().numpy().astype(np.float64) - ref)
     print("sdf max err", err.max(), flush=True)
     assert err.max() <= 1e-4
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_tangent_spheres_within_epsilon(gpu, mode):
+    """Spheres farther apart than Ri + Rj by less than the predicate's slack are neighbours whose
+    circle has radius^2 < 0. The reference's sector angle is then acos(clamp(NaN)) = acos(1) = 0
+    (helper_math clamp is fmaxf(lo, fminf(v, hi))): a point cap, which hides no part of the face."""
+    d = np.float32(np.sqrt(np.float64(6.2) ** 2 + 5e-5))
+    atoms = np.array([[0, 0, 0, 1.7], [d, 0, 0, 1.7]], np.float32)
+    surf, orc, rep = _check(atoms, mode)
+    if mode == 1:
+        return
+    data = surf.get_data()
+    assert data["sectors"].shape == (2, 4) and (data["sectors"][:, 3] == 0.0).all()
+    np.testing.assert_array_equal(orc["sectors"][:, 3], [0.0, 0.0])
+    import torch
+
+    # the face of each atom stays exposed: inside the SAS ball of atom 0, on its far side, the field
+    # is rp - (R - |q - c|) (a hidden face would leave only the degenerate circle to win)
+    q = np.array([[-2.5, 0.0, 0.0], [-2.0, 1.0, 0.3]], np.float32)
+    P = torch.from_numpy(np.c_[q, np.zeros(len(q), np.float32)]).cuda()
+    out = torch.empty(len(q), device="cuda")
+    surf.sample_points_ptr(P.data_ptr(), len(q), out.data_ptr(), band=2.0)
+    torch.cuda.synchronize()
+    ref = orc.sdf(q.astype(np.float64), band=2.0)
+    face = 1.4 - (3.1 - np.linalg.norm(q.astype(np.float64), axis=1))
+    np.testing.assert_allclose(ref, face, atol=1e-6)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, atol=1e-4)
