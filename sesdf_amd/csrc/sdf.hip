@@ -208,6 +208,29 @@ namespace sesdf
         }
     }
 
+    // The stencil narrowed to the cells within r_lane of some lane that still searches (need):
+    // rows outside it hold no atom whose row bound could pass. r_lane carries the same slack as the
+    // row bound, plus 1e-3 A for the rounding of this cell range.
+    template<bool COOP>
+    __device__ __forceinline__ Stencil narrow( Stencil t, const Molecule & M, const SdfLane & L, bool need, float r )
+    {
+        constexpr int Big = 1 << 30;
+        const float   rr  = r + 1e-3f;
+        int x0 = need ? int( floorf( ( L.q.x - rr - M.ox ) / M.cell ) ) : Big, x1 = need ? int( floorf( ( L.q.x + rr - M.ox ) / M.cell ) ) : -Big;
+        int y0 = need ? int( floorf( ( L.q.y - rr - M.oy ) / M.cell ) ) : Big, y1 = need ? int( floorf( ( L.q.y + rr - M.oy ) / M.cell ) ) : -Big;
+        int z0 = need ? int( floorf( ( L.q.z - rr - M.oz ) / M.cell ) ) : Big, z1 = need ? int( floorf( ( L.q.z + rr - M.oz ) / M.cell ) ) : -Big;
+        if constexpr ( COOP )
+        {
+            x0 = uniform_int<COOP>( wave_min_int( x0 ) ), x1 = uniform_int<COOP>( wave_max_int( x1 ) );
+            y0 = uniform_int<COOP>( wave_min_int( y0 ) ), y1 = uniform_int<COOP>( wave_max_int( y1 ) );
+            z0 = uniform_int<COOP>( wave_min_int( z0 ) ), z1 = uniform_int<COOP>( wave_max_int( z1 ) );
+        }
+        t.x0 = max( t.x0, x0 ), t.x1 = min( t.x1, x1 );
+        t.y0 = max( t.y0, y0 ), t.y1 = min( t.y1, y1 );
+        t.z0 = max( t.z0, z0 ), t.z1 = min( t.z1, z1 );
+        return t;
+    }
+
     // ---- primitive element tests -------------------------------------------------------------
     // (fsqrt / frcp: hardware forms, 1 ulp: the field is compared to FP64 at 1e-4 A, and the
     // correctly rounded forms cost over ten instructions each in these loops.)
@@ -551,7 +574,12 @@ namespace sesdf
                 lane_nearest( s, L, a0, a1, extC );
         };
         for_block( t, M, L.q, nearest );
-        for_rest( t, M, L.q, nearest );
+        {
+            // the rest of the stencil, narrowed to what the bounds found in the near block allow
+            const bool  needA = L.active && L.g >= 0.f;
+            const float gCut  = s.exterior ? L.g + 1e-4f : fminf( L.g, fmaxf( s.band - rp, 0.f ) );
+            for_rest( narrow<COOP>( t, M, L, needA, gCut + Rmax + 1e-3f ), M, L.q, nearest );
+        }
 
         // 0: +band (nothing within the band), 1: outside A, 2: inside (pass B)
         int state = 0;
@@ -585,7 +613,8 @@ namespace sesdf
             };
             for_block( t, M, L.q, [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, true, false ); } );
             for_block( t, M, L.q, [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, false, true ); } );
-            for_rest( t, M, L.q, [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, true, true ); } );
+            for_rest( narrow<COOP>( t, M, L, needB, L.best + Rmax + 1e-3f ), M, L.q,
+                      [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, true, true ); } );
         }
 
         if ( state == 0 )
