@@ -15,6 +15,7 @@ vp = ctypes.c_void_p
 fp = ctypes.POINTER(ctypes.c_float)
 
 SESDF_OK = 0
+ERR_INVALID = 1  # SESDF_ERR_INVALID
 MODE_COMPLETE = 0
 MODE_EXTERIOR = 1
 

@@ -29,6 +29,12 @@ namespace
         g_lastError = std::string( what ) + ": " + hipGetErrorString( e );
         return SESDF_ERR_HIP;
     }
+
+    sesdf_status set_error( sesdf_status st, const char * what )
+    {
+        g_lastError = what;
+        return st;
+    }
 } // namespace
 
 #define HIP_TRY( expr )                                                                                                          \
@@ -463,7 +469,14 @@ extern "C"
     {
         if ( !info || !out || !info->atom_offsets || info->molecule_nb == 0 )
             return SESDF_ERR_INVALID;
-        *out    = nullptr;
+        *out = nullptr;
+        // offsets: molecule m holds atoms [offsets[m], offsets[m + 1]); they start at 0 and never
+        // decrease (empty molecules are allowed). Checked before any device work.
+        if ( info->atom_offsets[ 0 ] != 0 )
+            return set_error( SESDF_ERR_INVALID, "atom_offsets[0] must be 0" );
+        for ( uint32_t m = 0; m < info->molecule_nb; m++ )
+            if ( info->atom_offsets[ m + 1 ] < info->atom_offsets[ m ] )
+                return set_error( SESDF_ERR_INVALID, "atom_offsets must not decrease" );
         auto * h = new Handle();
         sesdf_status st = common_setup( *h, info->device, info->stream, info->max_neighbors, info->max_intersection_neighbors,
                                         info->probe_radius, info->mode, info->cell_size );

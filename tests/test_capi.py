@@ -64,3 +64,26 @@ def test_synthetic_generators_deterministic():
     assert all(np.min(np.abs(r - np.array([1.52, 1.55, 1.70, 1.80]))) < 1e-6 for r in radii)
     sizes = synth.batch_sizes(4096)
     assert sizes.min() >= 20 and sizes.max() <= 5000
+
+
+def test_batch_offsets_validated_before_device_work():
+    """Offsets must start at 0 and never decrease; both are rejected without touching a GPU."""
+    import ctypes
+
+    import numpy as np
+
+    import sesdf_amd._lib as L
+
+    lib = L.lib()
+    atoms = np.zeros((4, 4), np.float32)
+    for offs in ([1, 2, 4], [0, 3, 2], [0, 4, 3]):
+        o = np.array(offs, np.uint64)
+        info = L.BatchInfo()
+        info.xyzr = atoms.ctypes.data
+        info.atom_offsets = o.ctypes.data
+        info.molecule_nb = len(offs) - 1
+        info.probe_radius = 1.4
+        h = ctypes.c_void_p()
+        assert lib.sesdf_batch_create(ctypes.byref(info), ctypes.byref(h)) == L.ERR_INVALID, offs
+        assert not h.value
+        assert b"atom_offsets" in lib.sesdf_last_error()

@@ -22,9 +22,7 @@ def _eq(a, b, what):
     np.testing.assert_array_equal(a, b, err_msg=what)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
-def test_batch_slices_equal_standalone_builds(gpu, mode):
-    atoms, offs = small_batch()
+def _slices_equal_standalone(atoms, offs, mode):
     cls_b, cls_s = (SesdfBatch, Sesdf) if mode == 0 else (SsesdfBatch, Ssesdf)
     print(f"[gpu] batch of {len(offs) - 1} molecules, {len(atoms)} atoms, mode {mode}", flush=True)
     batch = cls_b(atoms, offs)
@@ -36,7 +34,7 @@ def test_batch_slices_equal_standalone_builds(gpu, mode):
         sd = single.get_data()
         bm = molecule_slice(data, o, m)
         _eq(bm["atoms"], sd["atoms"], "atoms")
-        _eq(s2i[int(o["atoms"][m]):int(o["atoms"][m + 1])] - int(offs[m]), single.get_atom_indices(), "sorted_to_initial")
+        _eq(s2i[int(o["atoms"][m]):int(o["atoms"][m + 1])] - int(offs[m]), single.get_atom_indices().astype(np.int64), "sorted_to_initial")
         _eq(bm["concave_patches_id"][:, :3], sd["concave_patches_id"][:, :3], "concave ids")
         _eq(bm["concave_patches_position"][:, :3], sd["concave_patches_position"][:, :3], "concave positions")
         if mode == 0:
@@ -47,6 +45,53 @@ def test_batch_slices_equal_standalone_builds(gpu, mode):
         else:
             for k in ("circle_position", "circle_axis", "circle_vs"):
                 _eq(bm[k], sd[k], k)
+        single.close()
+    return batch
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_slices_equal_standalone_builds(gpu, mode):
+    atoms, offs = small_batch()
+    _slices_equal_standalone(atoms, offs, mode)
+
+
+def empty_and_tiny_batch():
+    """Molecules of 60, 0, 1, 2, 0, 200 and 0 atoms."""
+    g60 = synth.globule(60, synth.SEED_BASE + 35)
+    g200 = synth.globule(200, synth.SEED_BASE + 36) + np.float32([40.0, 0.0, 0.0, 0.0])
+    one = np.array([[5.0, 5.0, 5.0, 1.7]], np.float32)
+    two = np.array([[0.0, -9.0, 0.0, 1.7], [3.0, -9.0, 0.5, 1.55]], np.float32)
+    parts = [g60, np.zeros((0, 4), np.float32), one, two, np.zeros((0, 4), np.float32), g200, np.zeros((0, 4), np.float32)]
+    offs = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.uint64)
+    return np.concatenate(parts).astype(np.float32), offs
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_with_empty_and_tiny_molecules(gpu, mode):
+    atoms, offs = empty_and_tiny_batch()
+    batch = _slices_equal_standalone(atoms, offs, mode)
+    if mode == 1:
+        return
+    import torch
+
+    M, dim, margin, band = len(offs) - 1, 8, 3.0, 2.0
+    out = torch.empty(M * dim ** 3, device="cuda")
+    batch.sample_grids_ptr(dim, out.data_ptr(), margin=margin, band=band)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().reshape(M, -1)
+    for m in range(M):
+        mol = atoms[int(offs[m]):int(offs[m + 1])]
+        if len(mol) == 0:
+            assert (got[m] == band).all()  # no surface: +band everywhere
+            continue
+        single = Sesdf(mol)
+        lo = (mol[:, :3] - mol[:, 3:4]).min(0) - margin
+        hi = (mol[:, :3] + mol[:, 3:4]).max(0) + margin
+        sp = ((hi - lo) / (dim - 1)).astype(np.float32)
+        ref = torch.empty(dim ** 3, device="cuda")
+        single.sample_grid_ptr(lo.astype(np.float32), sp, (dim, dim, dim), ref.data_ptr(), band=band)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(got[m], ref.cpu().numpy(), atol=1e-5)
         single.close()
 
 
