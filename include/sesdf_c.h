@@ -131,7 +131,9 @@ typedef struct
     uint16_t   max_intersection_neighbors;
 } sesdf_internal;
 
-/* bcs::Sesdf::Sesdf / bcs::Ssesdf::Ssesdf (sesdf/src/sesdf/sesdf.cu:20-75, sesdf/src/ssesdf/ssesdf.cu:19-66). */
+/* bcs::Sesdf::Sesdf / bcs::Ssesdf::Ssesdf (sesdf/src/sesdf/sesdf.cu:20-75, sesdf/src/ssesdf/ssesdf.cu:19-66).
+ * SESDF_ERR_INVALID, before any device work, for non-finite coordinates or radii and for a given
+ * aabb that is not finite or has min > max. No atoms is valid (an empty surface). */
 sesdf_status sesdf_create( const sesdf_create_info * info, sesdf_handle ** out );
 /* bcs::Sesdf::~Sesdf (sesdf.cu:159-170). */
 sesdf_status sesdf_destroy( sesdf_handle * h );
@@ -223,6 +225,8 @@ typedef struct
     int32_t          use_graph;
 } sesdf_batch_info;
 
+/* SESDF_ERR_INVALID, before any device work, unless atom_offsets starts at 0 and never decreases
+ * (empty molecules are valid), and for non-finite coordinates or radii. */
 sesdf_status sesdf_batch_create( const sesdf_batch_info * info, sesdf_handle ** out );
 /* One dim^3 SDF vertex grid per molecule over its vdW bounding box grown by margin, written
  * molecule-major to d_out (molecule_nb * dim^3 floats). Complete-surface batches. */

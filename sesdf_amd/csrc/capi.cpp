@@ -357,6 +357,25 @@ namespace sesdf
     }
 } // namespace sesdf
 
+// A bounding box the grid can be laid out on: finite, min <= max on each axis (an infinite or
+// inverted box would give a brick of no cells, or none that fits).
+static bool box_ok( const float * bx )
+{
+    for ( int d = 0; d < 3; d++ )
+        if ( !std::isfinite( bx[ d ] ) || !std::isfinite( bx[ d + 3 ] ) || !( bx[ d ] <= bx[ d + 3 ] ) )
+            return false;
+    return true;
+}
+
+// Every coordinate and radius of atoms [b, e) finite.
+static bool atoms_finite( const float * xyzr, uint64_t b, uint64_t e )
+{
+    for ( uint64_t i = 4 * b; i < 4 * e; i++ )
+        if ( !std::isfinite( xyzr[ i ] ) )
+            return false;
+    return true;
+}
+
 static void box_of( const float * xyzr, uint64_t b, uint64_t e, float * out )
 {
     float mn[ 3 ] = { 3.4e38f, 3.4e38f, 3.4e38f }, mx[ 3 ] = { -3.4e38f, -3.4e38f, -3.4e38f };
@@ -428,6 +447,16 @@ extern "C"
         }
         if ( info->atom_nb && !info->xyzr )
             return SESDF_ERR_INVALID;
+        // input checks on the host, before any device work
+        if ( !atoms_finite( info->xyzr, 0, info->atom_nb ) )
+            return set_error( SESDF_ERR_INVALID, "atom coordinates and radii must be finite" );
+        {
+            bool given = false;
+            for ( int d = 0; d < 6; d++ )
+                given = given || info->aabb[ d ] != 0.f;
+            if ( given && !box_ok( info->aabb ) )
+                return set_error( SESDF_ERR_INVALID, "aabb must be finite with min <= max" );
+        }
         auto * h = new Handle();
         sesdf_status st = common_setup( *h, info->device, info->stream, info->max_neighbors, info->max_intersection_neighbors,
                                         info->probe_radius, info->mode, info->cell_size );
@@ -447,6 +476,11 @@ extern "C"
             std::memcpy( h->boxes.data(), info->aabb, 6 * sizeof( float ) );
         else
             box_of( info->xyzr, 0, info->atom_nb, h->boxes.data() );
+        if ( !box_ok( h->boxes.data() ) )
+        {
+            sesdf_destroy( reinterpret_cast<sesdf_handle *>( h ) );
+            return set_error( SESDF_ERR_INVALID, "aabb (given, or of the atoms) must be finite with min <= max" );
+        }
         h->mols.resize( 1 );
         h->mols[ 0 ]          = Molecule {};
         h->mols[ 0 ].atomBase = 0;
@@ -477,6 +511,10 @@ extern "C"
         for ( uint32_t m = 0; m < info->molecule_nb; m++ )
             if ( info->atom_offsets[ m + 1 ] < info->atom_offsets[ m ] )
                 return set_error( SESDF_ERR_INVALID, "atom_offsets must not decrease" );
+        if ( info->atom_offsets[ info->molecule_nb ] && !info->xyzr )
+            return SESDF_ERR_INVALID;
+        if ( info->atom_offsets[ info->molecule_nb ] < 0xffffffffull && !atoms_finite( info->xyzr, 0, info->atom_offsets[ info->molecule_nb ] ) )
+            return set_error( SESDF_ERR_INVALID, "atom coordinates and radii must be finite" );
         auto * h = new Handle();
         sesdf_status st = common_setup( *h, info->device, info->stream, info->max_neighbors, info->max_intersection_neighbors,
                                         info->probe_radius, info->mode, info->cell_size );
@@ -502,6 +540,11 @@ extern "C"
             h->mols[ m ].atomBase = uint32_t( info->atom_offsets[ m ] );
             h->mols[ m ].atomNb   = uint32_t( info->atom_offsets[ m + 1 ] - info->atom_offsets[ m ] );
             box_of( info->xyzr, info->atom_offsets[ m ], info->atom_offsets[ m + 1 ], h->boxes.data() + 6 * m );
+            if ( !box_ok( h->boxes.data() + 6 * m ) )
+            {
+                sesdf_destroy( reinterpret_cast<sesdf_handle *>( h ) );
+                return set_error( SESDF_ERR_INVALID, "atom coordinates must be finite" );
+            }
         }
         layout_grid( *h );
         initial_capacities( *h );

@@ -87,3 +87,42 @@ def test_batch_offsets_validated_before_device_work():
         assert lib.sesdf_batch_create(ctypes.byref(info), ctypes.byref(h)) == L.ERR_INVALID, offs
         assert not h.value
         assert b"atom_offsets" in lib.sesdf_last_error()
+
+
+def test_non_finite_input_and_bad_aabb_rejected_before_device_work():
+    import ctypes
+
+    import numpy as np
+
+    import sesdf_amd._lib as L
+
+    lib = L.lib()
+    good = np.array([[0, 0, 0, 1.7], [3, 0, 0, 1.7]], np.float32)
+    cases = []
+    for bad in (np.nan, np.inf):
+        a = good.copy()
+        a[1, 2] = bad
+        cases.append((a, [0.0] * 6))
+    cases.append((good, [0, 0, 0, -1, 5, 5]))        # min > max
+    cases.append((good, [0, 0, 0, np.inf, 5, 5]))    # not finite
+    for atoms, aabb in cases:
+        info = L.CreateInfo()
+        info.xyzr = atoms.ctypes.data
+        info.atom_nb = len(atoms)
+        info.probe_radius = 1.4
+        for d in range(6):
+            info.aabb[d] = aabb[d]
+        h = ctypes.c_void_p()
+        assert lib.sesdf_create(ctypes.byref(info), ctypes.byref(h)) == L.ERR_INVALID
+        assert not h.value
+    a = np.concatenate([good, good]).copy()
+    a[3, 3] = np.inf
+    o = np.array([0, 2, 4], np.uint64)
+    info = L.BatchInfo()
+    info.xyzr = a.ctypes.data
+    info.atom_offsets = o.ctypes.data
+    info.molecule_nb = 2
+    info.probe_radius = 1.4
+    h = ctypes.c_void_p()
+    assert lib.sesdf_batch_create(ctypes.byref(info), ctypes.byref(h)) == L.ERR_INVALID
+    assert b"finite" in lib.sesdf_last_error()
