@@ -711,35 +711,47 @@ namespace sesdf
         }
     }
 
-    // Exterior path: status-2 circles per atom (full, or promoted by a visible intersection).
+    // Exterior path: status-2 circles per atom (full, or promoted by a visible intersection), 16
+    // slots per load: statuses are 0, 1, 2, so bit 1 of a byte marks status 2. One thread per atom:
+    // an atom has about one torus record, so a wave per atom would leave its lanes idle.
+    __device__ __forceinline__ uint32_t torus_word( const uint4 v, uint32_t q )
+    {
+        return q == 0 ? v.x : q == 1 ? v.y : q == 2 ? v.z : v.w;
+    }
+    // status-2 bits of the 4 slots [4 wd, 4 wd + 4) below ni
+    __device__ __forceinline__ uint32_t torus_bits( uint32_t word, uint32_t wd, uint32_t ni )
+    {
+        const uint32_t valid = ni > 4 * wd ? min( ni - 4 * wd, 4u ) : 0u;
+        const uint32_t keep  = valid >= 4 ? ~0u : ( 1u << ( 8 * valid ) ) - 1u;
+        return word & keep & 0x02020202u;
+    }
+
     __global__ void __launch_bounds__( 256 ) torus_count_kernel( Buffers b )
     {
-        const int      w    = wave_in_block();
-        const int      lane = lane_id();
-        const uint32_t i    = xcd_block() * 4 + w;
+        const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
         if ( i >= b.n )
             return;
         const uint32_t ni  = b.nbCount[ i ];
+        const uint4 *  row = reinterpret_cast<const uint4 *>( b.status + size_t( i ) * b.K ); // K % 16 == 0
         uint32_t       cnt = 0;
-        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+        for ( uint32_t v = 0; 16 * v < ni; v++ )
         {
-            const uint32_t l = lb + lane;
-            cnt += uint32_t( __popcll( ballot( l < ni && b.status[ size_t( i ) * b.K + l ] == 2 ) ) );
+            const uint4 x = row[ v ];
+#pragma unroll
+            for ( uint32_t q = 0; q < 4; q++ )
+                cnt += uint32_t( __popc( torus_bits( torus_word( x, q ), 4 * v + q, ni ) ) );
         }
-        if ( lane == 0 )
-            b.atomCnt[ i ] = cnt;
+        b.atomCnt[ i ] = cnt;
     }
 
     // Torus records (MegaMol layout) of atom i's status-2 circles, in slot order.
     __global__ void __launch_bounds__( 256 ) torus_emit_kernel( Buffers b )
     {
-        const int      w    = wave_in_block();
-        const int      lane = lane_id();
-        const uint32_t i    = xcd_block() * 4 + w;
+        const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
         if ( i >= b.n )
             return;
-        const uint32_t cnt = b.torusBase[ i + 1 ] - b.torusBase[ i ];
-        if ( cnt == 0 )
+        uint32_t run = b.torusBase[ i ];
+        if ( b.torusBase[ i + 1 ] == run )
             return;
         const uint32_t K   = b.K;
         const float    rp  = b.probeRadius;
@@ -747,17 +759,23 @@ namespace sesdf
         const float4   ai  = b.atoms[ i ];
         const f3       ci  = xyz( ai );
         const float    Ri  = ai.w + rp;
-        uint32_t       run = b.torusBase[ i ];
-        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+        const uint4 *  row = reinterpret_cast<const uint4 *>( b.status + size_t( i ) * K );
+        for ( uint32_t v = 0; 16 * v < ni; v++ )
         {
-            const uint32_t l    = lb + lane;
-            const bool     is2  = l < ni && b.status[ size_t( i ) * K + l ] == 2;
-            const uint64_t mask = ballot( is2 );
-            if ( is2 )
+            const uint4 x = row[ v ];
+            for ( uint32_t q = 0; q < 4; q++ )
             {
-                const uint32_t dst = run + rank_below( mask );
-                if ( dst < b.capTorus )
+                uint32_t bits = torus_bits( torus_word( x, q ), 4 * v + q, ni );
+                while ( bits )
                 {
+                    const uint32_t l   = 4 * ( 4 * v + q ) + uint32_t( __ffs( int( bits ) ) - 1 ) / 8;
+                    const uint32_t dst = run++;
+                    bits &= bits - 1;
+                    if ( dst >= b.capTorus )
+                    {
+                        set_overflow( b.ctl, OverflowTorus );
+                        continue;
+                    }
                     const uint32_t  j     = b.nbIds[ size_t( i ) * K + l ];
                     const float4    aj    = b.atoms[ j ];
                     const f3        dj    = xyz( aj ) - ci;
@@ -776,10 +794,7 @@ namespace sesdf
                     b.torusAxis[ dst ]    = make_float4( ta.x, ta.y, ta.z, rp );
                     b.torusVs[ dst ]      = make_float4( Cout.x, Cout.y, Cout.z, dist );
                 }
-                else
-                    set_overflow( b.ctl, OverflowTorus );
             }
-            run += uint32_t( __popcll( mask ) );
         }
     }
 
@@ -892,7 +907,7 @@ namespace sesdf
     {
         if ( b.n == 0 || b.mode != 1 )
             return;
-        const uint32_t blocks = ( b.n + 3 ) / 4;
+        const uint32_t blocks = ( b.n + 255 ) / 256;
         torus_count_kernel<<<blocks, 256, 0, s>>>( b );
         ScanArgs a {};
         a.n           = b.n;
