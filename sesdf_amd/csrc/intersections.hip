@@ -20,7 +20,6 @@
 
 namespace sesdf
 {
-    constexpr int      InterWaves  = 4;
     constexpr uint32_t RecMinus    = 1u << 31;
     constexpr uint32_t RecHasNb    = 1u << 30;
     constexpr uint32_t PairRound   = 192; // candidate pairs per compaction round (LDS list per wave;
@@ -384,31 +383,72 @@ namespace sesdf
     // Place the staged records of atom i: with-neighbour ones at wStart[i] + rank (from the front),
     // the others at nStart[i] + rank (from the back) -- the |w|-descending order of the reference's
     // thrust::sort, made deterministic -- and count intersections per circle (complete surface).
+    // A wave takes 64 consecutive atoms (about one record each) and spreads their records over its
+    // lanes, 64 at a time; a record's rank within its atom is the count of same-kind records before
+    // it: those of its atom earlier in the chunk (ballot) plus those of earlier chunks (LDS carry).
+    constexpr int EmitWaves = 4;
     template<int MODE>
-    __global__ void __launch_bounds__( InterWaves * WaveSize ) intersections_emit_kernel( Buffers b )
+    __global__ void __launch_bounds__( EmitWaves * WaveSize ) intersections_emit_kernel( Buffers b )
     {
+        __shared__ uint32_t sPre[ EmitWaves ][ WaveSize + 1 ];
+        __shared__ uint32_t sCarry[ EmitWaves ][ 2 ][ WaveSize ];
         const int      w    = wave_in_block();
         const int      lane = lane_id();
-        const uint32_t i    = xcd_block() * InterWaves + w;
-        if ( i >= b.n )
+        const uint32_t a0   = ( xcd_block() * EmitWaves + uint32_t( w ) ) * WaveSize;
+        if ( a0 >= b.n )
             return;
-        const uint32_t nW   = b.wStart[ i + 1 ] - b.wStart[ i ];
-        const uint32_t nRec = nW + b.nStart[ i + 1 ] - b.nStart[ i ];
-        if ( nRec == 0 )
-            return;
-        const uint32_t   K     = b.K;
-        const uint32_t * stage = b.stage + size_t( i ) * b.stageStride;
-        const uint32_t   ci    = min( b.ctl->intersectedCircleNb, b.capTrimmed );
-        uint32_t         runW = b.wStart[ i ], runN = b.nStart[ i ];
-        for ( uint32_t rb = 0; rb < nRec; rb += WaveSize )
+        const uint32_t i0 = a0 + uint32_t( lane );
+        uint32_t       wS = 0, nS = 0, cnt = 0;
+        if ( i0 < b.n )
         {
-            const uint32_t r    = rb + lane;
-            const bool     live = r < nRec;
-            uint32_t       rec  = live ? stage[ r ] : 0u;
-            const bool     isW  = live && ( rec & RecHasNb );
-            const uint64_t mW = ballot( isW ), mN = ballot( live && !isW );
+            wS  = b.wStart[ i0 ];
+            nS  = b.nStart[ i0 ];
+            cnt = ( b.wStart[ i0 + 1 ] - wS ) + ( b.nStart[ i0 + 1 ] - nS );
+        }
+        const uint32_t incl  = wave_incl_scan( cnt );
+        const uint32_t total = wave_bcast( incl, WaveSize - 1 );
+        if ( total == 0 )
+            return;
+        sPre[ w ][ lane + 1 ]  = incl;
+        sCarry[ w ][ 0 ][ lane ] = 0u;
+        sCarry[ w ][ 1 ][ lane ] = 0u;
+        if ( lane == 0 )
+            sPre[ w ][ 0 ] = 0u;
+        wave_sync();
+        const uint32_t K  = b.K;
+        const uint32_t ci = min( b.ctl->intersectedCircleNb, b.capTrimmed );
+        for ( uint32_t base = 0; base < total; base += WaveSize )
+        {
+            const uint32_t e    = base + uint32_t( lane );
+            const bool     live = e < total;
+            int            t    = 0; // owning atom of record e: largest t with sPre[t] <= e
             if ( live )
             {
+                int lo = 0, hi = WaveSize;
+                while ( hi - lo > 1 )
+                {
+                    const int mid = ( lo + hi ) >> 1;
+                    if ( sPre[ w ][ mid ] <= e )
+                        lo = mid;
+                    else
+                        hi = mid;
+                }
+                t = lo;
+            }
+            const uint32_t i    = a0 + uint32_t( t );
+            const uint32_t r    = e - sPre[ w ][ t ];
+            uint32_t       rec  = live ? b.stage[ size_t( i ) * b.stageStride + r ] : 0u;
+            const bool     isW  = live && ( rec & RecHasNb );
+            const uint64_t mW = ballot( isW ), mN = ballot( live && !isW );
+            // lanes of this chunk that hold earlier records of the same atom
+            const uint32_t first = sPre[ w ][ t ] > base ? sPre[ w ][ t ] - base : 0u;
+            const uint64_t same  = lanes_below() & ~( ( uint64_t( 1 ) << first ) - 1 );
+            const uint32_t kind  = isW ? 0u : 1u;
+            const uint32_t rank  = sCarry[ w ][ kind ][ t ] + uint32_t( __popcll( ( isW ? mW : mN ) & same ) );
+            wave_sync();
+            if ( live )
+            {
+                atomicAdd( &sCarry[ w ][ kind ][ t ], 1u );
                 const uint32_t la = rec & 1023u, lk = ( rec >> 10 ) & 1023u;
                 const uint32_t j  = b.nbIds[ size_t( i ) * K + la ];
                 const uint32_t k  = b.nbIds[ size_t( i ) * K + lk ];
@@ -418,7 +458,7 @@ namespace sesdf
                 if ( found < 0 )
                     atomicAdd( &b.ctl->jkAnomalyNb, 1u );
                 rec |= jk << 20;
-                const uint32_t idx = isW ? runW + rank_below( mW ) : runN + rank_below( mN );
+                const uint32_t idx = ( isW ? b.wStart[ i ] : b.nStart[ i ] ) + rank;
                 if ( idx < b.capRecords )
                     b.records[ isW ? idx : b.capRecords - 1 - idx ] = make_uint4( i, j, k, rec );
                 else
@@ -437,8 +477,7 @@ namespace sesdf
                         atomicAdd( b.circleIntNb + t2, 1u );
                 }
             }
-            runW += uint32_t( __popcll( mW ) );
-            runN += uint32_t( __popcll( mN ) );
+            wave_sync(); // carries complete before the next chunk reads them
         }
     }
     // 8-bit compare-and-swap through the enclosing aligned 32-bit word.
@@ -830,7 +869,6 @@ namespace sesdf
     {
         if ( b.n == 0 )
             return;
-        const uint32_t blocks = ( b.n + InterWaves - 1 ) / InterWaves;
         ScanArgs       a {};
         a.n           = b.n;
         a.ticket      = TicketIntersections;
@@ -845,10 +883,11 @@ namespace sesdf
         a.maxOut      = &b.ctl->maxAtomRecords;
         a.sumOut      = &b.ctl->fp64ResolveNb;
         launch_scan<2>( b, a, s );
+        const uint32_t emitBlocks = ( b.n + EmitWaves * WaveSize - 1 ) / ( EmitWaves * WaveSize );
         if ( b.mode == 0 )
-            intersections_emit_kernel<0><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+            intersections_emit_kernel<0><<<emitBlocks, EmitWaves * WaveSize, 0, s>>>( b );
         else
-            intersections_emit_kernel<1><<<blocks, InterWaves * WaveSize, 0, s>>>( b );
+            intersections_emit_kernel<1><<<emitBlocks, EmitWaves * WaveSize, 0, s>>>( b );
     }
 
     // Per-circle list offsets (exclusive scan of the intersection counts) and segment offsets
