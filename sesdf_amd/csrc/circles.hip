@@ -43,6 +43,20 @@ namespace sesdf
         return c;
     }
 
+    // make_circle with hardware reciprocal square root (about 1 ulp): for the visibility
+    // decisions, whose epsilon is far wider.
+    __device__ __forceinline__ CircleTest make_circle_fast( f3 dj, float Ri2, float Rj2 )
+    {
+        CircleTest  c;
+        const float d2 = len2( dj );
+        const float r  = frsq( d2 );
+        c.centre       = dj * ( ( Ri2 - Rj2 + d2 ) * ( 0.5f * r * r ) );
+        c.radius2      = Ri2 - len2( c.centre );
+        c.normal       = dj * r;
+        c.fourR2       = 4.f * c.radius2;
+        return c;
+    }
+
     // orthogonalVector (sesdf/include/bcs/cuda/math.cuh:32-38)
     __device__ __forceinline__ f3 orthogonal_vector( f3 n )
     {
@@ -187,7 +201,9 @@ namespace sesdf
         // one dot product and three fused operations per (circle, sphere) instead of a cross product
         // and two squared lengths.
         uint32_t nV = 0, nT = 0, nF = 0;
-        for ( uint32_t lb = 0; lb < K; lb += WaveSize )
+        // slots past ni are never read (statuses, touch sets, intersected-circle bits: all readers
+        // stop at the neighbour count)
+        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
         {
             const uint32_t l      = lb + lane;
             const uint32_t j      = l < ni ? sIdx[ w ][ l ] : 0u;
@@ -197,7 +213,7 @@ namespace sesdf
             if ( __builtin_expect( ballot( active ) != 0, 1 ) )
             {
                 const float4     aj     = sAtom[ w ][ min( l, K - 1 ) ];
-                const CircleTest c      = make_circle( xyz( aj ), Ri2, aj.w );
+                const CircleTest c      = make_circle_fast( xyz( aj ), Ri2, aj.w );
                 const float      twoRho = 2.f * dot( c.centre, c.normal );
                 const float      fourR2 = fmaxf( c.fourR2, 0.f );
                 // Lanes run the k loop in lockstep (k is wave-uniform). With P = A |A| and lhs >= 0
@@ -315,8 +331,6 @@ namespace sesdf
                 if ( lb / 32 + 1 < K / 32 )
                     b.intMask[ size_t( i ) * ( K / 32 ) + lb / 32 + 1 ] = uint32_t( mT >> 32 );
             }
-            if ( lb + WaveSize >= ni && MODE == 1 )
-                break;
         }
         if ( lane == 0 )
         {
