@@ -212,3 +212,22 @@ def test_tangent_spheres_within_epsilon(gpu, mode):
     face = 1.4 - (3.1 - np.linalg.norm(q.astype(np.float64), axis=1))
     np.testing.assert_allclose(ref, face, atol=1e-6)
     np.testing.assert_allclose(out.cpu().numpy(), ref, atol=1e-4)
+
+
+def random_cluster(seed):
+    """5-60 atoms, radii 1.2-2.0 A, in a box sized for a liquid-like density (overlaps allowed)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(5, 61))
+    side = (n / 0.05) ** (1 / 3)
+    xyz = rng.uniform(0, side, (n, 3))
+    r = rng.uniform(1.2, 2.0, n)
+    return np.c_[xyz, r].astype(np.float32)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_small_clusters(gpu, seed):
+    """Random small clusters with random probe radii (1.0-2.0 A), both paths, against the oracle."""
+    atoms = random_cluster(synth.SEED_BASE + 100 + seed)
+    rp = float(np.random.default_rng(seed).uniform(1.0, 2.0))
+    for mode in (0, 1):
+        _check(atoms, mode, probe_radius=rp)
