@@ -66,6 +66,39 @@ namespace sesdf
         return unit( cross( n, ref ) );
     }
 
+    // Circle (i, slot l) tested again against every other sphere of N(i) with h^2 = |c_k x n|^2, for
+    // circles with a sphere centre within 0.03 A of their axis. |c_k|^2 - e^2 cancels there: its
+    // rounding (a few 1e-6 for |c_k| up to 6 A) moves 2 R h, the side the predicates compare
+    // against A, by about R 4e-6 / h, which reaches their 1e-3 scale as h shrinks (above 0.03 A it
+    // stays under 4e-4).
+    __device__ __forceinline__ void circle_exact( const float4 * atom, const float2 * aux, uint32_t ni, uint32_t l, const CircleTest & c,
+                                                  float twoRho, bool & hidden, bool & touched, uint32_t ( &wd )[ 4 ] )
+    {
+        hidden = false, touched = false;
+        wd[ 0 ] = wd[ 1 ] = wd[ 2 ] = wd[ 3 ] = 0u;
+        for ( uint32_t k = 0; k < ni && !hidden; k++ )
+        {
+            if ( k == l )
+                continue;
+            const float4 ak  = atom[ k ];
+            const float2 ax  = aux[ k ];
+            const float  e   = dot( c.normal, xyz( ak ) );
+            const float  h2  = len2( cross( xyz( ak ), c.normal ) );
+            const float  A   = fmaf( twoRho, e, ax.y );
+            const float  lhs = c.fourR2 * h2;
+            const float  A2  = A * A;
+            hidden           = A > 0.f && lhs < A2;
+            if ( A > 0.f || lhs > A2 )
+            {
+                touched = true;
+                wd[ 0 ] |= k < 32 ? 1u << k : 0u;
+                wd[ 1 ] |= k >= 32 && k < 64 ? 1u << ( k - 32 ) : 0u;
+                wd[ 2 ] |= k >= 64 && k < 96 ? 1u << ( k - 64 ) : 0u;
+                wd[ 3 ] |= k >= 96 && k < 128 ? 1u << ( k - 96 ) : 0u;
+            }
+        }
+    }
+
     template<int KMAX, int MODE, int WAVES>
     __global__ void __launch_bounds__( WAVES * WaveSize ) circles_count_kernel( Buffers b )
     {
@@ -184,9 +217,9 @@ namespace sesdf
             const float  K2  = len2( xyz( ak ) );
             sAux[ w ][ l ]   = make_float2( K2, ( ak.w + PredicateEps ) - K2 - Ri2 );
         }
-        // up to 3 padding slots past ni that neither hide, touch nor come near any circle: e = 0,
+        // up to 8 padding slots past ni that neither hide, touch nor come near any circle: e = 0,
         // h^2 = 1e30, A = -1e30 (so A |A| = -inf)
-        if ( uint32_t( lane ) < ( ( 4u - ( ni & 3u ) ) & 3u ) )
+        if ( uint32_t( lane ) < min( 8u, uint32_t( KMAX ) - ni ) )
         {
             sAtom[ w ][ ni + lane ] = make_float4( 0.f, 0.f, 0.f, 0.f );
             sAux[ w ][ ni + lane ]  = make_float2( 1e30f, -1e30f );
@@ -201,9 +234,86 @@ namespace sesdf
         // one dot product and three fused operations per (circle, sphere) instead of a cross product
         // and two squared lengths.
         uint32_t nV = 0, nT = 0, nF = 0;
+        // The exterior path tests only circles with j > i: the suffix of the ascending list from s0.
+        uint32_t s0 = 0;
+        if ( MODE == 1 )
+            for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+                s0 += uint32_t( __popcll( ballot( lb + lane < ni && sIdx[ w ][ lb + lane ] < i ) ) );
+        const uint32_t nc = ni - s0; // circles to test
+        if ( nc > 0 && nc <= 32 && ni <= 64 )
+        {
+            // Up to 32 circles: both half-waves take the same circle (lane & 31) and split the
+            // spheres, the lower half k in [0, hk), the upper [hk, 2 hk) (inert padding past ni),
+            // so the lockstep loop is half as long. Hidden / near-axis states join across the
+            // halves in the wave masks, the touch words by one lane exchange.
+            const uint32_t   c      = uint32_t( lane ) & 31u;
+            const bool       upper  = lane >= 32;
+            const uint32_t   l      = s0 + c;
+            const bool       act    = c < nc;
+            const uint32_t   j      = act ? sIdx[ w ][ l ] : 0u;
+            const float4     aj     = sAtom[ w ][ act ? l : 0u ];
+            const CircleTest cg     = make_circle_fast( xyz( aj ), Ri2, aj.w );
+            const float      twoRho = 2.f * dot( cg.centre, cg.normal );
+            const float      fourR2 = fmaxf( cg.fourR2, 0.f );
+            const uint32_t   hk     = ( ( ni + 1u ) / 2u + 3u ) & ~3u; // <= 32; 2 hk <= ni + 7
+            const uint32_t   off    = upper ? hk : 0u;
+            const uint64_t   actM   = ballot( act ) & 0xffffffffull;
+            uint64_t         hidM = 0, nearM = 0;
+            uint32_t         tw   = 0; // touch bits of slots off + [0, hk)
+            for ( uint32_t kk = 0; kk < hk; kk += 4 )
+            {
+#pragma unroll
+                for ( uint32_t u = 0; u < 4; u++ )
+                {
+                    const uint32_t k     = kk + u + off;
+                    const float4   ak    = sAtom[ w ][ k ];
+                    const float2   ax    = sAux[ w ][ k ];
+                    const float    e     = fmaf( cg.normal.z, ak.z, fmaf( cg.normal.y, ak.y, cg.normal.x * ak.x ) );
+                    const float    h2    = fmaf( -e, e, ax.x );
+                    const float    A     = fmaf( twoRho, e, ax.y );
+                    const float    lhs   = fourR2 * h2;
+                    const float    P     = A * fabsf( A );
+                    const bool     other = k != l;
+                    nearM |= ballot( other && h2 < 1e-3f );
+                    hidM |= ballot( other && lhs < P );
+                    tw |= other && lhs > -P ? 1u << ( kk + u ) : 0u;
+                }
+                if ( ( actM & ~( hidM | ( hidM >> 32 ) ) ) == 0 )
+                    break; // every circle is hidden: the rest no longer matters
+            }
+            const uint64_t hidC  = hidM | ( hidM >> 32 );
+            const uint64_t nearC = nearM | ( nearM >> 32 );
+            bool           hidden  = ( hidC >> c ) & 1u;
+            const bool     near    = ( nearC >> c ) & 1u;
+            const uint32_t twUpper = __shfl_xor( tw, 32, WaveSize );
+            const uint64_t T       = upper ? 0ull : ( uint64_t( tw ) | ( uint64_t( twUpper ) << hk ) ); // slots [0, 2 hk)
+            uint32_t       wd[ 4 ] = { uint32_t( T ), uint32_t( T >> 32 ), 0u, 0u };
+            bool           touched = T != 0ull;
+            if ( __builtin_expect( ballot( !upper && act && near ) != 0, 0 ) )
+                if ( !upper && act && near )
+                    circle_exact( sAtom[ w ], sAux[ w ], ni, l, cg, twoRho, hidden, touched, wd );
+            const bool     own = !upper && act;
+            const uint32_t st  = own ? ( hidden ? 0u : ( touched ? 1u : 2u ) ) : 0u;
+            for ( uint32_t z = lane; z < s0; z += WaveSize ) // exterior: circles with j < i are not evaluated
+                b.status[ size_t( i ) * K + z ] = 0u;
+            if ( own )
+                b.status[ size_t( i ) * K + l ] = uint8_t( st );
+            const uint64_t mT = ballot( own && st == 1 && j > i );
+            if ( own && st == 1 && j > i )
+            {
+                const uint32_t r = rank_below( mT );
+                if ( r < TouchStride && K <= 128 )
+                    b.touch[ size_t( i ) * TouchStride + r ] = make_uint4( wd[ 0 ], wd[ 1 ], wd[ 2 ], wd[ 3 ] );
+            }
+            nV = uint32_t( __popcll( ballot( own && st != 0 ) ) );
+            nT = uint32_t( __popcll( mT ) );
+            nF = uint32_t( __popcll( ballot( own && st == 2 && j > i ) ) );
+            if ( MODE == 0 && lane == 0 )
+                b.intMask[ size_t( i ) * ( K / 32 ) ] = uint32_t( mT ); // s0 = 0: lane c is slot c
+        }
         // slots past ni are never read (statuses, touch sets, intersected-circle bits: all readers
         // stop at the neighbour count)
-        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+        else for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
         {
             const uint32_t l      = lb + lane;
             const uint32_t j      = l < ni ? sIdx[ w ][ l ] : 0u;
@@ -274,36 +384,11 @@ namespace sesdf
                 bool touched = anyT != 0u;
                 if ( __builtin_expect( ballot( active && near ) != 0, 0 ) )
                 {
-                    // Some sphere centre lies within 0.03 A of a circle's axis. |c_k|^2 - e^2 cancels
-                    // there: its rounding (a few 1e-6 for |c_k| up to 6 A) moves 2 R h, the side
-                    // the predicates compare against A, by about R 4e-6 / h, which reaches their
-                    // 1e-3 scale as h shrinks (above 0.03 A it stays under 4e-4). Those circles
-                    // are tested again with h^2 = |c_k x n|^2.
                     if ( active && near )
                     {
-                        hidden = false, touched = false;
-                        w0 = w1 = w2 = w3 = 0u;
-                        for ( uint32_t k = 0; k < ni && !hidden; k++ )
-                        {
-                            if ( k == l )
-                                continue;
-                            const float4 ak  = sAtom[ w ][ k ];
-                            const float2 ax  = sAux[ w ][ k ];
-                            const float  e   = dot( c.normal, xyz( ak ) );
-                            const float  h2  = len2( cross( xyz( ak ), c.normal ) );
-                            const float  A   = fmaf( twoRho, e, ax.y );
-                            const float  lhs = c.fourR2 * h2;
-                            const float  A2  = A * A;
-                            hidden           = A > 0.f && lhs < A2;
-                            if ( A > 0.f || lhs > A2 )
-                            {
-                                touched = true;
-                                w0 |= k < 32 ? 1u << k : 0u;
-                                w1 |= k >= 32 && k < 64 ? 1u << ( k - 32 ) : 0u;
-                                w2 |= k >= 64 && k < 96 ? 1u << ( k - 64 ) : 0u;
-                                w3 |= k >= 96 && k < 128 ? 1u << ( k - 96 ) : 0u;
-                            }
-                        }
+                        uint32_t wd[ 4 ];
+                        circle_exact( sAtom[ w ], sAux[ w ], ni, l, c, twoRho, hidden, touched, wd );
+                        w0 = wd[ 0 ], w1 = wd[ 1 ], w2 = wd[ 2 ], w3 = wd[ 3 ];
                     }
                 }
                 touch0 = uint64_t( w0 ) | ( uint64_t( w1 ) << 32 );
