@@ -118,11 +118,23 @@ def kernel_roofline(kernel, alg, kms):
             "traffic": profiled_traffic(kernel), "kernel": kernel, "alg_bytes_per_launch": int(alg), "kernel_ms": kms}
 
 
+def tuned(surf, args):
+    """Launch tuning before any timed build: the autotuner's choice (cell edge, block shapes) unless
+    --cell fixes the cell edge."""
+    if args.cell > 0:
+        return {"cell_size": args.cell, "circle_waves": 4, "intersection_waves": 4}
+    from sesdf_amd.autotune import autotune
+
+    t = autotune(surf, log=log)
+    return {k: t[k] for k in ("cell_size", "circle_waves", "intersection_waves")}
+
+
 def leg_main(args, atoms, out):
     from sesdf_amd import Sesdf
 
     n = atoms.shape[0]
     surf = Sesdf(atoms, probe_radius=1.4, cell_size=args.cell, use_graph=True)
+    tune = tuned(surf, args)
     ms = time_builds(surf, args.steps, args.warmup)
     counts = surf.counts()
     stages = stage_times(surf)
@@ -138,6 +150,7 @@ def leg_main(args, atoms, out):
         "counts": {k: int(v) for k, v in counts.items()},
         "roofline": lines[dominant],
         "roofline_kernels": lines,
+        "tuning": tune,
     })
     log(f"main: {ms:.3f} ms/build, {n / (ms * 1e-3) / 1e6:.1f} M atoms/s")
     return surf
@@ -147,9 +160,10 @@ def leg_exterior(args, atoms, out):
     from sesdf_amd import Ssesdf
 
     ext = Ssesdf(atoms, probe_radius=1.4, cell_size=args.cell, use_graph=True)
+    tune = tuned(ext, args)
     ms = time_builds(ext, args.steps, args.warmup)
     out["exterior"] = {"atoms_per_s": atoms.shape[0] / (ms * 1e-3), "ms_per_build": ms,
-                       "torus": int(ext.counts()["torus_nb"]), "concave": int(ext.counts()["concave_patch_nb"])}
+                       "torus": int(ext.counts()["torus_nb"]), "concave": int(ext.counts()["concave_patch_nb"]), "tuning": tune}
     log(f"exterior: {ms:.3f} ms/build")
 
 
@@ -163,6 +177,14 @@ def leg_sdf(args, atoms, surf, out):
     dim = 256
     sp = ((hi - lo) / (dim - 1)).astype(np.float32)
     res = torch.empty(dim ** 3, device="cuda")
+    tune = None
+    if args.cell <= 0:
+        # the grid that suits the build does not suit the sampling: tune for build + grid
+        from sesdf_amd.autotune import autotune
+
+        t = autotune(surf, log=log, after=lambda: surf.sample_grid_ptr(lo, sp, (dim,) * 3, res.data_ptr(), band=2.0),
+                     tag=f"+sdf{dim}")
+        tune = {k: t[k] for k in ("cell_size", "circle_waves", "intersection_waves")}
     stream = surf.stream()
     surf.sample_grid_ptr(lo, sp, (dim,) * 3, res.data_ptr(), band=2.0)  # warm-up
     L.device_synchronize()
@@ -176,7 +198,8 @@ def leg_sdf(args, atoms, surf, out):
     ms = e0.elapsed_ms(e1) / reps
     r = res.cpu().numpy()
     out["sdf"] = {"grid": "256^3", "band_A": 2.0, "ms": ms, "samples_per_s": dim ** 3 / (ms * 1e-3),
-                  "inside_fraction": float((r < 0).mean()), "surface_band_fraction": float((np.abs(r) < 2).mean())}
+                  "inside_fraction": float((r < 0).mean()), "surface_band_fraction": float((np.abs(r) < 2).mean()),
+                  "tuning": tune}
     log(f"sdf 256^3: {ms:.3f} ms")
 
 
@@ -188,6 +211,7 @@ def leg_large(args, out):
     L.device_synchronize()
     free0, _ = L.mem_info()
     sbig = Ssesdf(big, probe_radius=1.4, cell_size=args.cell, use_graph=True)
+    tune = tuned(sbig, args)
     ms = time_builds(sbig, max(5, args.steps // 5), 2)
     free1, _ = L.mem_info()
     cb = sbig.counts()
@@ -195,7 +219,7 @@ def leg_large(args, out):
         "atoms": int(big.shape[0]), "ms_per_build": ms, "atoms_per_s": big.shape[0] / (ms * 1e-3),
         "concave": int(cb["concave_patch_nb"]), "torus": int(cb["torus_nb"]),
         "handle_device_bytes": int(cb["device_bytes"]), "device_mem_delta_bytes": int(free0 - free1),
-        "stage_ms": stage_times(sbig, 3)}
+        "stage_ms": stage_times(sbig, 3), "tuning": tune}
     log(f"large: {ms:.3f} ms/build")
     sbig.close()
 
@@ -371,7 +395,8 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "100k-atom globule with 10 voids, probe 1.4 A, complete surface, hipGraph replay",
-                   "atoms": int(atoms.shape[0]), "probe_radius": 1.4, "cell_size": args.cell if args.cell > 0 else 3.2},
+                   "atoms": int(atoms.shape[0]), "probe_radius": 1.4,
+                   "launch": "autotuned (cell edge, block shapes)" if args.cell <= 0 else f"cell {args.cell} A"},
     }
     surf = leg_main(args, atoms, out) if "main" in legs else None
     if "exterior" in legs:

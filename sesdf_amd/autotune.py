@@ -57,12 +57,14 @@ def load_cache():
 
 def autotune(surf, reps=5, use_cache=True, log=None, after=None, tag=""):
     """Pick (cell_size, circle_waves, intersection_waves) for this handle, apply it, return it.
-    With `after`, the objective is the build plus the work it enqueues (`tag` names it in the cache)."""
+    With `after`, the objective is the build plus the work it enqueues (`tag` names it in the cache).
+    The handle is left built with the chosen tuning (a new cell edge re-lays out the grid)."""
     sig = signature(surf) + tag
     cache = load_cache()
     if use_cache and sig in cache:
         best = cache[sig]
         set_tuning(surf, best["cell_size"], best["circle_waves"], best["intersection_waves"])
+        surf.build()
         return dict(best, cached=True)
     cur = {"cell_size": 3.2, "circle_waves": 4, "intersection_waves": 4}
     trials = []
@@ -85,6 +87,7 @@ def autotune(surf, reps=5, use_cache=True, log=None, after=None, tag=""):
                 best_ms, best_v = ms, v
         cur[key] = best_v
     set_tuning(surf, cur["cell_size"], cur["circle_waves"], cur["intersection_waves"])
+    surf.build()
     result = dict(cur, ms=best_ms, signature=sig)
     cache[sig] = result
     try:

@@ -154,3 +154,34 @@ def test_batch_sdf_grids_equal_standalone(gpu):
         single.sample_grid_ptr(lo.astype(np.float32), sp, (dim, dim, dim), ref.data_ptr(), band=band)
         torch.cuda.synchronize()
         np.testing.assert_allclose(got[m], ref.cpu().numpy(), atol=1e-5)
+
+
+def test_autotune_leaves_the_handle_built_and_outputs_unchanged(gpu, tmp_path, monkeypatch):
+    """A tuned handle (fresh search, then from the cache) samples the same grid as an untuned one."""
+    import torch
+
+    import sesdf_amd.autotune as T
+
+    monkeypatch.setattr(T, "_CACHE", str(tmp_path / "tuning.json"))
+    atoms = synth.globule(1500, synth.SEED_BASE + 37)
+    ref_s = Sesdf(atoms)
+    lo = atoms[:, :3].min(0) - 4
+    sp = ((atoms[:, :3].max(0) + 4 - lo) / 15).astype(np.float32)
+    ref = torch.empty(16 ** 3, device="cuda")
+    ref_s.sample_grid_ptr(lo.astype(np.float32), sp, (16, 16, 16), ref.data_ptr(), band=2.0)
+    for cached in (False, True):
+        s = Sesdf(atoms)
+        t = T.autotune(s, reps=1)
+        assert t["cached"] == cached
+        got = torch.empty(16 ** 3, device="cuda")
+        s.sample_grid_ptr(lo.astype(np.float32), sp, (16, 16, 16), got.data_ptr(), band=2.0)
+        torch.cuda.synchronize()
+        # another cell edge may settle near-ties between elements differently: last-bit differences
+        np.testing.assert_allclose(got.cpu().numpy(), ref.cpu().numpy(), atol=1e-5)
+        # the same probes, in input indices (the sorted order follows the cell edge)
+        def triples(h):
+            ids = h.get_data()["concave_patches_id"][:, :3].astype(np.int64)
+            t = np.sort(h.get_atom_indices().astype(np.int64)[ids], axis=1)
+            return t[np.lexsort(t.T[::-1])]
+        _eq(triples(s), triples(ref_s), "concave triples")
+        s.close()
