@@ -181,8 +181,8 @@ namespace sesdf
         ALLOC_TRY( h, b.torusPos, h.cap.torus );
         ALLOC_TRY( h, b.torusAxis, h.cap.torus );
         ALLOC_TRY( h, b.torusVs, h.cap.torus );
-        // look-back flags: one word per scan tile of 2048 items
-        const uint64_t tiles = std::max<uint64_t>( std::max<uint64_t>( h.cellNb, n ), h.cap.trimmed ) / 2048 + 2;
+        // look-back flags: one word per scan tile (ScanMinTile items or more)
+        const uint64_t tiles = std::max<uint64_t>( std::max<uint64_t>( h.cellNb, n ), h.cap.trimmed ) / ScanMinTile + 2;
         b.flagStride         = uint32_t( tiles + 64 );
         ALLOC_TRY( h, b.flags, uint64_t( b.flagStride ) * 4, true );
 

@@ -54,20 +54,28 @@ namespace sesdf
     }
 
     constexpr int ScanThreads = 256;
-    constexpr int ScanItems   = 8;
-    constexpr int ScanTile    = ScanThreads * ScanItems;
 
-    template<int NC>
+    // Exclusive scan, one tile of ScanThreads x ITEMS items per ticket (tiles in ticket order, so
+    // a tile only waits for tiles already being worked on). Each thread owns ITEMS consecutive
+    // items. TR: the tile's columns are read and written coalesced (item q * 256 + thread) and
+    // transposed through LDS (padded: one word per 32); without it each thread reads its own run.
+    // Small arrays take small tiles (more workgroups at once; a scan of 1e5 items is latency-bound),
+    // large ones large tiles (fewer look-back steps and tickets).
+    template<int NC, int ITEMS, bool TR>
     __global__ void __launch_bounds__( ScanThreads ) scan_kernel( ScanArgs a, Control * ctl, uint64_t * flags, uint32_t flagStride )
     {
-        __shared__ uint32_t waveSums[ NC ][ ScanThreads / WaveSize ];
+        constexpr int       T = ScanThreads, W = T / WaveSize, TILE = T * ITEMS;
+        __shared__ uint32_t lds[ TR ? TILE + TILE / 32 : 1 ];
+        __shared__ uint32_t waveSums[ NC ][ W ];
         __shared__ uint32_t tileBase[ NC ];
         __shared__ uint32_t ticket;
-        __shared__ uint32_t redMax[ ScanThreads / WaveSize ], redSum[ ScanThreads / WaveSize ];
+        __shared__ uint32_t redMax[ W ], redSum[ W ];
         const uint32_t      n     = a.countPtr ? min( *a.countPtr, a.n ) : a.n;
-        const uint32_t      tiles = ( n + ScanTile - 1 ) / ScanTile;
+        const uint32_t      tiles = ( n + TILE - 1 ) / TILE;
         const uint32_t      tag   = stage_tag( ctl, a.stage );
         const int           w     = threadIdx.x / WaveSize;
+        const int           tid   = threadIdx.x;
+        auto                pad   = []( int i ) { return i + ( i >> 5 ); };
         if ( n == 0 )
         {
             if ( blockIdx.x == 0 && threadIdx.x == 0 )
@@ -87,31 +95,55 @@ namespace sesdf
             const uint32_t t = ticket;
             if ( t >= tiles )
                 break;
-            const uint32_t base = t * ScanTile + threadIdx.x * ScanItems;
-            uint32_t       v[ NC ][ ScanItems ];
+            const uint32_t tile0 = t * TILE;
+            const uint32_t base  = tile0 + tid * ITEMS;
+            uint32_t       v[ NC ][ ITEMS ];
             uint32_t       s[ NC ], incl[ NC ];
             uint32_t       mx = 0, sm = 0;
 #pragma unroll
             for ( int c = 0; c < NC; c++ )
+            {
+                const bool       half = a.halfSecond && c == 1;
+                const uint32_t * src  = half ? a.in[ 0 ] : a.in[ c ];
+                if constexpr ( TR )
+                {
+#pragma unroll
+                    for ( int q = 0; q < ITEMS; q++ )
+                    {
+                        const uint32_t i      = tile0 + q * T + tid;
+                        lds[ pad( q * T + tid ) ] = i < n ? src[ i ] : 0u;
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for ( int q = 0; q < ITEMS; q++ )
+                        v[ c ][ q ] = lds[ pad( tid * ITEMS + q ) ];
+                    __syncthreads();
+                }
+                else
+                {
+#pragma unroll
+                    for ( int q = 0; q < ITEMS; q++ )
+                        v[ c ][ q ] = base + q < n ? src[ base + q ] : 0u;
+                }
                 s[ c ] = 0;
 #pragma unroll
-            for ( int q = 0; q < ScanItems; q++ )
-            {
-                const bool in = base + q < n;
-#pragma unroll
-                for ( int c = 0; c < NC; c++ )
+                for ( int q = 0; q < ITEMS; q++ )
                 {
-                    uint32_t x = 0;
-                    if ( in )
-                        x = ( a.halfSecond && c == 1 ) ? a.in[ 0 ][ base + q ] / 2 : a.in[ c ][ base + q ];
-                    v[ c ][ q ] = x;
-                    s[ c ] += x;
+                    v[ c ][ q ] = half ? v[ c ][ q ] / 2 : v[ c ][ q ];
+                    s[ c ] += v[ c ][ q ];
                 }
-                if ( in && a.maxOut )
-                    mx = max( mx, v[ 0 ][ q ] + ( NC > 1 && !a.halfSecond ? v[ 1 ][ q ] : 0u ) );
-                if ( in && a.sumOut )
-                    sm += a.in[ NC ][ base + q ];
             }
+            if ( a.maxOut )
+#pragma unroll
+                for ( int q = 0; q < ITEMS; q++ )
+                    mx = max( mx, v[ 0 ][ q ] + ( NC > 1 && !a.halfSecond ? v[ 1 ][ q ] : 0u ) );
+            if ( a.sumOut ) // a plain sum: read in coalesced order
+#pragma unroll
+                for ( int q = 0; q < ITEMS; q++ )
+                {
+                    const uint32_t i = tile0 + q * T + tid;
+                    sm += i < n ? a.in[ NC ][ i ] : 0u;
+                }
 #pragma unroll
             for ( int c = 0; c < NC; c++ )
             {
@@ -137,7 +169,7 @@ namespace sesdf
                 for ( int c = 0; c < NC; c++ )
                 {
                     local[ c ] = 0;
-                    for ( int q = 0; q < ScanThreads / WaveSize; q++ )
+                    for ( int q = 0; q < W; q++ )
                         local[ c ] += waveSums[ c ][ q ];
                 }
                 wave_lookback<NC>( flags, flagStride, t, tag, local, excl );
@@ -154,10 +186,13 @@ namespace sesdf
                                 *a.totals[ c ] = excl[ c ] + local[ c ];
                         }
                     }
+                    uint32_t m = 0, total = 0;
+                    for ( int q = 0; q < W; q++ )
+                        m = max( m, redMax[ q ] ), total += redSum[ q ];
                     if ( a.maxOut )
-                        atomicMax( a.maxOut, max( max( redMax[ 0 ], redMax[ 1 ] ), max( redMax[ 2 ], redMax[ 3 ] ) ) );
+                        atomicMax( a.maxOut, m );
                     if ( a.sumOut )
-                        atomicAdd( a.sumOut, redSum[ 0 ] + redSum[ 1 ] + redSum[ 2 ] + redSum[ 3 ] );
+                        atomicAdd( a.sumOut, total );
                 }
             }
             __syncthreads();
@@ -167,37 +202,76 @@ namespace sesdf
                 uint32_t run = tileBase[ c ] + incl[ c ] - s[ c ];
                 for ( int q = 0; q < w; q++ )
                     run += waveSums[ c ][ q ];
+                if constexpr ( TR )
+                {
 #pragma unroll
-                for ( int q = 0; q < ScanItems; q++ )
-                    if ( base + q < n )
+                    for ( int q = 0; q < ITEMS; q++ )
                     {
-                        a.out[ c ][ base + q ] = run;
+                        lds[ pad( tid * ITEMS + q ) ] = run;
                         run += v[ c ][ q ];
                     }
+                    __syncthreads();
+#pragma unroll
+                    for ( int q = 0; q < ITEMS; q++ )
+                    {
+                        const uint32_t i = tile0 + q * T + tid;
+                        if ( i < n )
+                            a.out[ c ][ i ] = lds[ pad( q * T + tid ) ];
+                    }
+                    __syncthreads();
+                }
+                else
+                {
+#pragma unroll
+                    for ( int q = 0; q < ITEMS; q++ )
+                        if ( base + q < n )
+                        {
+                            a.out[ c ][ base + q ] = run;
+                            run += v[ c ][ q ];
+                        }
+                }
             }
             if ( a.clearInput || a.clearAlso )
 #pragma unroll
-                for ( int q = 0; q < ScanItems; q++ )
-                    if ( base + q < n )
+                for ( int q = 0; q < ITEMS; q++ )
+                {
+                    const uint32_t i = tile0 + q * T + tid;
+                    if ( i < n )
                     {
                         if ( a.clearInput )
                             for ( int c = 0; c < NC; c++ )
                                 if ( !( a.halfSecond && c == 1 ) )
-                                    a.in[ c ][ base + q ] = 0u;
+                                    a.in[ c ][ i ] = 0u;
                         if ( a.clearAlso )
-                            a.clearAlso[ base + q ] = 0u;
+                            a.clearAlso[ i ] = 0u;
                     }
+                }
             __syncthreads();
         }
+    }
+
+    template<int NC, int ITEMS, bool TR>
+    static void launch_scan_tiles( const Buffers & b, const ScanArgs & a, hipStream_t s )
+    {
+        const uint32_t tile  = uint32_t( ScanThreads * ITEMS );
+        const uint32_t tiles = ( a.n + tile - 1 ) / tile;
+        const uint32_t cap   = uint32_t( device_cu_count() ) * 4u;
+        const uint32_t grid  = tiles < 1 ? 1 : ( tiles < cap ? tiles : cap );
+        scan_kernel<NC, ITEMS, TR><<<grid, ScanThreads, 0, s>>>( a, b.ctl, b.flags, b.flagStride );
     }
 
     template<int NC>
     void launch_scan( const Buffers & b, const ScanArgs & a, hipStream_t s )
     {
-        const uint32_t tiles = ( a.n + ScanTile - 1 ) / ScanTile;
-        const uint32_t cap   = uint32_t( device_cu_count() ) * 4u;
-        const uint32_t grid  = tiles < 1 ? 1 : ( tiles < cap ? tiles : cap );
-        scan_kernel<NC><<<grid, ScanThreads, 0, s>>>( a, b.ctl, b.flags, b.flagStride );
+        // a.n bounds the item count (a device-side count may lower it): tiles of 1024 up to 192k
+        // items, 2048 up to 768k, 4096 (coalesced through LDS) beyond
+        static_assert( ScanThreads * 4 == ScanMinTile, "flag arrays are sized for 1024-item tiles" );
+        if ( a.n <= 192u * 1024u )
+            launch_scan_tiles<NC, 4, false>( b, a, s );
+        else if ( a.n <= 768u * 1024u )
+            launch_scan_tiles<NC, 8, false>( b, a, s );
+        else
+            launch_scan_tiles<NC, 16, true>( b, a, s );
     }
     template void launch_scan<1>( const Buffers &, const ScanArgs &, hipStream_t );
     template void launch_scan<2>( const Buffers &, const ScanArgs &, hipStream_t );

@@ -82,8 +82,9 @@ namespace sesdf
         uint32_t capSectors, capFull, capTrimmed, capLists, capSegments, capRecords, capProbeNb, capTorus;
     };
 
-    // Exclusive scan of up to 4 per-item count arrays (tiles of 2048 items, decoupled look-back on
-    // one ticket per tile). out[c] receives n + 1 values (total last); totals[c], when set, too.
+    // Exclusive scan of up to 4 per-item count arrays (tiles of ScanMinTile items or more, decoupled
+    // look-back on one ticket per tile). out[c] receives n + 1 values (total last); totals[c], when set, too.
+    constexpr uint32_t ScanMinTile = 1024; // smallest scan tile: look-back flag arrays hold n / ScanMinTile + 2 words
     struct ScanArgs
     {
         uint32_t *       in[ 4 ];
