@@ -8,7 +8,8 @@ replayed as one hipGraph. Further legs (same JSON line):
   sdf        SDF of the complete surface on a 256^3 grid over the bounding box + 3 A;
   large      exterior path on 2.5M atoms (500 globules of 5,000 on a 360 A Fibonacci sphere) + memory;
   batch      4,096 molecules (N ~ logU[20, 5000]) in one launch sequence + 32^3 SDF grid each, vs
-             one launch sequence per molecule (timed on the first 512 molecules);
+             one launch sequence per molecule (timed on the first 512 molecules); also the builds
+             alone both ways;
   grad       20,000-atom globule, 1M queries within |SDF| <= 2 A, loss = sum w_i SDF(q_i), w ~ N(0, 1):
              forward (build + sampling) vs backward (stored / recomputed winners);
   cpu        host C++ build of the same stage sequence (the FP32 oracle restatement), box's cores.
@@ -275,6 +276,12 @@ def leg_batch(args, out):
         s.sample_grid_ptr(descs[m][0], descs[m][1], (dim,) * 3, outs.data_ptr() + 4 * m * dim ** 3, band=band)
     L.device_synchronize()
     ms_single = (time.perf_counter() - t0) * 1e3
+    # the builds alone, both ways (no SDF grids)
+    t0 = time.perf_counter()
+    for s in singles:
+        L.check(L.lib().sesdf_enqueue(s._h), "enqueue")
+    L.device_synchronize()
+    ms_single_build = (time.perf_counter() - t0) * 1e3
     # same accuracy: the batch's grids equal the standalone grids
     same = bool(torch.allclose(grids[:P * dim ** 3], outs, atol=1e-5))
     batched_rate = M / (ms_batched * 1e-3)
@@ -284,9 +291,12 @@ def leg_batch(args, out):
                     "molecules_per_s_batched": batched_rate,
                     "per_molecule_launch_molecules": P, "ms_per_molecule_launches": ms_single,
                     "molecules_per_s_per_launch": single_rate, "speedup": batched_rate / single_rate,
+                    "ms_per_molecule_launches_build_only": ms_single_build,
+                    "speedup_build_only": (M / ms_build) / (P / ms_single_build),
                     "grids_equal_standalone": same,
                     "tuning": {k: tune[k] for k in ("cell_size", "circle_waves", "intersection_waves")}}
-    log(f"batch: {batched_rate:.0f} mol/s batched vs {single_rate:.0f} per launch")
+    log(f"batch: {batched_rate:.0f} mol/s batched vs {single_rate:.0f} per launch; "
+        f"builds alone {M / ms_build * 1e3:.0f} vs {P / ms_single_build * 1e3:.0f}")
     for s in singles:
         s.close()
     batch.close()

@@ -14,7 +14,7 @@ import os
 from . import _lib as L
 from .batch import set_tuning
 
-CELLS = (2.0, 2.5, 3.2, 3.4, 4.0, 5.0, 6.5)
+CELLS = (1.5, 2.0, 2.5, 3.2, 3.4, 4.0, 5.0, 6.5)
 WAVES = (2, 4, 8)
 _CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
