@@ -46,7 +46,7 @@ def lib():
         L.oracle_components.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_components.restype = ctypes.c_uint32
         L.oracle_sdf_grad.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
-                                      ctypes.c_double, ctypes.c_void_p]
+                                      ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
         L.oracle_sdf_grad.restype = ctypes.c_int
         L.oracle_sdf_fd.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_double, ctypes.c_void_p,
@@ -89,13 +89,13 @@ class OracleResult:
             raise ValueError("oracle SDF needs a complete-surface build")
         return (out, cd) if codes else out
 
-    def sdf_grad(self, points, weights, band=2.0, exterior=False):
+    def sdf_grad(self, points, weights, band=2.0, exterior=False, threads=8):
         """FP64 analytic gradient of sum_q w_q SDF(q) w.r.t. (x, y, z, r) per atom, input order [n, 4]."""
         p = np.ascontiguousarray(np.asarray(points, np.float64).reshape(-1, 3))
         w = np.ascontiguousarray(np.asarray(weights, np.float64).reshape(-1))
         g = np.zeros((self.n, 4), np.float64)
         rc = lib().oracle_sdf_grad(self._h, ctypes.c_void_p(p.ctypes.data), ctypes.c_void_p(w.ctypes.data), p.shape[0],
-                                   1 if exterior else 0, band, ctypes.c_void_p(g.ctypes.data))
+                                   1 if exterior else 0, band, ctypes.c_void_p(g.ctypes.data), threads)
         if rc:
             raise ValueError("oracle SDF needs a complete-surface build")
         return g

@@ -1407,24 +1407,40 @@ extern "C"
 
     // Surface component label per sorted atom and the exterior label (last entry).
     // FP64 analytic gradient of sum_q w_q SDF(q) w.r.t. (x, y, z, r) of every atom, INPUT order (grad[4 n]).
-    int oracle_sdf_grad( void * h, const double * pts, const double * w, uint64_t n, int flags, double band, double * grad )
+    // Points are split into `threads` contiguous ranges, each summed into its own array; the arrays
+    // are then added in range order, so the result depends on the thread count only by rounding.
+    int oracle_sdf_grad( void * h, const double * pts, const double * w, uint64_t n, int flags, double band, double * grad, int threads )
     {
         Result * r = static_cast<Result *>( h );
         if ( r->mode != 0 )
             return 1;
         SdfOracle o( *r );
         o.buildGrid( band );
-        std::vector<double> g( size_t( r->n ) * 4, 0.0 );
-        for ( uint64_t q = 0; q < n; q++ )
-        {
-            int64_t          code;
-            const V3<double> p { pts[ 3 * q ], pts[ 3 * q + 1 ], pts[ 3 * q + 2 ] };
-            o.eval( p, band, flags & 1, code );
-            o.grad( p, code, w[ q ], g.data() );
-        }
+        const int                        T = std::max( 1, std::min( threads, int( ( n + 4095 ) / 4096 ) ) );
+        std::vector<std::vector<double>> g( T, std::vector<double>( size_t( r->n ) * 4, 0.0 ) );
+        auto                             range = [ & ]( int t ) {
+            for ( uint64_t q = n * t / T; q < n * ( t + 1 ) / T; q++ )
+            {
+                int64_t          code;
+                const V3<double> p { pts[ 3 * q ], pts[ 3 * q + 1 ], pts[ 3 * q + 2 ] };
+                o.eval( p, band, flags & 1, code );
+                o.grad( p, code, w[ q ], g[ t ].data() );
+            }
+        };
+        std::vector<std::thread> pool;
+        for ( int t = 1; t < T; t++ )
+            pool.emplace_back( range, t );
+        range( 0 );
+        for ( auto & th : pool )
+            th.join();
         for ( uint32_t a = 0; a < r->n; a++ )
             for ( int d = 0; d < 4; d++ )
-                grad[ 4 * size_t( r->sortedToInitial[ a ] ) + d ] = g[ 4 * size_t( a ) + d ];
+            {
+                double v = 0.0;
+                for ( int t = 0; t < T; t++ )
+                    v += g[ t ][ 4 * size_t( a ) + d ];
+                grad[ 4 * size_t( r->sortedToInitial[ a ] ) + d ] = v;
+            }
         return 0;
     }
 

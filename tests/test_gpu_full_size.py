@@ -249,3 +249,75 @@ def test_batch_4096_molecules_match_fp64(gpu):
          f"{len(atoms[int(offs[chosen[-1]]):int(offs[chosen[-1] + 1])])} atoms: grid max |GPU - FP64| {worst:.2e}, "
          f"{excused_total} of {40 * dim ** 3} vertices near band differences")
     batch.close()
+
+
+def test_differentiation_workload(gpu):
+    """20,000-atom globule, 1M queries drawn uniformly within 2 A of the surface, w ~ N(0, 1): the
+    SDF values, the backward with stored and with recomputed winners against the FP64 analytic
+    gradient, and against FP64 central differences (h = 1e-4 A, topology held fixed) on 256 random
+    surface atoms away from winner changes."""
+    import torch
+
+    atoms = synth.globule(20000, synth.SEED_BASE + 4)
+    _say("20k globule, 1M queries: values and gradients")
+    surf = Sesdf(atoms)
+    orc = O.build(atoms, mode=0, precision=64, track=True, threads=16)
+    rng = np.random.default_rng(29)
+    band = 2.0
+    lo, hi = atoms[:, :3].min(0) - 4, atoms[:, :3].max(0) + 4
+    cand = (lo + (hi - lo) * rng.random((4_600_000, 3))).astype(np.float32)  # about 25% fall in the band
+    s = orc.sdf(cand.astype(np.float64), band=band, threads=16)
+    pts = cand[np.abs(s) < band][:1_000_000]
+    assert len(pts) == 1_000_000
+    p64 = pts.astype(np.float64)
+    w = rng.standard_normal(len(pts)).astype(np.float32)
+    P = torch.from_numpy(np.c_[pts, np.zeros(len(pts), np.float32)]).cuda()
+    W = torch.from_numpy(w).cuda()
+    val = torch.empty(len(pts), device="cuda")
+    codes = torch.empty(len(pts), device="cuda", dtype=torch.int64)
+    surf.sample_points_ptr(P.data_ptr(), len(pts), val.data_ptr(), band=band, d_codes=codes.data_ptr())
+    grads = {}
+    for store in (True, False):
+        grad = torch.zeros(len(atoms), 4, device="cuda")
+        scratch = torch.empty(len(atoms), 4, device="cuda")
+        surf.sdf_backward_ptr(P.data_ptr(), W.data_ptr(), len(pts), grad.data_ptr(), scratch.data_ptr(), band=band,
+                              d_codes=codes.data_ptr() if store else None)
+        torch.cuda.synchronize()
+        grads[store] = grad.cpu().numpy().astype(np.float64)
+    v, c = val.cpu().numpy(), codes.cpu().numpy()
+
+    # values, outside the points decided by elements of band differences
+    o, oc = orc.sdf(p64, band=band, threads=16, codes=True)
+    g = parity.gpu_arrays(surf)
+    rep = PF.compare_fast(g, orc, atoms, 0)
+    PF.assert_parity(rep)
+    wa_g = _winner_atoms(c, g["s2i"].astype(np.int64), g["full_circles"], g["segments"], g["concave_ids"])
+    wa_o = _winner_atoms(oc, orc["sorted_to_initial"].astype(np.int64), orc["full_circles"], orc["segments"], orc["concave_ids"])
+    excused = (np.isin(wa_g, rep["band_atoms"]) | np.isin(wa_o, rep["band_atoms"])).any(axis=1)
+    verr = np.abs(v - o)[~excused]
+    _say(f"values: max |GPU - FP64| {verr.max():.2e}, {int(excused.sum())} decided by band elements")
+    assert verr.max() <= SDF_TOL
+
+    # gradients against the FP64 analytic gradient: per-atom error over the gradient scale
+    ref = orc.sdf_grad(p64, w.astype(np.float64), band=band, threads=16)
+    scale = np.abs(ref).max()
+    for store, gg in grads.items():
+        err = np.abs(gg - ref).max(1) / scale
+        _say(f"backward, stored winners {store}: max {err.max():.2e}, p99.9 {np.quantile(err, 0.999):.2e}")
+        assert np.quantile(err, 0.999) <= 1e-3
+        assert np.mean(err > 1e-3) < 1e-3
+
+    # and against central differences on 256 random surface atoms
+    surface = np.nonzero(np.abs(ref).max(1) > 1e-3 * scale)[0]
+    ids = np.sort(rng.choice(surface, 256, replace=False)).astype(np.uint32)
+    fd, stable = [], []
+    for half in (ids[:128], ids[128:]):  # two calls of about a minute each, with a line between
+        f_, s_ = orc.sdf_fd(p64, w.astype(np.float64), half, h=1e-4, band=band, threads=16)
+        fd.append(f_), stable.append(s_)
+        _say(f"finite differences on {len(half)} atoms done")
+    fd, stable = np.concatenate(fd), np.concatenate(stable)
+    rel = np.abs(fd - grads[True][ids]).max(1) / scale
+    _say(f"finite differences: {int(stable.sum())} of 256 atoms stable, max rel err {rel[stable].max():.2e}")
+    assert stable.mean() > 0.5
+    assert np.all(rel[stable] <= 1e-3)
+    surf.close()
