@@ -161,10 +161,56 @@ namespace sesdf
 
     // Called by a whole wave with identical t / local[] in every lane. flags holds NC arrays of
     // `stride` words each. Returns the exclusive prefixes in excl[] (identical in every lane).
-    // The columns walk back together: each round reads one window of 64 predecessors of every
-    // column still open, so the columns share the latency of each round.
     template<int NC>
     __device__ void wave_lookback( uint64_t * flags, uint32_t stride, uint32_t t, uint32_t tag, const uint32_t ( &local )[ NC ], uint32_t ( &excl )[ NC ] )
+    {
+        const int lane = lane_id();
+        if ( lane == 0 )
+            for ( int c = 0; c < NC; c++ )
+                flag_store( flags + size_t( c ) * stride + t, pack_flag( t == 0 ? FlagInclusive : FlagAggregate, tag, local[ c ] ) );
+        for ( int c = 0; c < NC; c++ )
+            excl[ c ] = 0;
+        if ( t == 0 )
+            return;
+        const uint32_t tag30 = tag & 0x3fffffffu;
+        for ( int c = 0; c < NC; c++ )
+        {
+            const uint64_t * f   = flags + size_t( c ) * stride;
+            int64_t          pos = int64_t( t ) - 1;
+            uint32_t         acc = 0;
+            for ( ;; )
+            {
+                const int64_t q   = pos - lane;
+                uint64_t      w   = q >= 0 ? flag_load( f + q ) : pack_flag( FlagInclusive, tag30, 0 );
+                uint32_t      st  = uint32_t( w >> 62 );
+                if ( uint32_t( ( w >> 32 ) & 0x3fffffffu ) != tag30 )
+                    st = 0;
+                const uint64_t incMask = ballot( st == 2 );
+                const uint64_t badMask = ballot( st == 0 );
+                const int      upto    = incMask ? __ffsll( (unsigned long long)incMask ) - 1 : WaveSize - 1;
+                const uint64_t window  = upto == 63 ? ~0ull : ( ( 1ull << ( upto + 1 ) ) - 1 );
+                if ( badMask & window )
+                {
+                    __builtin_amdgcn_s_sleep( 1 );
+                    continue; // a predecessor has not published yet
+                }
+                const uint32_t v = lane <= upto ? uint32_t( w ) : 0u;
+                acc += wave_sum( v );
+                if ( incMask )
+                    break;
+                pos -= WaveSize;
+            }
+            excl[ c ] = acc;
+        }
+        if ( lane == 0 )
+            for ( int c = 0; c < NC; c++ )
+                flag_store( flags + size_t( c ) * stride + t, pack_flag( FlagInclusive, tag, excl[ c ] + local[ c ] ) );
+    }
+
+    // As wave_lookback, with the columns walking back together: each round reads one window of 64
+    // predecessors of every column still open, so the columns share the latency of each round.
+    template<int NC>
+    __device__ void wave_lookback_joint( uint64_t * flags, uint32_t stride, uint32_t t, uint32_t tag, const uint32_t ( &local )[ NC ], uint32_t ( &excl )[ NC ] )
     {
         const int lane = lane_id();
         auto      mine = [ & ]( const uint32_t( &v )[ NC ] ) { // column `lane` of v

@@ -59,8 +59,7 @@ namespace sesdf
     // a tile only waits for tiles already being worked on). Each thread owns ITEMS consecutive
     // items. TR: the tile's columns are read and written coalesced (item q * 256 + thread) and
     // transposed through LDS (padded: one word per 32); without it each thread reads its own run.
-    // Small arrays take small tiles (more workgroups at once; a scan of 1e5 items is latency-bound),
-    // large ones large tiles (fewer look-back steps and tickets).
+    // Large arrays take large tiles (fewer look-back steps and tickets).
     template<int NC, int ITEMS, bool TR>
     __global__ void __launch_bounds__( ScanThreads ) scan_kernel( ScanArgs a, Control * ctl, uint64_t * flags, uint32_t flagStride )
     {
@@ -100,50 +99,66 @@ namespace sesdf
             uint32_t       v[ NC ][ ITEMS ];
             uint32_t       s[ NC ], incl[ NC ];
             uint32_t       mx = 0, sm = 0;
-#pragma unroll
-            for ( int c = 0; c < NC; c++ )
+            if constexpr ( TR )
             {
-                const bool       half = a.halfSecond && c == 1;
-                const uint32_t * src  = half ? a.in[ 0 ] : a.in[ c ];
-                if constexpr ( TR )
+#pragma unroll
+                for ( int c = 0; c < NC; c++ )
                 {
+                    const bool       half = a.halfSecond && c == 1;
+                    const uint32_t * src  = half ? a.in[ 0 ] : a.in[ c ];
 #pragma unroll
                     for ( int q = 0; q < ITEMS; q++ )
                     {
-                        const uint32_t i      = tile0 + q * T + tid;
+                        const uint32_t i          = tile0 + q * T + tid;
                         lds[ pad( q * T + tid ) ] = i < n ? src[ i ] : 0u;
                     }
                     __syncthreads();
+                    s[ c ] = 0;
 #pragma unroll
                     for ( int q = 0; q < ITEMS; q++ )
-                        v[ c ][ q ] = lds[ pad( tid * ITEMS + q ) ];
+                    {
+                        const uint32_t x = lds[ pad( tid * ITEMS + q ) ];
+                        v[ c ][ q ]      = half ? x / 2 : x;
+                        s[ c ] += v[ c ][ q ];
+                    }
                     __syncthreads();
                 }
-                else
-                {
+                if ( a.maxOut )
 #pragma unroll
                     for ( int q = 0; q < ITEMS; q++ )
-                        v[ c ][ q ] = base + q < n ? src[ base + q ] : 0u;
-                }
-                s[ c ] = 0;
+                        mx = max( mx, v[ 0 ][ q ] + ( NC > 1 && !a.halfSecond ? v[ 1 ][ q ] : 0u ) );
+                if ( a.sumOut ) // a plain sum: read in coalesced order
+#pragma unroll
+                    for ( int q = 0; q < ITEMS; q++ )
+                    {
+                        const uint32_t i = tile0 + q * T + tid;
+                        sm += i < n ? a.in[ NC ][ i ] : 0u;
+                    }
+            }
+            else
+            {
+#pragma unroll
+                for ( int c = 0; c < NC; c++ )
+                    s[ c ] = 0;
 #pragma unroll
                 for ( int q = 0; q < ITEMS; q++ )
                 {
-                    v[ c ][ q ] = half ? v[ c ][ q ] / 2 : v[ c ][ q ];
-                    s[ c ] += v[ c ][ q ];
+                    const bool in = base + q < n;
+#pragma unroll
+                    for ( int c = 0; c < NC; c++ )
+                    {
+                        uint32_t x = 0;
+                        if ( in )
+                            x = ( a.halfSecond && c == 1 ) ? a.in[ 0 ][ base + q ] / 2 : a.in[ c ][ base + q ];
+                        v[ c ][ q ] = x;
+                        s[ c ] += x;
+                    }
+                    if ( in && a.maxOut )
+                        mx = max( mx, v[ 0 ][ q ] + ( NC > 1 && !a.halfSecond ? v[ 1 ][ q ] : 0u ) );
+                    if ( in && a.sumOut )
+                        sm += a.in[ NC ][ base + q ];
                 }
             }
-            if ( a.maxOut )
-#pragma unroll
-                for ( int q = 0; q < ITEMS; q++ )
-                    mx = max( mx, v[ 0 ][ q ] + ( NC > 1 && !a.halfSecond ? v[ 1 ][ q ] : 0u ) );
-            if ( a.sumOut ) // a plain sum: read in coalesced order
-#pragma unroll
-                for ( int q = 0; q < ITEMS; q++ )
-                {
-                    const uint32_t i = tile0 + q * T + tid;
-                    sm += i < n ? a.in[ NC ][ i ] : 0u;
-                }
 #pragma unroll
             for ( int c = 0; c < NC; c++ )
             {
@@ -172,7 +187,10 @@ namespace sesdf
                     for ( int q = 0; q < W; q++ )
                         local[ c ] += waveSums[ c ][ q ];
                 }
-                wave_lookback<NC>( flags, flagStride, t, tag, local, excl );
+                if constexpr ( TR )
+                    wave_lookback_joint<NC>( flags, flagStride, t, tag, local, excl );
+                else
+                    wave_lookback<NC>( flags, flagStride, t, tag, local, excl );
                 if ( lane_id() == 0 )
                 {
 #pragma unroll
@@ -263,12 +281,10 @@ namespace sesdf
     template<int NC>
     void launch_scan( const Buffers & b, const ScanArgs & a, hipStream_t s )
     {
-        // a.n bounds the item count (a device-side count may lower it): tiles of 1024 up to 192k
-        // items, 2048 up to 768k, 4096 (coalesced through LDS) beyond
-        static_assert( ScanThreads * 4 == ScanMinTile, "flag arrays are sized for 1024-item tiles" );
-        if ( a.n <= 192u * 1024u )
-            launch_scan_tiles<NC, 4, false>( b, a, s );
-        else if ( a.n <= 768u * 1024u )
+        // a.n bounds the item count (a device-side count may lower it): tiles of 2048 up to 768k
+        // items, 4096 (coalesced through LDS) beyond
+        static_assert( ScanThreads * 8 == ScanMinTile, "flag arrays are sized for the smallest tile" );
+        if ( a.n <= 768u * 1024u )
             launch_scan_tiles<NC, 8, false>( b, a, s );
         else
             launch_scan_tiles<NC, 16, true>( b, a, s );

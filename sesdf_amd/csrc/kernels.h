@@ -84,7 +84,7 @@ namespace sesdf
 
     // Exclusive scan of up to 4 per-item count arrays (tiles of ScanMinTile items or more, decoupled
     // look-back on one ticket per tile). out[c] receives n + 1 values (total last); totals[c], when set, too.
-    constexpr uint32_t ScanMinTile = 1024; // smallest scan tile: look-back flag arrays hold n / ScanMinTile + 2 words
+    constexpr uint32_t ScanMinTile = 2048; // smallest scan tile: look-back flag arrays hold n / ScanMinTile + 2 words
     struct ScanArgs
     {
         uint32_t *       in[ 4 ];
