@@ -562,13 +562,20 @@ namespace sesdf
     }
 
     constexpr int SegThreads = 128;
-    constexpr int SegMax     = 16; // the reference's MaxIntersectionsPerCircles (sesdf.cu:17)
+    constexpr int SegMax     = 16;   // the reference's MaxIntersectionsPerCircles (sesdf.cu:17)
+    constexpr int SegRange   = 1024; // intersected circles scanned per block and round
 
+    // Intersected circles whose lists hold two or more patches (about one in fourteen on a globule)
+    // are first gathered into an LDS queue per range of SegRange circles, then taken one per thread,
+    // so that the lanes that sort lists and measure angles are dense instead of one in fourteen.
+    // The output position of each circle's segments comes from segStart, not from the order.
     __global__ void __launch_bounds__( SegThreads ) segments_kernel( Buffers b )
     {
         __shared__ uint32_t sList[ SegMax ][ SegThreads ];
         __shared__ float    sAng[ SegMax ][ SegThreads ];
         __shared__ uint8_t  sOrd[ SegMax ][ SegThreads ];
+        __shared__ uint32_t sQueue[ SegRange ];
+        __shared__ uint32_t sQueued;
         const int      tx = threadIdx.x;
         const uint32_t Ci = min( b.ctl->intersectedCircleNb, b.capTrimmed );
         const float    rp = b.probeRadius;
@@ -576,81 +583,103 @@ namespace sesdf
         if ( b.ctl->overflow || b.ctl->intersectionWithNbNb + b.ctl->intersectionNoNbNb > b.capRecords ||
              b.ctl->circleListNb > b.capLists )
             return;
-        for ( uint32_t c = blockIdx.x * blockDim.x + tx; c < Ci; c += gridDim.x * blockDim.x )
+        for ( uint32_t r0 = blockIdx.x * uint32_t( SegRange ); r0 < Ci; r0 += gridDim.x * uint32_t( SegRange ) )
         {
-            uint32_t cnt = b.circleIntNb[ c ];
-            if ( cnt / 2 == 0 )
-                continue;
-            if ( cnt > SegMax - 1 )
+            if ( tx == 0 )
+                sQueued = 0;
+            __syncthreads();
+            const uint32_t r1 = min( r0 + uint32_t( SegRange ), Ci );
+            for ( uint32_t k = r0 + uint32_t( tx ); k - uint32_t( tx ) < r1; k += SegThreads )
             {
-                atomicAdd( &b.ctl->circleIntOverflowNb, 1u );
-                cnt = SegMax - 1;
+                const bool     take = k < r1 && b.circleIntNb[ k ] / 2 != 0;
+                const uint64_t m    = ballot( take );
+                uint32_t       base = 0;
+                if ( lane_id() == 0 && m )
+                    base = atomicAdd( &sQueued, uint32_t( __popcll( m ) ) );
+                base = uint32_t( __shfl( int( base ), 0, WaveSize ) );
+                if ( take )
+                    sQueue[ base + rank_below( m ) ] = k;
             }
-            const uint32_t g  = b.trimmedToGlobal[ c ];
-            const uint32_t i  = g / b.K;
-            const uint32_t j  = b.nbIds[ g ];
-            const uint32_t st = b.circleStart[ c ];
-            // deterministic start: the lowest record index
-            for ( uint32_t q = 0; q < cnt; q++ )
+            __syncthreads();
+            const uint32_t queued = sQueued;
+            for ( uint32_t q0 = 0; q0 < queued; q0 += SegThreads )
             {
-                const uint32_t v = b.circleLists[ st + q ];
-                int            p = int( q ) - 1;
-                while ( p >= 0 && sList[ p ][ tx ] > v )
+                if ( q0 + uint32_t( tx ) >= queued )
+                    continue;
+                const uint32_t c   = sQueue[ q0 + tx ];
+                uint32_t       cnt = b.circleIntNb[ c ];
+                if ( cnt > SegMax - 1 )
                 {
-                    sList[ p + 1 ][ tx ] = sList[ p ][ tx ];
-                    p--;
+                    atomicAdd( &b.ctl->circleIntOverflowNb, 1u );
+                    cnt = SegMax - 1;
                 }
-                sList[ p + 1 ][ tx ] = v;
+                const uint32_t g  = b.trimmedToGlobal[ c ];
+                const uint32_t i  = g / b.K;
+                const uint32_t j  = b.nbIds[ g ];
+                const uint32_t st = b.circleStart[ c ];
+                // deterministic start: the lowest record index
+                for ( uint32_t q = 0; q < cnt; q++ )
+                {
+                    const uint32_t v = b.circleLists[ st + q ];
+                    int            p = int( q ) - 1;
+                    while ( p >= 0 && sList[ p ][ tx ] > v )
+                    {
+                        sList[ p + 1 ][ tx ] = sList[ p ][ tx ];
+                        p--;
+                    }
+                    sList[ p + 1 ][ tx ] = v;
+                }
+                const float4    ai   = b.atoms[ i ], aj = b.atoms[ j ];
+                const f3        ci   = xyz( ai );
+                const float     Ri   = ai.w + rp, Rj = aj.w + rp;
+                const CircleGeo geo  = circle_geo( xyz( aj ) - ci, Ri * Ri, Rj * Rj );
+                const uint32_t  s0   = sList[ 0 ][ tx ];
+                const int4      sIds = b.concaveIds[ s0 ];
+                const float4    sPos = b.concavePos[ s0 ];
+                const bool      secondary = i == uint32_t( sIds.x ) && j == uint32_t( sIds.z );
+                const float     dir       = copysignf( 1.f, sPos.w ) * ( secondary ? -1.f : 1.f );
+                const f3        ref       = ( xyz( sPos ) - ci ) - geo.centre;
+                sAng[ 0 ][ tx ]           = 0.f;
+                sOrd[ 0 ][ tx ]           = 0;
+                for ( uint32_t q = 1; q < cnt; q++ )
+                {
+                    const float4 p  = b.concavePos[ sList[ q ][ tx ] ];
+                    f3           v1 = ref, v2 = unit( ( xyz( p ) - ci ) - geo.centre );
+                    if ( dir < 0.f )
+                    {
+                        const f3 tmp = v1;
+                        v1           = v2;
+                        v2           = tmp;
+                    }
+                    const float a = angle_between_edges( v1, v2, geo.normal );
+                    // stable insertion by angle
+                    int pidx = int( q ) - 1;
+                    while ( pidx >= 0 && sAng[ pidx ][ tx ] > a )
+                    {
+                        sAng[ pidx + 1 ][ tx ] = sAng[ pidx ][ tx ];
+                        sOrd[ pidx + 1 ][ tx ] = sOrd[ pidx ][ tx ];
+                        pidx--;
+                    }
+                    sAng[ pidx + 1 ][ tx ] = a;
+                    sOrd[ pidx + 1 ][ tx ] = uint8_t( q );
+                }
+                const uint32_t gs = b.segStart[ c ];
+                for ( uint32_t q = 0; q < cnt / 2; q++ )
+                {
+                    uint32_t A = sList[ sOrd[ 2 * q ][ tx ] ][ tx ], B = sList[ sOrd[ 2 * q + 1 ][ tx ] ][ tx ];
+                    if ( dir < 0.f )
+                    {
+                        const uint32_t tmp = A;
+                        A                  = B;
+                        B                  = tmp;
+                    }
+                    if ( gs + q < b.capSegments )
+                        b.segments[ gs + q ] = make_uint4( i, j, A, B );
+                    else
+                        set_overflow( b.ctl, OverflowSegments );
+                }
             }
-            const float4    ai   = b.atoms[ i ], aj = b.atoms[ j ];
-            const f3        ci   = xyz( ai );
-            const float     Ri   = ai.w + rp, Rj = aj.w + rp;
-            const CircleGeo geo  = circle_geo( xyz( aj ) - ci, Ri * Ri, Rj * Rj );
-            const uint32_t  s0   = sList[ 0 ][ tx ];
-            const int4      sIds = b.concaveIds[ s0 ];
-            const float4    sPos = b.concavePos[ s0 ];
-            const bool      secondary = i == uint32_t( sIds.x ) && j == uint32_t( sIds.z );
-            const float     dir       = copysignf( 1.f, sPos.w ) * ( secondary ? -1.f : 1.f );
-            const f3        ref       = ( xyz( sPos ) - ci ) - geo.centre;
-            sAng[ 0 ][ tx ]           = 0.f;
-            sOrd[ 0 ][ tx ]           = 0;
-            for ( uint32_t q = 1; q < cnt; q++ )
-            {
-                const float4 p  = b.concavePos[ sList[ q ][ tx ] ];
-                f3           v1 = ref, v2 = unit( ( xyz( p ) - ci ) - geo.centre );
-                if ( dir < 0.f )
-                {
-                    const f3 tmp = v1;
-                    v1           = v2;
-                    v2           = tmp;
-                }
-                const float a = angle_between_edges( v1, v2, geo.normal );
-                // stable insertion by angle
-                int pidx = int( q ) - 1;
-                while ( pidx >= 0 && sAng[ pidx ][ tx ] > a )
-                {
-                    sAng[ pidx + 1 ][ tx ] = sAng[ pidx ][ tx ];
-                    sOrd[ pidx + 1 ][ tx ] = sOrd[ pidx ][ tx ];
-                    pidx--;
-                }
-                sAng[ pidx + 1 ][ tx ] = a;
-                sOrd[ pidx + 1 ][ tx ] = uint8_t( q );
-            }
-            const uint32_t gs = b.segStart[ c ];
-            for ( uint32_t q = 0; q < cnt / 2; q++ )
-            {
-                uint32_t A = sList[ sOrd[ 2 * q ][ tx ] ][ tx ], B = sList[ sOrd[ 2 * q + 1 ][ tx ] ][ tx ];
-                if ( dir < 0.f )
-                {
-                    const uint32_t tmp = A;
-                    A                  = B;
-                    B                  = tmp;
-                }
-                if ( gs + q < b.capSegments )
-                    b.segments[ gs + q ] = make_uint4( i, j, A, B );
-                else
-                    set_overflow( b.ctl, OverflowSegments );
-            }
+            __syncthreads();
         }
     }
 
@@ -926,7 +955,7 @@ namespace sesdf
     {
         if ( b.n == 0 || b.mode != 0 )
             return;
-        const uint32_t blocks = persistent_blocks( b.capTrimmed, SegThreads, 8 );
+        const uint32_t blocks = persistent_blocks( b.capTrimmed, SegRange, 8 );
         segments_kernel<<<blocks, SegThreads, 0, s>>>( b );
     }
 
