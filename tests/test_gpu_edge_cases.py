@@ -231,3 +231,20 @@ def test_random_small_clusters(gpu, seed):
     rp = float(np.random.default_rng(seed).uniform(1.0, 2.0))
     for mode in (0, 1):
         _check(atoms, mode, probe_radius=rp)
+
+
+@pytest.mark.parametrize("rp", [0.0, 0.3])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_small_probe_radius(gpu, mode, rp):
+    """Probe radius 0 (the SES is the van der Waals surface: probes sit on triple points of the
+    spheres, none has neighbours) and 0.3 A; the complete-surface SDF too."""
+    atoms = synth.globule(1500, synth.SEED_BASE + 77)
+    surf, orc, rep = _check(atoms, mode, probe_radius=rp)
+    assert rep["concave_nb"] > 1000
+    if mode == 0:
+        from test_gpu_sdf import gpu_sdf, near_surface_points
+
+        pts = near_surface_points(atoms, orc, 20000, 2.0, 11)
+        g = gpu_sdf(surf, pts, 2.0)
+        o = orc.sdf(pts.astype(np.float64), band=2.0, threads=8)
+        assert np.abs(g - o).max() <= 1e-4
