@@ -248,3 +248,20 @@ def test_small_probe_radius(gpu, mode, rp):
         g = gpu_sdf(surf, pts, 2.0)
         o = orc.sdf(pts.astype(np.float64), band=2.0, threads=8)
         assert np.abs(g - o).max() <= 1e-4
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_clusters_far_apart(gpu, mode):
+    """Two globules 20,000 A apart: the dense cell brick would exceed its cap, so its cells are
+    enlarged (to about 60 A); the topology must not change. Probe positions are FP32 outputs: at
+    |x| = 20,000 A one ulp is 2e-3 A, so they are held to the FP32 rounding of the FP64 position."""
+    a = synth.globule(800, synth.SEED_BASE + 81)
+    b = synth.globule(800, synth.SEED_BASE + 82) + np.float32([20000.0, 0.0, 0.0, 0.0])
+    atoms = np.concatenate([a, b])
+    surf = _surf(atoms, mode, cell_size=2.5)
+    orc = O.build(atoms, mode=mode, precision=64, track=True, threads=8)
+    rep = parity.compare(parity.gpu_arrays(surf), orc, atoms, mode)
+    print({k: rep[k] for k in rep if k != "mismatch"}, rep["mismatch"], flush=True)
+    assert not any(rep["mismatch"].values()), rep
+    assert rep["concave_nb"] > 1000
+    assert rep["probe_pos_max_err"] <= 1e-4 + float(np.spacing(np.float32(20000.0)))
