@@ -764,7 +764,7 @@ namespace sesdf
 
     // Scattered points: a wave whose points fall in a compact block of cells is evaluated
     // cooperatively, any other wave lane by lane.
-    __global__ void __launch_bounds__( 256 ) sdf_points_kernel( Buffers b, SdfArgs s )
+    __global__ void __launch_bounds__( 256, 5 ) sdf_points_kernel( Buffers b, SdfArgs s ) // five waves per SIMD, as the grids
     {
         __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
         __shared__ uint64_t masks[ 256 / WaveSize ][ 2 * MaskRows ];
