@@ -185,7 +185,9 @@ typedef struct
 
 #define SESDF_SDF_EXTERIOR 1 /* flags: exterior variant (cavity surfaces excluded, cavities inside) */
 
-/* SDF at npoints device points (float4 xyz_), written to d_out (float). Complete-surface handles. */
+/* SDF at npoints device points (float4 xyz_), written to d_out (float). Complete-surface handles.
+ * From 8,192 points on, the points are evaluated in cell order (a radix sort of their cell keys in
+ * a workspace the handle allocates on first use and keeps); results come back in the caller's order. */
 sesdf_status sesdf_sample_points( sesdf_handle * h, const float * d_points, uint64_t npoints, float band, float * d_out );
 /* Same with flags and, when d_codes is not null, the winning element of every query (uint64:
  * type << 40 | index; types 0 outside, 1 face, 2 full circle, 3 segment, 4 vertex, 5 clamped). */
@@ -202,8 +204,9 @@ sesdf_status sesdf_sdf_backward( sesdf_handle * h, const float * d_points, const
 /* Surface component label of every grid-sorted atom (union of atoms joined by exposed arcs) and
  * the exterior label (component of the atom with the largest x + r), copied to host memory. */
 sesdf_status sesdf_surface_components( sesdf_handle * h, uint32_t * host_labels, uint32_t * exterior_label );
-/* d_codes: winners stored by sesdf_sample_points_ex (store) or null (recompute); d_grad_sorted:
- * device scratch of float4 * atom_nb (gradient in grid-sorted order). */
+/* d_codes: winners stored by sesdf_sample_points_ex (store) or null (recompute: from 8,192 points
+ * on, a forward pass in cell order writes them to scratch codes kept with the handle, 12 bytes per
+ * point); d_grad_sorted: device scratch of float4 * atom_nb (gradient in grid-sorted order). */
 sesdf_status sesdf_sdf_backward_ex( sesdf_handle * h, const float * d_points, const float * d_weights, uint64_t npoints, float band,
                                     int flags, const uint64_t * d_codes, float * d_grad_sorted, float * d_grad );
 

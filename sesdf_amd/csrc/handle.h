@@ -33,6 +33,14 @@ namespace sesdf
         uint32_t   geoSectors    = 0;
         uint32_t   geoSegments   = 0;
         uint64_t   packSerial    = ~0ull; // build the packed / surface arrays belong to
+        float4 *   ptSorted    = nullptr;  // scattered points in cell order (ptCap)
+        uint32_t * ptWords     = nullptr;  // 4 x ptCap: cell keys and point indices, double-buffered
+        uint32_t   ptCap       = 0;
+        void *     ptTemp      = nullptr;  // radix sort workspace
+        size_t     ptTempBytes = 0;
+        uint64_t * ptCodes     = nullptr;  // backward without stored winners: codes of a forward pass
+        float *    ptVals      = nullptr;  //   and its values (ptEvalCap each)
+        uint64_t   ptEvalCap   = 0;
         float4 *   molGrids    = nullptr;  // batches: per-molecule grid origin / spacing
         uint32_t   molGridDim  = 0;
         float      molGridMargin = 0.f;

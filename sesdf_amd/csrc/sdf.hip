@@ -15,6 +15,8 @@
 // element (1 for a face, 2 for an arc, 3 for a vertex):
 //   dSDF/dc_m = -nu_m (p - c_m),  dSDF/dr_m = -nu_m R_m   (envelope theorem);
 // outside A: dSDF/dc_a = -(q - c_a) / |q - c_a|, dSDF/dr_a = -1.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -51,6 +53,7 @@ namespace sesdf
         int              recompute;  // backward: re-derive the winners instead of reading codes
         float4 *         gradSorted; // backward: per sorted atom (dx, dy, dz, dr)
         const float4 *   molGrids;   // batches: per molecule (origin, 0), (spacing, 0); grid dx^3 each
+        const uint32_t * perm;       // points given in cell order: result q goes to perm[q] (null: q)
         const float4 *   sAtoms;        // per sorted atom: centre, R = r + rp
         const float4 *   surfAtoms;     // surface atoms (compacted, cell order): centre, R
         const int4 *     surfRanges;    // per surface atom: (full circles, segments), (W vertices, N vertices)
@@ -793,11 +796,33 @@ namespace sesdf
                 v = sdf_eval_lanes<false>( b, s, 0, L, code, nullptr, nullptr );
             if ( L.active )
             {
-                s.out[ q ] = v;
+                const uint64_t dst = s.perm ? uint64_t( s.perm[ q ] ) : q;
+                s.out[ dst ]       = v;
                 if ( s.codes )
-                    s.codes[ q ] = code;
+                    s.codes[ dst ] = code;
             }
         }
+    }
+
+    // Cell key (molecule 0's brick) and index of every point, for the sort into cell order.
+    __global__ void __launch_bounds__( 256 ) sdf_point_keys_kernel( Buffers b, const float4 * pts, uint32_t n, uint32_t * keys, uint32_t * idx )
+    {
+        const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+        if ( q >= n )
+            return;
+        const Molecule M = b.mols[ 0 ];
+        const float4   p = pts[ q ];
+        int            gx, gy, gz;
+        cell_of( M, p.x, p.y, p.z, gx, gy, gz );
+        keys[ q ] = cell_index( M, gx, gy, gz );
+        idx[ q ]  = q;
+    }
+
+    __global__ void __launch_bounds__( 256 ) sdf_point_gather_kernel( const float4 * pts, const uint32_t * idx, uint32_t n, float4 * sorted )
+    {
+        const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+        if ( k < n )
+            sorted[ k ] = pts[ idx[ k ] ];
     }
 
     // Regular grids: one wave per 4 x 4 x 4 brick of grid vertices (batches: per molecule grid).
@@ -1228,6 +1253,62 @@ namespace
     // One thread per item (costs vary per query): dynamic dispatch balances; beyond 2^31 blocks the
     // kernels stride.
     uint32_t grid_blocks( uint64_t n ) { return uint32_t( std::max<uint64_t>( 1, std::min<uint64_t>( ( n + 255 ) / 256, 0x7fffffffull ) ) ); }
+
+    // Scattered points are evaluated in cell order: a wave of 64 consecutive points then falls in a
+    // compact block of cells and runs cooperatively instead of lane by lane (a stable radix sort of
+    // the cell keys; the results go back to the callers' order through the permutation).
+    constexpr uint64_t PointSortMin = 8192;
+    constexpr uint64_t PointSortMax = 0x7fffffffull; // the sort counts items in int
+
+    sesdf_status sort_points( Handle & h, SdfArgs & s )
+    {
+        SdfState &     d = h.sdf;
+        const uint32_t n = uint32_t( s.n );
+        hipStream_t    st = h.stream;
+        int            bits = 1;
+        while ( bits < 32 && ( uint64_t( 1 ) << bits ) < uint64_t( h.cellNb ) )
+            bits++;
+        if ( n > d.ptCap )
+        {
+            // grown on demand, kept with the other sampling buffers
+            const uint64_t cap   = std::min<uint64_t>( std::max<uint64_t>( n, 2 * uint64_t( d.ptCap ) ), PointSortMax );
+            const size_t   words = size_t( 4 ) * cap; // two key and two index buffers
+            void *         p     = nullptr;
+            if ( hipMalloc( &p, words * sizeof( uint32_t ) + cap * sizeof( float4 ) ) != hipSuccess )
+                return SESDF_ERR_CAPACITY;
+            h.sdfAllocs.push_back( p );
+            h.sdfBytes += words * sizeof( uint32_t ) + cap * sizeof( float4 );
+            float4 *   sorted = static_cast<float4 *>( p );
+            uint32_t * w      = reinterpret_cast<uint32_t *>( sorted + cap );
+            size_t     need   = 0;
+            hipcub::DoubleBuffer<uint32_t> k0( w, w + cap ), v0( w + 2 * cap, w + 3 * cap );
+            if ( hipcub::DeviceRadixSort::SortPairs( nullptr, need, k0, v0, int( cap ), 0, 32, st ) != hipSuccess )
+                return SESDF_ERR_HIP;
+            if ( need > d.ptTempBytes )
+            {
+                void * t = nullptr;
+                if ( hipMalloc( &t, need ) != hipSuccess )
+                    return SESDF_ERR_CAPACITY;
+                h.sdfAllocs.push_back( t );
+                h.sdfBytes += need;
+                d.ptTemp      = t;
+                d.ptTempBytes = need;
+            }
+            d.ptSorted = sorted;
+            d.ptWords  = w;
+            d.ptCap    = uint32_t( cap );
+        }
+        const uint64_t cap = d.ptCap;
+        hipcub::DoubleBuffer<uint32_t> keys( d.ptWords, d.ptWords + cap ), idx( d.ptWords + 2 * cap, d.ptWords + 3 * cap );
+        sdf_point_keys_kernel<<<( n + 255 ) / 256, 256, 0, st>>>( h.b, s.points, n, keys.Current(), idx.Current() );
+        size_t tempBytes = d.ptTempBytes;
+        if ( hipcub::DeviceRadixSort::SortPairs( d.ptTemp, tempBytes, keys, idx, int( n ), 0, bits, st ) != hipSuccess )
+            return SESDF_ERR_HIP;
+        sdf_point_gather_kernel<<<( n + 255 ) / 256, 256, 0, st>>>( s.points, idx.Current(), n, d.ptSorted );
+        s.points = d.ptSorted;
+        s.perm   = idx.Current();
+        return SESDF_OK;
+    }
 } // namespace
 
 extern "C"
@@ -1250,7 +1331,15 @@ extern "C"
         if ( s.empty )
             sdf_fill_empty_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( d_out, d_codes, npoints, band );
         else
+        {
+            if ( npoints >= PointSortMin && npoints <= PointSortMax )
+            {
+                sesdf_status ps = sort_points( h, s );
+                if ( ps != SESDF_OK )
+                    return ps;
+            }
             sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+        }
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -1376,7 +1465,35 @@ extern "C"
         sesdf_status st = sdf_common( h, s, flags );
         if ( st != SESDF_OK || s.empty )
             return st; // no atoms: no gradient
-        // cleared by a kernel, not a memset node, so that callers may capture this call in a graph
+        if ( s.recompute && npoints >= PointSortMin && npoints <= PointSortMax )
+        {
+            // winners re-derived by a forward pass in cell order (cooperative waves) into scratch
+            // codes, instead of one thread per query: the backward then reads them as stored ones
+            SdfState & d = h.sdf;
+            if ( npoints > d.ptEvalCap )
+            {
+                void * p = nullptr;
+                if ( hipMalloc( &p, npoints * ( sizeof( uint64_t ) + sizeof( float ) ) ) != hipSuccess )
+                    return SESDF_ERR_CAPACITY;
+                h.sdfAllocs.push_back( p );
+                h.sdfBytes += npoints * ( sizeof( uint64_t ) + sizeof( float ) );
+                d.ptCodes   = static_cast<uint64_t *>( p );
+                d.ptVals    = reinterpret_cast<float *>( d.ptCodes + npoints );
+                d.ptEvalCap = npoints;
+            }
+            SdfArgs f   = s;
+            f.out       = d.ptVals;
+            f.codes     = d.ptCodes;
+            f.weights   = nullptr;
+            f.recompute = 0;
+            sesdf_status ps = sort_points( h, f );
+            if ( ps != SESDF_OK )
+                return ps;
+            sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, f );
+            s.codes     = d.ptCodes;
+            s.recompute = 0;
+        }
+        // cleared by a kernel, not a memset node (memset nodes of captured graphs are avoided)
         zero_grad_kernel<<<( h.n + 255 ) / 256, 256, 0, h.stream>>>( s.gradSorted, h.n );
         if ( npoints )
             sdf_backward_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
