@@ -650,27 +650,33 @@ namespace sesdf
     // One wave per atom, lanes over its neighbours.
     __global__ void __launch_bounds__( 256 ) sdf_pack_kernel( Buffers b, float4 * pa, uint32_t * surfFlag, uint32_t * info )
     {
-        const uint32_t a    = xcd_block() * 4 + threadIdx.x / WaveSize;
-        const uint32_t lane = uint32_t( lane_id() );
-        if ( a >= b.n )
-            return;
-        const float4 at = b.atoms[ a ];
-        const float  Ra = at.w + b.probeRadius;
+        // four atoms per wave, 16 lanes each (an atom has about 45 neighbours: three rounds)
+        const uint32_t lane = uint32_t( lane_id() ), sub = lane & 15u, grp = lane >> 4;
+        const uint32_t a    = ( xcd_block() * 4 + threadIdx.x / WaveSize ) * 4 + grp;
+        const bool     live = a < b.n;
         // inside: the sphere lies in a neighbour's ball; cap: some neighbour robustly cuts a
         // non-empty cap (the caps of visible circles then bound any exposed part)
         bool           inside = false, cap = false;
-        const uint32_t na     = b.nbCount[ a ];
-        for ( uint32_t l = lane; l < na; l += WaveSize )
+        float4         at     = make_float4( 0.f, 0.f, 0.f, 0.f );
+        float          Ra     = 0.f;
+        if ( live )
         {
-            const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
-            const float  Rl = al.w + b.probeRadius;
-            const float  d  = sqrtf( len2( xyz( al ) - xyz( at ) ) );
-            inside          = inside || d + Ra <= Rl;
-            cap             = cap || ( d < Ra + Rl - 1e-3f && d + Rl > Ra + 1e-3f );
+            at                = b.atoms[ a ];
+            Ra                = at.w + b.probeRadius;
+            const uint32_t na = b.nbCount[ a ];
+            for ( uint32_t l = sub; l < na; l += 16 )
+            {
+                const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
+                const float  Rl = al.w + b.probeRadius;
+                const float  d  = sqrtf( len2( xyz( al ) - xyz( at ) ) );
+                inside          = inside || d + Ra <= Rl;
+                cap             = cap || ( d < Ra + Rl - 1e-3f && d + Rl > Ra + 1e-3f );
+            }
         }
-        inside = ballot( inside ) != 0;
-        cap    = ballot( cap ) != 0;
-        if ( lane != 0 )
+        const uint64_t mine = uint64_t( 0xffff ) << ( 16 * grp );
+        inside              = ( ballot( inside ) & mine ) != 0;
+        cap                 = ( ballot( cap ) & mine ) != 0;
+        if ( !live || sub != 0 )
             return;
         pa[ a ]          = make_float4( at.x, at.y, at.z, Ra );
         const bool elems = b.fullBase[ a + 1 ] > b.fullBase[ a ] || b.segStart[ b.trimBase[ a + 1 ] ] > b.segStart[ b.trimBase[ a ] ] ||
@@ -728,40 +734,48 @@ namespace sesdf
     // One wave per atom: up to 64 caps are ranked in registers (key, then index: a total order, so
     // ranks are distinct) and written to their places; longer lists are insertion-sorted by lane 0.
     // Caps of non-existent circles (contained spheres) carry a NaN cosine and are ranked last.
-    __global__ void __launch_bounds__( 256 ) sdf_sort_caps_kernel( Buffers b, float4 * caps, const uint32_t * open )
+    __global__ void __launch_bounds__( 256 ) sdf_sort_caps_kernel( Buffers b, float4 * caps, const uint32_t * surfIds, const uint32_t * surfTotal )
     {
-        const uint32_t a    = xcd_block() * 4 + threadIdx.x / WaveSize;
-        const int      lane = lane_id();
-        if ( a >= b.n || !open[ a ] )
-            return; // only faces that may be exposed are ever tested against caps
-        const uint32_t c0 = b.vStart[ a ], c1 = min( b.vStart[ a + 1 ], b.capSectors );
-        const int      n  = int( c1 > c0 ? c1 - c0 : 0u );
-        auto           key = []( float w ) { return w == w ? w : 3.0f; };
-        if ( n <= WaveSize )
+        // only faces that may be exposed are ever tested against caps: the surface atoms flagged
+        // so in the compacted list, a wave per atom over a grid that strides the list
+        const int      lane   = lane_id();
+        const uint32_t total  = *surfTotal;
+        const uint32_t stride = gridDim.x * ( blockDim.x / WaveSize );
+        for ( uint32_t k = xcd_block() * ( blockDim.x / WaveSize ) + uint32_t( wave_in_block() ); k < total; k += stride )
         {
-            const bool   live = lane < n;
-            const float4 mine = live ? caps[ c0 + lane ] : make_float4( 0.f, 0.f, 0.f, 0.f );
-            const float  k    = key( mine.w );
-            int          rank = 0;
-            for ( int m = 0; m < n; m++ )
+            const uint32_t id = surfIds[ k ];
+            if ( !( id >> 31 ) )
+                continue;
+            const uint32_t a  = id & 0x7fffffffu;
+            const uint32_t c0 = b.vStart[ a ], c1 = min( b.vStart[ a + 1 ], b.capSectors );
+            const int      n  = int( c1 > c0 ? c1 - c0 : 0u );
+            auto           key = []( float w ) { return w == w ? w : 3.0f; };
+            if ( n <= WaveSize )
             {
-                const float km = __shfl( k, m, WaveSize );
-                rank += ( km < k || ( km == k && m < lane ) ) ? 1 : 0;
+                const bool   live = lane < n;
+                const float4 mine = live ? caps[ c0 + lane ] : make_float4( 0.f, 0.f, 0.f, 0.f );
+                const float  kk   = key( mine.w );
+                int          rank = 0;
+                for ( int m = 0; m < n; m++ )
+                {
+                    const float km = __shfl( kk, m, WaveSize );
+                    rank += ( km < kk || ( km == kk && m < lane ) ) ? 1 : 0;
+                }
+                wave_sync(); // every lane has read its cap before any is overwritten
+                if ( live )
+                    caps[ c0 + rank ] = mine;
+                wave_sync();
+                continue;
             }
-            wave_sync(); // every lane has read its cap before any is overwritten
-            if ( live )
-                caps[ c0 + rank ] = mine;
-            return;
-        }
-        if ( lane != 0 )
-            return;
-        for ( uint32_t i = c0 + 1; i < c1; i++ )
-        {
-            const float4 x = caps[ i ];
-            uint32_t     j = i;
-            for ( ; j > c0 && key( caps[ j - 1 ].w ) > key( x.w ); j-- )
-                caps[ j ] = caps[ j - 1 ];
-            caps[ j ] = x;
+            if ( lane == 0 )
+                for ( uint32_t i = c0 + 1; i < c1; i++ )
+                {
+                    const float4 x = caps[ i ];
+                    uint32_t     j = i;
+                    for ( ; j > c0 && key( caps[ j - 1 ].w ) > key( x.w ); j-- )
+                        caps[ j ] = caps[ j - 1 ];
+                    caps[ j ] = x;
+                }
         }
     }
 
@@ -1167,7 +1181,7 @@ namespace
         }
         hipStream_t s = h.stream;
         if ( n )
-            sdf_pack_kernel<<<( n + 3 ) / 4, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo );
+            sdf_pack_kernel<<<( n + 15 ) / 16, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo );
         ScanArgs a {};
         a.in[ 0 ]     = d.surfFlag;
         a.out[ 0 ]    = d.surfPrefix;
@@ -1185,7 +1199,8 @@ namespace
             sdf_geometry_kernel<<<std::min<uint32_t>( ( geoItems + 255 ) / 256, uint32_t( device_cu_count() ) * 8 ), 256, 0, s>>>(
                 h.b, d.geo, d.geo + d.geoSectors );
         if ( n )
-            sdf_sort_caps_kernel<<<( n + 3 ) / 4, 256, 0, s>>>( h.b, d.geo, d.surfInfo );
+            sdf_sort_caps_kernel<<<std::min<uint32_t>( ( n + 3 ) / 4, uint32_t( device_cu_count() ) * 16 ), 256, 0, s>>>( h.b, d.geo, d.surfIds,
+                                                                                                                      d.surfTotal );
         if ( hipGetLastError() != hipSuccess )
             return SESDF_ERR_HIP;
         d.packSerial = h.buildSerial;
