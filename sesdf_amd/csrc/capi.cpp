@@ -143,6 +143,7 @@ namespace sesdf
         h.input = input;
         b.input = input;
         ALLOC_TRY( h, b.atomCell, n );
+        ALLOC_TRY( h, b.atomMol, h.mols.size() > 1 ? n : 1 );
         ALLOC_TRY( h, b.cellCount, h.cellNb, true );
         ALLOC_TRY( h, b.cellStart, uint64_t( h.cellNb ) + 1 );
         ALLOC_TRY( h, b.cellFill, h.cellNb, true );

@@ -121,7 +121,7 @@ namespace sesdf
         const f3       ci  = xyz( ai );
         const float    Ri  = ai.w + rp;
         const float    Ri2 = Ri * Ri;
-        const Molecule M   = b.mols[ find_molecule( b.mols, b.molNb, i ) ];
+        const Molecule M   = b.mols[ b.molNb > 1 ? b.atomMol[ i ] : 0u ];
 
         // ---- neighbour query over the flattened rows of the cell stencil -------------------------
         const float range = ai.w + rmax + 2.f * rp + 1e-3f;

@@ -34,6 +34,8 @@ namespace sesdf
             const float4   p = b.input[ a ];
             const uint32_t m = find_molecule( b.mols, b.molNb, a );
             const Molecule M = b.mols[ m ];
+            if ( b.molNb > 1 )
+                b.atomMol[ a ] = m; // molecule-major order: the same for the sorted index a
             int            gx, gy, gz;
             cell_of( M, p.x, p.y, p.z, gx, gy, gz );
             const uint32_t c = cell_index( M, gx, gy, gz );

@@ -705,7 +705,7 @@ namespace sesdf
         {
         const uint32_t owner = b.records[ q ].x;
         const float4   pq    = b.concavePos[ q ];
-        const Molecule M     = b.mols[ find_molecule( b.mols, b.molNb, owner ) ];
+        const Molecule M     = b.mols[ b.molNb > 1 ? b.atomMol[ owner ] : 0u ];
         // a probe within 2 rp of pq lies on the SAS sphere of its owner atom, so that atom is within
         // 2 rp + (r_max + rp) of pq
         const float search = range + rmax + rp + 1e-3f;

@@ -19,6 +19,7 @@ namespace sesdf
         // input and grid
         const float4 * input;          // N input atoms (input order)
         uint32_t *     atomCell;       // N: global cell index of each input atom
+        uint32_t *     atomMol;        // N: molecule of each atom (batches; input and sorted ranges coincide)
         uint32_t *     cellCount;      // cellNb: atoms per cell (self-clearing)
         uint32_t *     cellStart;      // cellNb + 1: exclusive scan of cellCount
         uint32_t *     cellFill;       // cellNb: scatter cursor (self-clearing)
