@@ -429,35 +429,48 @@ namespace sesdf
         }
     }
 
-    // Sectors, intersected circles and full circles of atom i at the offsets of the scan.
+    // Sectors, intersected circles and full circles of atom i at the offsets of the scan. Four
+    // atoms per wave, 16 lanes each (about 45 slots: three rounds); ranks come from the wave's
+    // ballots restricted to the atom's 16 lanes.
+    constexpr int EmitAtomsPerWave = 4;
     __global__ void __launch_bounds__( CirclesWaves * WaveSize ) circles_emit_kernel( Buffers b )
     {
-        const int      w    = wave_in_block();
-        const int      lane = lane_id();
-        const uint32_t i    = xcd_block() * CirclesWaves + w;
-        if ( i >= b.n )
-            return;
+        const uint32_t lane = uint32_t( lane_id() ), sub = lane & 15u, grp = lane >> 4;
+        const uint32_t i    = ( xcd_block() * CirclesWaves + uint32_t( wave_in_block() ) ) * EmitAtomsPerWave + grp;
+        const bool     atom = i < b.n;
+        const uint64_t mine = uint64_t( 0xffff ) << ( 16 * grp );
         const uint32_t K    = b.K;
         const float    rp   = b.probeRadius;
-        const uint32_t ni   = b.nbCount[ i ];
-        const uint32_t v0   = b.vStart[ i ];
-        if ( lane == 0 )
-            b.convexPatches[ i ] = make_uint2( v0, b.vStart[ i + 1 ] );
-        if ( ni == 0 )
-            return;
-        const float4 ai  = b.atoms[ i ];
-        const f3     ci  = xyz( ai );
-        const float  Ri  = ai.w + rp;
-        const float  Ri2 = Ri * Ri;
-        uint32_t     runV = v0, runT = b.trimBase[ i ], runF = b.fullBase[ i ];
-        for ( uint32_t lb = 0; lb < ni; lb += WaveSize )
+        const uint32_t ni   = atom ? b.nbCount[ i ] : 0u;
+        uint32_t       runV = 0, runT = 0, runF = 0;
+        float4         ai   = make_float4( 0.f, 0.f, 0.f, 0.f );
+        if ( atom )
         {
-            const uint32_t l    = lb + lane;
+            runV = b.vStart[ i ];
+            if ( sub == 0 )
+                b.convexPatches[ i ] = make_uint2( runV, b.vStart[ i + 1 ] );
+            if ( ni )
+            {
+                runT = b.trimBase[ i ];
+                runF = b.fullBase[ i ];
+                ai   = b.atoms[ i ];
+            }
+        }
+        const f3       ci     = xyz( ai );
+        const float    Ri     = ai.w + rp;
+        const float    Ri2    = Ri * Ri;
+        uint32_t       most   = ( ni + 15u ) / 16u; // rounds of the wave: its fullest atom's
+        for ( int o = 32; o > 0; o >>= 1 )
+            most = max( most, uint32_t( __shfl_xor( int( most ), o, WaveSize ) ) );
+        const uint32_t rounds = uint32_t( __builtin_amdgcn_readfirstlane( int( most ) ) );
+        for ( uint32_t r = 0; r < rounds; r++ )
+        {
+            const uint32_t l    = 16u * r + sub;
             const bool     live = l < ni;
             const uint32_t st   = live ? b.status[ size_t( i ) * K + l ] : 0u;
             const uint32_t j    = live ? b.nbIds[ size_t( i ) * K + l ] : 0u;
             const bool     isV = live && st != 0, isT = live && st == 1 && j > i, isF = live && st == 2 && j > i;
-            const uint64_t mV = ballot( isV ), mT = ballot( isT ), mF = ballot( isF );
+            const uint64_t mV = ballot( isV ) & mine, mT = ballot( isT ) & mine, mF = ballot( isF ) & mine;
             if ( isV )
             {
                 const uint32_t dst = runV + rank_below( mV );
@@ -564,7 +577,8 @@ namespace sesdf
     {
         if ( b.n == 0 )
             return;
-        const uint32_t blocks = ( b.n + CirclesWaves - 1 ) / CirclesWaves;
+        const uint32_t perBlock = CirclesWaves * EmitAtomsPerWave;
+        const uint32_t blocks   = ( b.n + perBlock - 1 ) / perBlock;
         ScanArgs       a {};
         a.n           = b.n;
         a.ticket      = TicketCircles;
