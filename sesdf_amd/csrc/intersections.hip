@@ -812,57 +812,96 @@ namespace sesdf
         b.atomCnt[ i ] = cnt;
     }
 
-    // Torus records (MegaMol layout) of atom i's status-2 circles, in slot order.
-    __global__ void __launch_bounds__( 256 ) torus_emit_kernel( Buffers b )
+    // Torus records (MegaMol layout) of atom i's status-2 circles, in slot order. A wave takes 64
+    // atoms and spreads their records over its lanes (an atom has about one record, some several):
+    // record r of the wave belongs to the atom whose exclusive prefix is the largest one <= r, and
+    // is that atom's k-th status-2 slot, found by a popcount walk over its status row.
+    constexpr int TorusWaves = 4;
+    __global__ void __launch_bounds__( TorusWaves * WaveSize ) torus_emit_kernel( Buffers b )
     {
-        const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-        if ( i >= b.n )
+        __shared__ uint32_t sPre[ TorusWaves ][ WaveSize ];
+        const int      w    = wave_in_block();
+        const uint32_t lane = uint32_t( lane_id() );
+        const uint32_t base = ( xcd_block() * TorusWaves + uint32_t( w ) ) * WaveSize;
+        if ( base >= b.n )
             return;
-        uint32_t run = b.torusBase[ i ];
-        if ( b.torusBase[ i + 1 ] == run )
-            return;
-        const uint32_t K   = b.K;
-        const float    rp  = b.probeRadius;
-        const uint32_t ni  = b.nbCount[ i ];
-        const float4   ai  = b.atoms[ i ];
-        const f3       ci  = xyz( ai );
-        const float    Ri  = ai.w + rp;
-        const uint4 *  row = reinterpret_cast<const uint4 *>( b.status + size_t( i ) * K );
-        for ( uint32_t v = 0; 16 * v < ni; v++ )
+        const uint32_t K     = b.K;
+        const float    rp    = b.probeRadius;
+        const uint32_t last  = min( base + WaveSize, b.n );
+        const uint32_t i0    = base + lane;
+        const uint32_t cnt   = i0 < last ? b.torusBase[ i0 + 1 ] - b.torusBase[ i0 ] : 0u;
+        const uint32_t incl  = wave_incl_scan( cnt );
+        const uint32_t total = wave_bcast( incl, WaveSize - 1 );
+        const uint32_t first = b.torusBase[ base ];
+        sPre[ w ][ lane ]    = incl - cnt;
+        wave_sync();
+        for ( uint32_t r0 = 0; r0 < total; r0 += WaveSize )
         {
-            const uint4 x = row[ v ];
-            for ( uint32_t q = 0; q < 4; q++ )
+            const uint32_t r = r0 + lane;
+            if ( r >= total )
+                break;
+            // atom slot: the last one whose exclusive prefix is <= r (empty atoms share prefixes)
+            int lo = 0, hi = WaveSize;
+            while ( hi - lo > 1 )
             {
-                uint32_t bits = torus_bits( torus_word( x, q ), 4 * v + q, ni );
-                while ( bits )
-                {
-                    const uint32_t l   = 4 * ( 4 * v + q ) + uint32_t( __ffs( int( bits ) ) - 1 ) / 8;
-                    const uint32_t dst = run++;
-                    bits &= bits - 1;
-                    if ( dst >= b.capTorus )
-                    {
-                        set_overflow( b.ctl, OverflowTorus );
-                        continue;
-                    }
-                    const uint32_t  j     = b.nbIds[ size_t( i ) * K + l ];
-                    const float4    aj    = b.atoms[ j ];
-                    const f3        dj    = xyz( aj ) - ci;
-                    const float     Rj    = aj.w + rp;
-                    const CircleGeo geo   = circle_geo( dj, Ri * Ri, Rj * Rj );
-                    const float     rad   = sqrtf( geo.radius2 );
-                    const f3        ta    = unit( geo.centre );
-                    const f3        ortho = unit( cross( ta, mk( 0.f, 0.f, 1.f ) ) );
-                    const f3        P     = geo.centre + ortho * rad;
-                    const f3        X     = unit( P ) * ai.w;
-                    const float     lpi   = sqrtf( len2( P ) );
-                    const f3        C     = dj * ( lpi / ( sqrtf( len2( P - dj ) ) + lpi ) );
-                    const float     dist  = sqrtf( len2( X - C ) );
-                    const f3        Cout  = C - geo.centre;
-                    b.torusPos[ dst ]     = make_float4( ci.x + geo.centre.x, ci.y + geo.centre.y, ci.z + geo.centre.z, rad );
-                    b.torusAxis[ dst ]    = make_float4( ta.x, ta.y, ta.z, rp );
-                    b.torusVs[ dst ]      = make_float4( Cout.x, Cout.y, Cout.z, dist );
-                }
+                const int mid = ( lo + hi ) >> 1;
+                if ( sPre[ w ][ mid ] <= r )
+                    lo = mid;
+                else
+                    hi = mid;
             }
+            const uint32_t i   = base + uint32_t( lo );
+            uint32_t       k   = r - sPre[ w ][ lo ];
+            const uint32_t dst = first + r;
+            if ( dst >= b.capTorus )
+            {
+                set_overflow( b.ctl, OverflowTorus );
+                continue;
+            }
+            const uint32_t ni  = b.nbCount[ i ];
+            const uint4 *  row = reinterpret_cast<const uint4 *>( b.status + size_t( i ) * K );
+            uint32_t       l   = 0;
+            for ( uint32_t v = 0; 16 * v < ni; v++ )
+            {
+                const uint4 x     = row[ v ];
+                bool        found = false;
+                for ( uint32_t q = 0; q < 4 && !found; q++ )
+                {
+                    uint32_t       bits = torus_bits( torus_word( x, q ), 4 * v + q, ni );
+                    const uint32_t c    = uint32_t( __popc( bits ) );
+                    if ( k < c )
+                    {
+                        for ( uint32_t e = 0; e < k; e++ )
+                            bits &= bits - 1;
+                        l     = 4 * ( 4 * v + q ) + uint32_t( __ffs( int( bits ) ) - 1 ) / 8;
+                        found = true;
+                    }
+                    else
+                        k -= c;
+                }
+                if ( found )
+                    break;
+            }
+            const float4    ai    = b.atoms[ i ];
+            const f3        ci    = xyz( ai );
+            const float     Ri    = ai.w + rp;
+            const uint32_t  j     = b.nbIds[ size_t( i ) * K + l ];
+            const float4    aj    = b.atoms[ j ];
+            const f3        dj    = xyz( aj ) - ci;
+            const float     Rj    = aj.w + rp;
+            const CircleGeo geo   = circle_geo( dj, Ri * Ri, Rj * Rj );
+            const float     rad   = sqrtf( geo.radius2 );
+            const f3        ta    = unit( geo.centre );
+            const f3        ortho = unit( cross( ta, mk( 0.f, 0.f, 1.f ) ) );
+            const f3        P     = geo.centre + ortho * rad;
+            const f3        X     = unit( P ) * ai.w;
+            const float     lpi   = sqrtf( len2( P ) );
+            const f3        C     = dj * ( lpi / ( sqrtf( len2( P - dj ) ) + lpi ) );
+            const float     dist  = sqrtf( len2( X - C ) );
+            const f3        Cout  = C - geo.centre;
+            b.torusPos[ dst ]     = make_float4( ci.x + geo.centre.x, ci.y + geo.centre.y, ci.z + geo.centre.z, rad );
+            b.torusAxis[ dst ]    = make_float4( ta.x, ta.y, ta.z, rp );
+            b.torusVs[ dst ]      = make_float4( Cout.x, Cout.y, Cout.z, dist );
         }
     }
 
@@ -985,6 +1024,6 @@ namespace sesdf
         a.out[ 0 ]    = b.torusBase;
         a.totals[ 0 ] = &b.ctl->torusNb;
         launch_scan<1>( b, a, s );
-        torus_emit_kernel<<<blocks, 256, 0, s>>>( b );
+        torus_emit_kernel<<<( b.n + TorusWaves * WaveSize - 1 ) / ( TorusWaves * WaveSize ), TorusWaves * WaveSize, 0, s>>>( b );
     }
 } // namespace sesdf

@@ -2,8 +2,9 @@
 (test helper; the same contract as ``parity.compare`` for the rows it covers).
 
 Covered: neighbour pairs, circle statuses (unordered in the exterior path), full circles, sector
-values and segments (complete surface), concave patches keyed by (sorted input triple, side) with
-their probe positions and hasNeighbors flags. Probe-neighbour sets are left to ``parity.compare``
+values and segments (complete surface), torus records keyed by circle (exterior path; values where
+both sides built a record from the same head atom), concave patches keyed by (sorted input triple,
+side) with their probe positions and hasNeighbors flags. Probe-neighbour sets are left to ``parity.compare``
 on the smaller globules. Every difference must fall in the oracle's epsilon band; ``band_atoms``
 lists the atoms of the differences that do (input indices).
 
@@ -246,6 +247,35 @@ def compare_fast(gpu, orc, atoms_in, mode, orc_to_input=None, keep=None, keep_pa
         rep["segment_nb"] = int(len(sgo))
         rep["segment_band_circles"] = np.unique(d["u"][b])
         band_atoms += [d["u"][b] // n_in, d["u"][b] % n_in]
+    # -- exterior path: torus records, keyed by circle (head atom, partner) --------------------------------
+    if mode == 1 and "torus_pos" in gpu:
+        def torus_keys(r, j, s2i):
+            return s2i[r] * n_in + s2i[j.astype(np.int64)]
+
+        # the GPU emits an atom's records in slot order, atoms in sorted order
+        rr, ll, jj = _slots(gpu["nb_count"], gpu["nb_ids"], K_g)
+        two = gpu["status"].reshape(-1, K_g)[rr, ll] == 2
+        tg = torus_keys(rr[two], jj[two], s2i_g)
+        vg = np.concatenate([gpu["torus_pos"], gpu["torus_axis"], gpu["torus_vs"]], axis=1).astype(np.float64)
+        assert len(tg) == len(vg), (len(tg), len(vg))
+        slots = orc["torus_slots"].astype(np.int64)
+        to = torus_keys(slots // K_o, orc["nb_ids"][slots], s2i_o)
+        vo = orc["torus"].reshape(3, -1, 4).transpose(1, 0, 2).reshape(-1, 12).astype(np.float64)
+        if keep_mask is not None:
+            fg_, fo_ = keep_mask[tg // n_in] & keep_mask[tg % n_in], keep_mask[to // n_in] & keep_mask[to % n_in]
+            tg, vg, to, vo = tg[fg_], vg[fg_], to[fo_], vo[fo_]
+        ug, uo = _ukey(tg // n_in, tg % n_in, n_in), _ukey(to // n_in, to % n_in, n_in)
+        d = np.setxor1d(ug, uo)
+        b = np.isin(d, amb_circles) | np.isin(d, promo)
+        band_atoms += [d[b] // n_in, d[b] % n_in]
+        rep["mismatch"]["torus"] = int((~b).sum())
+        rep["band"]["torus"] = int(b.sum())
+        rep["torus_nb"] = int(len(uo))
+        # values where both sides built the record from the same head atom (same sorted order);
+        # reported here, bounded by torus_values_within_fp32 (some fields are ill-conditioned)
+        _, a, c = np.intersect1d(tg, to, return_indices=True)
+        rep["torus_same_head"] = int(len(a))
+        rep["torus_max_err"] = float(np.nanmax(np.abs(vg[a] - vo[c]))) if len(a) else 0.0
     rep["band_atoms"] = np.unique(np.concatenate([np.asarray(x, np.int64) for x in band_atoms])) if band_atoms else np.zeros(0, np.int64)
     return rep
 
@@ -256,3 +286,38 @@ def assert_parity(rep):
     assert rep["probe_pos_max_err"] <= POS_TOL, rep
     if "sector_max_err" in rep:
         assert rep["sector_max_err"] <= SECTOR_TOL, rep
+
+
+
+def _torus_table(res_or_gpu, is_gpu, K, s2i, n_in):
+    if is_gpu:
+        rr, ll, jj = _slots(res_or_gpu["nb_count"], res_or_gpu["nb_ids"], K)
+        two = res_or_gpu["status"].reshape(-1, K)[rr, ll] == 2
+        keys = s2i[rr[two]] * n_in + s2i[jj[two].astype(np.int64)]
+        vals = np.concatenate([res_or_gpu["torus_pos"], res_or_gpu["torus_axis"], res_or_gpu["torus_vs"]], axis=1)
+    else:
+        slots = res_or_gpu["torus_slots"].astype(np.int64)
+        keys = s2i[slots // K] * n_in + s2i[res_or_gpu["nb_ids"][slots].astype(np.int64)]
+        vals = res_or_gpu["torus"].reshape(3, -1, 4).transpose(1, 0, 2).reshape(-1, 12)
+    return keys, vals.astype(np.float64)
+
+
+def torus_values_within_fp32(gpu, orc64, orc32, atoms_in, slack=1e-4):
+    """Torus record values (centre, radius, axis, visibility sphere) of the GPU against FP64, where
+    the FP64 and FP32 oracles ran in the GPU's sorted order (the same head atom builds each record).
+    Some fields are ill-conditioned in FP32 (the radius of a near-tangent circle, the axis of a
+    circle whose centre lies near its head atom's centre), so each field of each record may differ
+    from FP64 by twice what the FP32 restatement of the reference's algorithm differs, plus
+    `slack`. A NaN must be a NaN on both sides. Returns (records compared, records outside)."""
+    n_in = len(atoms_in)
+    kg, vg = _torus_table(gpu, True, gpu["K"], gpu["s2i"].astype(np.int64), n_in)
+    k64, v64 = _torus_table(orc64, False, orc64.K, orc64["sorted_to_initial"].astype(np.int64), n_in)
+    k32, v32 = _torus_table(orc32, False, orc32.K, orc32["sorted_to_initial"].astype(np.int64), n_in)
+    both, a, b = np.intersect1d(kg, k64, return_indices=True)
+    _, c, d = np.intersect1d(both, k32, return_indices=True)
+    g, r64, r32 = vg[a[c]], v64[b[c]], v32[d]
+    nan_ok = np.isnan(g) == np.isnan(r64)
+    eg = np.where(np.isnan(g) & np.isnan(r64), 0.0, np.abs(g - r64))
+    e32 = np.where(np.isnan(r32) & np.isnan(r64), 0.0, np.abs(r32 - r64))
+    bad = ~nan_ok | ~( eg <= 2 * np.nan_to_num(e32, nan=0.0) + slack )
+    return int(len(g)), int(bad.any(axis=1).sum())

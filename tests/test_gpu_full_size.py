@@ -42,9 +42,18 @@ def test_headline_build_matches_fp64(gpu, headline, mode):
     g = parity.gpu_arrays(surf)
     orc = O.build(atoms, mode=mode, precision=64, track=True, threads=16)
     rep = PF.compare_fast(g, orc, atoms, mode)
-    _say({k: rep[k] for k in rep if k not in ("mismatch", "band")}, rep["mismatch"], rep["band"])
+    _say({k: rep[k] for k in rep if k not in ("mismatch", "band", "band_atoms", "segment_band_circles")}, rep["mismatch"], rep["band"])
     PF.assert_parity(rep)
     assert rep["neighbor_pairs"] > 4_000_000 and rep["concave_nb"] > 50_000
+    if mode == 1:
+        # torus record values: the oracles take the GPU's sorted order, so that the same head atom
+        # builds each record on both sides
+        order = surf.get_atom_indices()
+        o64 = O.build(atoms, mode=1, precision=64, track=False, threads=16, order=order)
+        o32 = O.build(atoms, mode=1, precision=32, track=False, threads=16, order=order)
+        compared, outside = PF.torus_values_within_fp32(g, o64, o32, atoms)
+        _say(f"torus records: {rep['torus_nb']}, values compared {compared}, outside the FP32 bound {outside}")
+        assert rep["torus_nb"] > 50_000 and compared > 0.99 * rep["torus_nb"] and outside == 0
     surf.close()
 
 

@@ -82,3 +82,20 @@ def test_ball_cut_out_is_exact_inside(mode):
     rim = PF.compare_fast(whole, sub, atoms, mode, orc_to_input=ball, keep=np.nonzero(d < 19.5)[0],
                           keep_patches=np.nonzero(d < 19.5)[0])
     assert rim["mismatch"]["neighbors"] > 0
+
+
+def test_torus_records_compared_by_circle():
+    """Exterior path: torus records keyed by circle, their values held to the FP32 restatement's
+    own error; a record moved by 1e-3 A is reported."""
+    atoms = synth.globule(800, synth.SEED_BASE + 19)
+    f64 = O.build(atoms, mode=1, precision=64, track=True)
+    arr = as_build_arrays(O.build(atoms, mode=1, precision=32, track=False))
+    t = O.build(atoms, mode=1, precision=32, track=False)["torus"].reshape(3, -1, 4)
+    arr.update(torus_pos=t[0].copy(), torus_axis=t[1].copy(), torus_vs=t[2].copy())
+    f32 = O.build(atoms, mode=1, precision=32, track=False)
+    rep = PF.compare_fast(arr, f64, atoms, 1)
+    assert rep["torus_nb"] > 100 and rep["torus_same_head"] == rep["torus_nb"]
+    PF.assert_parity(rep)
+    assert PF.torus_values_within_fp32(arr, f64, f32, atoms) == (rep["torus_nb"], 0)
+    arr["torus_pos"][len(t[0]) // 2, 1] += 1e-3
+    assert PF.torus_values_within_fp32(arr, f64, f32, atoms)[1] == 1
