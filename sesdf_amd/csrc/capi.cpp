@@ -146,7 +146,6 @@ namespace sesdf
         ALLOC_TRY( h, b.atomMol, h.mols.size() > 1 ? n : 1 );
         ALLOC_TRY( h, b.cellCount, h.cellNb, true );
         ALLOC_TRY( h, b.cellStart, uint64_t( h.cellNb ) + 1 );
-        ALLOC_TRY( h, b.cellFill, h.cellNb, true );
         ALLOC_TRY( h, b.slotInput, n );
         b.cellNb = h.cellNb;
         ALLOC_TRY( h, b.atoms, n );

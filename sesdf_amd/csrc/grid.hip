@@ -40,7 +40,9 @@ namespace sesdf
             cell_of( M, p.x, p.y, p.z, gx, gy, gz );
             const uint32_t c = cell_index( M, gx, gy, gz );
             b.atomCell[ a ]  = c;
-            atomicAdd( b.cellCount + c, 1u );
+            // the arrival rank in the cell places the atom in the scatter (no second atomic pass);
+            // atomCnt is free until the circle stage
+            b.atomCnt[ a ] = atomicAdd( b.cellCount + c, 1u );
             r = fmaxf( p.w, 0.f );
         }
         for ( int o = 32; o > 0; o >>= 1 )
@@ -301,7 +303,7 @@ namespace sesdf
         if ( a >= b.n )
             return;
         const uint32_t c    = b.atomCell[ a ];
-        const uint32_t slot = b.cellStart[ c ] + atomicAdd( b.cellFill + c, 1u );
+        const uint32_t slot = b.cellStart[ c ] + b.atomCnt[ a ];
         b.slotInput[ slot ] = a;
     }
 
@@ -321,8 +323,6 @@ namespace sesdf
         const uint32_t dst       = st + rank;
         b.sortedToInitial[ dst ] = id;
         b.atoms[ dst ]           = b.input[ id ];
-        if ( s == st )
-            b.cellFill[ c ] = 0u;
     }
 
     static inline uint32_t blocks_for( uint64_t n, uint32_t threads ) { return uint32_t( ( n + threads - 1 ) / threads ); }

@@ -22,7 +22,6 @@ namespace sesdf
         uint32_t *     atomMol;        // N: molecule of each atom (batches; input and sorted ranges coincide)
         uint32_t *     cellCount;      // cellNb: atoms per cell (self-clearing)
         uint32_t *     cellStart;      // cellNb + 1: exclusive scan of cellCount
-        uint32_t *     cellFill;       // cellNb: scatter cursor (self-clearing)
         uint32_t *     slotInput;      // N: input index placed at each sorted slot (before in-cell sort)
         uint32_t       cellNb;
 
