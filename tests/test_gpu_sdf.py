@@ -70,14 +70,17 @@ def spatial_order(pts, edge=2.0):
     return pts[np.lexsort((key[:, 0], key[:, 1], key[:, 2]))]
 
 
-@pytest.mark.parametrize("exterior,order", [(False, "random"), (True, "random"), (False, "sorted"), (True, "sorted")])
+@pytest.mark.parametrize("exterior,order", [(False, "random"), (True, "random"), (False, "sorted"), (True, "sorted"),
+                                            (False, "few"), (True, "few")])
 def test_sdf_points_match_fp64(gpu, exterior, order):
-    """Random order: lanes of a wave evaluate alone; sorted: waves evaluate cooperatively."""
+    """40,000 points, random or spatial order: the library sorts them into cell order and waves
+    evaluate cooperatively. 6,000 random points (below the sort threshold of 8,192): lanes of a
+    wave evaluate alone."""
     atoms = synth.globule(3000, synth.SEED_BASE + 60)
     surf = Sesdf(atoms)
     orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
     band = 2.0
-    pts = near_surface_points(atoms, orc, 40000, band, 3)
+    pts = near_surface_points(atoms, orc, 6000 if order == "few" else 40000, band, 3)
     if order == "sorted":
         pts = spatial_order(pts)
     g = gpu_sdf(surf, pts, band, exterior)
@@ -116,14 +119,16 @@ def test_sdf_grid_64_config0(gpu, exterior):
     assert (g < 0).any() and (g > 0).any()
 
 
-@pytest.mark.parametrize("store", [True, False])
-def test_backward_matches_fp64_gradient(gpu, store):
+@pytest.mark.parametrize("store,count", [(True, 60000), (False, 60000), (False, 5000)])
+def test_backward_matches_fp64_gradient(gpu, store, count):
+    """Stored winners; recomputed ones (from 8,192 queries on by a forward pass in cell order into
+    scratch codes, below that one thread per query)."""
     torch = _torch()
     atoms = synth.globule(3000, synth.SEED_BASE + 61)
     surf = Sesdf(atoms)
     orc = O.build(atoms, mode=0, precision=64, track=False, threads=8)
     band = 2.0
-    pts = near_surface_points(atoms, orc, 60000, band, 5)
+    pts = near_surface_points(atoms, orc, count, band, 5)
     w = np.random.default_rng(7).standard_normal(len(pts)).astype(np.float32)
     P = torch.from_numpy(np.c_[pts, np.zeros(len(pts), np.float32)]).cuda()
     W = torch.from_numpy(w).cuda()
