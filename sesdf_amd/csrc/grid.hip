@@ -108,21 +108,28 @@ namespace sesdf
 #pragma unroll
                 for ( int c = 0; c < NC; c++ )
                 {
-                    const bool       half = a.halfSecond && c == 1;
-                    const uint32_t * src  = half ? a.in[ 0 ] : a.in[ c ];
+                    s[ c ] = 0;
+                    if ( a.halfSecond && c == 1 ) // halves of column 0, already in registers
+                    {
+#pragma unroll
+                        for ( int q = 0; q < ITEMS; q++ )
+                        {
+                            v[ c ][ q ] = v[ 0 ][ q ] / 2;
+                            s[ c ] += v[ c ][ q ];
+                        }
+                        continue;
+                    }
 #pragma unroll
                     for ( int q = 0; q < ITEMS; q++ )
                     {
                         const uint32_t i          = tile0 + q * T + tid;
-                        lds[ pad( q * T + tid ) ] = i < n ? src[ i ] : 0u;
+                        lds[ pad( q * T + tid ) ] = i < n ? a.in[ c ][ i ] : 0u;
                     }
                     __syncthreads();
-                    s[ c ] = 0;
 #pragma unroll
                     for ( int q = 0; q < ITEMS; q++ )
                     {
-                        const uint32_t x = lds[ pad( tid * ITEMS + q ) ];
-                        v[ c ][ q ]      = half ? x / 2 : x;
+                        v[ c ][ q ] = lds[ pad( tid * ITEMS + q ) ];
                         s[ c ] += v[ c ][ q ];
                     }
                     __syncthreads();
@@ -153,7 +160,7 @@ namespace sesdf
                     {
                         uint32_t x = 0;
                         if ( in )
-                            x = ( a.halfSecond && c == 1 ) ? a.in[ 0 ][ base + q ] / 2 : a.in[ c ][ base + q ];
+                            x = ( a.halfSecond && c == 1 ) ? v[ 0 ][ q ] / 2 : a.in[ c ][ base + q ];
                         v[ c ][ q ] = x;
                         s[ c ] += x;
                     }
