@@ -1,8 +1,8 @@
 """ORACLE front-end (test infrastructure only): ctypes over oracle/liboracle.so.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module.
-It runs the CPU restatement of the reference's stage sequence (oracle/sesdf_oracle.cpp) and the
-FP64 SDF / gradient oracle (oracle/sdf_oracle.cpp) and returns numpy arrays.
+It runs the CPU restatement of the reference's stage sequence and the FP64 SDF / gradient oracle
+(both in oracle/sesdf_oracle.cpp) and returns numpy arrays.
 """
 import ctypes
 import os
