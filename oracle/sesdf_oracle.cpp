@@ -1043,9 +1043,10 @@ namespace orc
     //   arcs     : full circles and segments (A -> B counter-clockwise about n(i -> j));
     //   vertices : concave patch (probe) positions.
     // Outside A, f = min_a |q - c_a| - R_a. The zero level set is the SES; the field is 1-Lipschitz.
-    // Exterior variant: only elements of the surface component that contains the atom with the
-    // largest x + r count, and points outside A whose nearest sphere belongs to another component
-    // (probe-accessible cavities) are inside.
+    // Exterior variant: only elements of the boundary component that holds the point of largest x
+    // count (components are formed over arcs, vertices and faces, see buildComponents), and points
+    // outside A whose region of the complement is bounded by another component (probe-accessible
+    // cavities) are inside.
     // Winner codes: type << 40 | index; types 0 outside(atom), 1 face(atom), 2 full circle,
     // 3 segment, 4 vertex, 5 clamped.
     // ------------------------------------------------------------------------------------------------
@@ -1056,7 +1057,6 @@ namespace orc
         std::vector<V3<double>> c;
         std::vector<double>   R;
         std::vector<uint32_t> fullStart, segStart, vtxStart, vtxList;
-        std::vector<uint32_t> comp;
         uint32_t              extComp = 0;
         // uniform grid over atoms
         double                cell = 1, ox = 0, oy = 0, oz = 0;
@@ -1099,33 +1099,332 @@ namespace orc
             std::vector<uint32_t> fillv( vtxStart.begin(), vtxStart.end() - 1 );
             for ( size_t q = 0; q < I; q++ )
                 vtxList[ fillv[ vo[ q ] ]++ ] = uint32_t( q );
-            // surface components: union-find over arcs
-            comp.resize( n );
-            for ( uint32_t a = 0; a < n; a++ )
-                comp[ a ] = a;
+            buildComponents();
+        }
+
+        // ---- surface components (exterior variant) -------------------------------------------------
+        // The boundary of A is a closed surface made of faces (exposed regions of a sphere), arcs and
+        // vertices; each of its connected pieces bounds one connected region of the complement of A.
+        // Components are formed over arcs (full circles, segments): a segment joins its two end
+        // vertices, and the boundary loops of one face (an atom's exposed region may be split into
+        // several faces, each bounded by one or more loops) are joined. Loops of atom a bound the same
+        // face iff a path on the sphere between points just inside them crosses every loop of the
+        // atom an even number of times. The exterior component holds the point of largest x of the
+        // atom with the largest x + r (ties: lowest sorted index).
+        //   atomLabel[a]: component of atom a's boundary when it is one (an atom without arcs is its
+        //                 own component if exposed), MixedLabel when it spans more, NoneLabel if buried.
+        static constexpr uint32_t MixedLabel = 0xfffffffeu, NoneLabel = 0xffffffffu;
+        uint32_t              Cf = 0, G = 0, I = 0;
+        std::vector<uint32_t> arcLabel, vtxLabel, atomLabel;
+        std::vector<uint32_t> incStart, incArcs; // arcs incident to each atom (arc id: f < Cf, Cf + segment)
+
+        uint32_t arcU( uint32_t e ) const { return e < Cf ? r.fullCircles[ 2 * e ] : r.segments[ 4 * ( e - Cf ) ]; }
+        uint32_t arcV( uint32_t e ) const { return e < Cf ? r.fullCircles[ 2 * e + 1 ] : r.segments[ 4 * ( e - Cf ) + 1 ]; }
+        uint32_t arcOther( uint32_t e, uint32_t a ) const { return arcU( e ) == a ? arcV( e ) : arcU( e ); }
+        // cap of circle (a, o) on sphere a: pole (unit, towards c_o) and cos of its half angle
+        void capOf( uint32_t a, uint32_t o, V3<double> & pole, double & cosT ) const
+        {
+            const V3<double> d  = c[ o ] - c[ a ];
+            const double     dd = len( d );
+            pole                = d * ( 1.0 / dd );
+            cosT                = ( R[ a ] * R[ a ] - R[ o ] * R[ o ] + dd * dd ) / ( 2 * dd * R[ a ] );
+        }
+        // x (world, on circle (i, j) of the arc) within the arc: segments run from A counter-clockwise
+        // about n(i -> j) to B
+        bool onArc( uint32_t e, V3<double> x ) const
+        {
+            if ( e < Cf )
+                return true;
+            const uint32_t * sg = &r.segments[ 4 * size_t( e - Cf ) ];
+            V3<double>       ctr, nrm;
+            double           rad;
+            circle( sg[ 0 ], sg[ 1 ], ctr, nrm, rad );
+            const V3<double> A = pos64( sg[ 2 ] ) - ctr, B = pos64( sg[ 3 ] ) - ctr, X = x - ctr;
+            auto an = [ & ]( V3<double> u ) {
+                double t = std::atan2( dot( cross( A, u ), nrm ), dot( A, u ) );
+                return t < 0 ? t + 2 * M_PI : t;
+            };
+            return an( X ) <= an( B );
+        }
+        // angles phi in [0, Phi] where the geodesic v(phi) = cos phi v1 + sin phi w (v1 -> v2 on
+        // sphere a) meets arc e; returns the count, writes up to 2 angles
+        int crossArc( uint32_t a, V3<double> v1, V3<double> w, double Phi, uint32_t e, double * phis ) const
+        {
+            V3<double> pole;
+            double     cosT;
+            capOf( a, arcOther( e, a ), pole, cosT );
+            const double p = dot( v1, pole ), q = dot( w, pole ), rho = std::hypot( p, q );
+            if ( !( rho > std::fabs( cosT ) ) )
+                return 0;
+            const double ph0 = std::atan2( q, p ), dph = std::acos( std::min( std::max( cosT / rho, -1.0 ), 1.0 ) );
+            int          cnt = 0;
+            for ( double ph : { ph0 - dph, ph0 + dph } )
+            {
+                ph = std::fmod( ph + 4 * M_PI, 2 * M_PI );
+                if ( ph > Phi )
+                    continue;
+                const V3<double> v = v1 * std::cos( ph ) + w * std::sin( ph );
+                if ( onArc( e, c[ a ] + v * R[ a ] ) )
+                    phis[ cnt++ ] = ph;
+            }
+            return cnt;
+        }
+        // geodesic frame from v1 towards v2 (unit vectors); false if they are (nearly) antipodal
+        static bool frame( V3<double> v1, V3<double> v2, V3<double> & w, double & Phi )
+        {
+            const V3<double> m = cross( v1, v2 );
+            const double     s = len( m );
+            if ( s < 1e-9 && dot( v1, v2 ) < 0 )
+                return false;
+            const V3<double> t = v2 - v1 * dot( v1, v2 );
+            const double     tl = len( t );
+            w                   = tl > 0 ? t * ( 1.0 / tl ) : V3<double> { 0, 0, 0 };
+            Phi                 = std::atan2( s, dot( v1, v2 ) );
+            return true;
+        }
+        // arc crossed first by the geodesic from u (an exposed point of sphere a) towards the pole
+        // of one of a's caps (a buried point): it bounds the face holding u. NoneLabel if none.
+        uint32_t firstHit( uint32_t a, V3<double> u ) const
+        {
+            for ( uint32_t t = incStart[ a ]; t < incStart[ a + 1 ]; t++ )
+            {
+                V3<double> pole, w;
+                double     cosT, Phi;
+                capOf( a, arcOther( incArcs[ t ], a ), pole, cosT );
+                if ( !frame( u, pole, w, Phi ) )
+                    continue;
+                double   bestPhi = 1e300;
+                uint32_t bestArc = NoneLabel;
+                for ( uint32_t t2 = incStart[ a ]; t2 < incStart[ a + 1 ]; t2++ )
+                {
+                    double   ph[ 2 ];
+                    const int k = crossArc( a, u, w, Phi, incArcs[ t2 ], ph );
+                    for ( int q = 0; q < k; q++ )
+                        if ( ph[ q ] < bestPhi )
+                            bestPhi = ph[ q ], bestArc = incArcs[ t2 ];
+                }
+                return bestArc;
+            }
+            return NoneLabel;
+        }
+        // component of the exposed point c_a + R_a u of sphere a
+        uint32_t labelAt( uint32_t a, V3<double> u ) const
+        {
+            if ( atomLabel[ a ] != MixedLabel )
+                return atomLabel[ a ];
+            const uint32_t e = firstHit( a, u );
+            return e == NoneLabel ? NoneLabel : arcLabel[ e ];
+        }
+        // a point of arc e just off it on the exposed side of sphere a (unit vector from c_a)
+        V3<double> insidePoint( uint32_t a, uint32_t e, double delta ) const
+        {
+            const uint32_t i = arcU( e ), j = arcV( e );
+            V3<double>     ctr, nrm;
+            double         rad;
+            circle( i, j, ctr, nrm, rad );
+            V3<double> x;
+            if ( e < Cf )
+            {
+                const V3<double> ref = std::fabs( nrm.x ) < 0.9 ? V3<double> { 1, 0, 0 } : V3<double> { 0, 1, 0 };
+                x                    = ctr + normalize( cross( nrm, ref ) ) * rad;
+            }
+            else
+            {
+                const uint32_t * sg = &r.segments[ 4 * size_t( e - Cf ) ];
+                const V3<double> A = pos64( sg[ 2 ] ) - ctr, B = pos64( sg[ 3 ] ) - ctr;
+                double           ang = std::atan2( dot( cross( A, B ), nrm ), dot( A, B ) );
+                if ( ang <= 0 )
+                    ang += 2 * M_PI;
+                const double h = 0.5 * ang;
+                x              = ctr + A * std::cos( h ) + cross( nrm, A ) * std::sin( h );
+            }
+            V3<double> pole;
+            double     cosT;
+            capOf( a, arcOther( e, a ), pole, cosT );
+            const V3<double> v = normalize( x - c[ a ] );
+            const V3<double> t = normalize( v * dot( v, pole ) - pole ); // tangent, away from the cap
+            return normalize( v + t * delta );
+        }
+
+        void buildComponents()
+        {
+            const uint32_t n = r.n;
+            Cf = uint32_t( r.fullCircles.size() / 2 ), G = uint32_t( r.segments.size() / 4 ), I = uint32_t( r.concaveIds.size() / 4 );
+            const uint32_t E = Cf + G;
+            std::vector<uint32_t> par( size_t( E ) + I + n );
+            for ( size_t q = 0; q < par.size(); q++ )
+                par[ q ] = uint32_t( q );
             auto find = [ & ]( uint32_t a ) {
-                while ( comp[ a ] != a )
-                    a = comp[ a ] = comp[ comp[ a ] ];
+                while ( par[ a ] != a )
+                    a = par[ a ] = par[ par[ a ] ];
                 return a;
             };
             auto unite = [ & ]( uint32_t a, uint32_t b ) {
                 a = find( a ), b = find( b );
                 if ( a != b )
-                    comp[ std::max( a, b ) ] = std::min( a, b );
+                    par[ std::max( a, b ) ] = std::min( a, b );
             };
-            for ( size_t q = 0; q + 1 < r.fullCircles.size(); q += 2 )
-                unite( r.fullCircles[ q ], r.fullCircles[ q + 1 ] );
-            for ( size_t q = 0; q + 3 < r.segments.size(); q += 4 )
-                unite( r.segments[ q ], r.segments[ q + 1 ] );
+            // a segment joins its end vertices
+            for ( uint32_t s = 0; s < G; s++ )
+            {
+                unite( Cf + s, E + r.segments[ 4 * s + 2 ] );
+                unite( Cf + s, E + r.segments[ 4 * s + 3 ] );
+            }
+            // arcs incident to each atom
+            incStart.assign( n + 1, 0 );
+            for ( uint32_t e = 0; e < E; e++ )
+            {
+                incStart[ arcU( e ) + 1 ]++; // both atoms of the arc
+                incStart[ arcV( e ) + 1 ]++;
+            }
             for ( uint32_t a = 0; a < n; a++ )
-                comp[ a ] = find( a );
+                incStart[ a + 1 ] += incStart[ a ];
+            incArcs.resize( incStart[ n ] );
+            {
+                std::vector<uint32_t> f( incStart.begin(), incStart.end() - 1 );
+                for ( uint32_t e = 0; e < E; e++ )
+                {
+                    incArcs[ f[ arcU( e ) ]++ ] = e;
+                    incArcs[ f[ arcV( e ) ]++ ] = e;
+                }
+            }
+            // loops of every atom, and the faces they bound
+            for ( uint32_t a = 0; a < n; a++ )
+            {
+                const uint32_t b0 = incStart[ a ], m = incStart[ a + 1 ] - b0;
+                if ( m < 2 )
+                    continue;
+                std::vector<uint32_t> lp( m );
+                for ( uint32_t x = 0; x < m; x++ )
+                    lp[ x ] = x;
+                auto lf = [ & ]( uint32_t x ) {
+                    while ( lp[ x ] != x )
+                        x = lp[ x ] = lp[ lp[ x ] ];
+                    return x;
+                };
+                auto lu = [ & ]( uint32_t x, uint32_t y ) {
+                    x = lf( x ), y = lf( y );
+                    if ( x != y )
+                        lp[ std::max( x, y ) ] = std::min( x, y );
+                };
+                // arcs of atom a chain at shared vertices
+                for ( uint32_t x = 0; x < m; x++ )
+                    for ( uint32_t y = x + 1; y < m; y++ )
+                    {
+                        const uint32_t ex = incArcs[ b0 + x ], ey = incArcs[ b0 + y ];
+                        if ( ex < Cf || ey < Cf )
+                            continue;
+                        const uint32_t * sx = &r.segments[ 4 * size_t( ex - Cf ) ];
+                        const uint32_t * sy = &r.segments[ 4 * size_t( ey - Cf ) ];
+                        if ( sx[ 2 ] == sy[ 2 ] || sx[ 2 ] == sy[ 3 ] || sx[ 3 ] == sy[ 2 ] || sx[ 3 ] == sy[ 3 ] )
+                            lu( x, y );
+                    }
+                std::vector<uint32_t> loopOf( m ), reps;
+                for ( uint32_t x = 0; x < m; x++ )
+                {
+                    loopOf[ x ] = lf( x );
+                    if ( loopOf[ x ] == x )
+                        reps.push_back( x );
+                }
+                if ( reps.size() < 2 )
+                    continue;
+                // points just inside each loop's face; two loops bound the same face iff every loop
+                // of the atom is crossed an even number of times between them
+                std::vector<V3<double>> pts;
+                for ( uint32_t x : reps )
+                    pts.push_back( insidePoint( a, incArcs[ b0 + x ], 1e-6 ) );
+                for ( size_t p = 0; p < reps.size(); p++ )
+                    for ( size_t q = p + 1; q < reps.size(); q++ )
+                    {
+                        if ( lf( reps[ p ] ) == lf( reps[ q ] ) )
+                            continue;
+                        // geodesic p -> q, through an intermediate point when nearly antipodal
+                        V3<double> path[ 3 ] = { pts[ p ], pts[ q ], pts[ q ] };
+                        int        legs      = 1;
+                        V3<double> w { 0, 0, 0 };
+                        double     Phi = 0;
+                        if ( !frame( pts[ p ], pts[ q ], w, Phi ) )
+                        {
+                            const V3<double> ref = std::fabs( pts[ p ].x ) < 0.9 ? V3<double> { 1, 0, 0 } : V3<double> { 0, 1, 0 };
+                            path[ 1 ]            = normalize( cross( pts[ p ], ref ) );
+                            legs                 = 2;
+                        }
+                        std::vector<uint32_t> par2( m, 0 );
+                        for ( int l = 0; l < legs; l++ )
+                        {
+                            frame( path[ l ], path[ l + 1 ], w, Phi );
+                            for ( uint32_t x = 0; x < m; x++ )
+                            {
+                                double ph[ 2 ];
+                                par2[ loopOf[ x ] ] += uint32_t( crossArc( a, path[ l ], w, Phi, incArcs[ b0 + x ], ph ) );
+                            }
+                        }
+                        bool even = true;
+                        for ( uint32_t x : reps )
+                            even = even && par2[ x ] % 2 == 0;
+                        if ( even )
+                        {
+                            lu( reps[ p ], reps[ q ] );
+                            unite( incArcs[ b0 + reps[ p ] ], incArcs[ b0 + reps[ q ] ] );
+                        }
+                    }
+                // loops of one face, and the arcs of one loop (joined by vertices already)
+                for ( uint32_t x = 0; x < m; x++ )
+                    unite( incArcs[ b0 + x ], incArcs[ b0 + loopOf[ x ] ] );
+            }
+            arcLabel.resize( E );
+            for ( uint32_t e = 0; e < E; e++ )
+                arcLabel[ e ] = find( e );
+            vtxLabel.resize( I );
+            for ( uint32_t v = 0; v < I; v++ )
+                vtxLabel[ v ] = find( E + v );
+            atomLabel.assign( n, NoneLabel );
+            for ( uint32_t a = 0; a < n; a++ )
+            {
+                if ( incStart[ a + 1 ] == incStart[ a ] )
+                {
+                    // no arcs: the exposed region has no boundary, so it is all of the sphere (no
+                    // neighbour cuts a cap from it or contains it) or empty (buried)
+                    bool buried = false;
+                    for ( uint32_t l = 0; l < r.nbCount[ a ] && !buried; l++ )
+                    {
+                        const uint32_t j = r.nbIds[ size_t( a ) * r.K + l ];
+                        const double   d = len( c[ j ] - c[ a ] );
+                        buried           = d < R[ a ] + R[ j ] && d + R[ j ] > R[ a ];
+                    }
+                    atomLabel[ a ] = buried ? NoneLabel : find( E + I + a );
+                    continue;
+                }
+                uint32_t lab = arcLabel[ incArcs[ incStart[ a ] ] ];
+                for ( uint32_t t = incStart[ a ] + 1; t < incStart[ a + 1 ]; t++ )
+                    if ( arcLabel[ incArcs[ t ] ] != lab )
+                        lab = MixedLabel;
+                atomLabel[ a ] = lab;
+            }
             uint32_t best = 0;
             for ( uint32_t a = 1; a < n; a++ )
-                if ( double( r.atoms[ 4 * a ] ) + double( r.atoms[ 4 * a + 3 ] ) >
-                     double( r.atoms[ 4 * best ] ) + double( r.atoms[ 4 * best + 3 ] ) )
+                if ( double( r.atoms[ 4 * a ] ) + double( r.atoms[ 4 * a + 3 ] ) > double( r.atoms[ 4 * best ] ) + double( r.atoms[ 4 * best + 3 ] ) )
                     best = a;
-            extComp = n ? comp[ best ] : 0;
+            extComp = n ? labelAt( best, V3<double> { 1, 0, 0 } ) : 0;
+            // exterior status per atom: 0 no exterior boundary, 1 all of it, 2 part of it
+            extStatus.assign( n, 0 );
+            for ( uint32_t a = 0; a < n; a++ )
+            {
+                if ( atomLabel[ a ] == MixedLabel )
+                {
+                    bool any = false, all = true;
+                    for ( uint32_t t = incStart[ a ]; t < incStart[ a + 1 ]; t++ )
+                    {
+                        const bool x = arcLabel[ incArcs[ t ] ] == extComp;
+                        any = any || x, all = all && x;
+                    }
+                    extStatus[ a ] = all ? 1 : any ? 2 : 0;
+                }
+                else
+                    extStatus[ a ] = atomLabel[ a ] == extComp ? 1 : 0;
+            }
         }
+        std::vector<uint8_t> extStatus;
 
         void buildGrid( double band )
         {
@@ -1292,9 +1591,10 @@ namespace orc
                             const double     sd   = dist - R[ a ];
                             if ( sd < g || ( sd == g && int64_t( a ) < gA ) )
                                 g = sd, gA = a;
-                            if ( exterior && comp[ a ] == extComp )
+                            const int st = exterior ? extStatus[ a ] : 1; // 2: check element by element
+                            if ( st == 1 )
                                 gExt = std::min( gExt, sd );
-                            if ( exterior && comp[ a ] != extComp )
+                            if ( st == 0 )
                                 continue;
                             if ( std::fabs( sd ) < best && dist > 0 )
                             {
@@ -1305,11 +1605,15 @@ namespace orc
                                     const uint32_t j = r.nbIds[ size_t( a ) * r.K + l ];
                                     exp              = len2( p - c[ j ] ) - R[ j ] * R[ j ] >= 0;
                                 }
+                                if ( exp && st == 2 )
+                                    exp = labelAt( a, dq * ( 1.0 / dist ) ) == extComp;
                                 if ( exp )
                                     best = std::fabs( sd ), bc = ( int64_t( 1 ) << 40 ) | a;
                             }
                             for ( uint32_t f = fullStart[ a ]; f < fullStart[ a + 1 ]; f++ )
                             {
+                                if ( st == 2 && arcLabel[ f ] != extComp )
+                                    continue;
                                 V3<double> ctr, nrm;
                                 double     rad;
                                 circle( r.fullCircles[ 2 * f ], r.fullCircles[ 2 * f + 1 ], ctr, nrm, rad );
@@ -1323,6 +1627,8 @@ namespace orc
                             }
                             for ( uint32_t s = segStart[ a ]; s < segStart[ a + 1 ]; s++ )
                             {
+                                if ( st == 2 && arcLabel[ Cf + s ] != extComp )
+                                    continue;
                                 const uint32_t * sg = &r.segments[ 4 * size_t( s ) ];
                                 V3<double>       ctr, nrm;
                                 double           rad;
@@ -1347,6 +1653,8 @@ namespace orc
                             for ( uint32_t t = vtxStart[ a ]; t < vtxStart[ a + 1 ]; t++ )
                             {
                                 const uint32_t vq = vtxList[ t ];
+                                if ( st == 2 && vtxLabel[ vq ] != extComp )
+                                    continue;
                                 const double   d  = len( q - pos64( vq ) );
                                 if ( d < best )
                                     best = d, bc = ( int64_t( 4 ) << 40 ) | vq;
@@ -1361,8 +1669,16 @@ namespace orc
                 code = int64_t( 5 ) << 40;
                 return band;
             }
-            // ties between components (within 1e-4 A) count as exterior
-            if ( g >= 0 && ( !exterior || gExt <= g + 1e-4 ) )
+            // outside A the point lies in the complement region bounded by the face that holds its
+            // projection onto the nearest sphere; ties between components (within 1e-4 A) count as
+            // exterior
+            bool outsideExt = g >= 0 && ( !exterior || gExt <= g + 1e-4 );
+            if ( g >= 0 && !outsideExt && extStatus[ gA ] == 2 )
+            {
+                const V3<double> dq = q - c[ gA ];
+                outsideExt          = labelAt( uint32_t( gA ), dq * ( 1.0 / len( dq ) ) ) == extComp;
+            }
+            if ( outsideExt )
             {
                 f    = g;
                 code = gA;
@@ -1515,7 +1831,7 @@ extern "C"
         Result * r = static_cast<Result *>( h );
         SdfOracle o( *r );
         for ( uint32_t a = 0; a < r->n; a++ )
-            out[ a ] = o.comp[ a ];
+            out[ a ] = o.atomLabel[ a ];
         return o.extComp;
     }
 
