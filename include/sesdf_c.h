@@ -201,8 +201,13 @@ sesdf_status sesdf_sample_grid_ex( sesdf_handle * h, const sesdf_grid_desc * gri
  * winning element of every query (recompute); sesdf_sdf_backward_ex can read stored codes. */
 sesdf_status sesdf_sdf_backward( sesdf_handle * h, const float * d_points, const float * d_weights, uint64_t npoints, float band,
                                  int recompute, float * d_grad );
-/* Surface component label of every grid-sorted atom (union of atoms joined by exposed arcs) and
- * the exterior label (component of the atom with the largest x + r), copied to host memory. */
+/* Surface components of the SAS boundary (connected pieces of faces, arcs and vertices; each bounds
+ * one region of the complement of the SAS union), copied to host memory: per grid-sorted atom the
+ * label of its boundary, SESDF_COMPONENT_MIXED when its exposed faces lie in several components,
+ * SESDF_COMPONENT_NONE when it has no exposed boundary; and the exterior label (the component
+ * holding the point of largest x of the atom with the largest x + r). */
+#define SESDF_COMPONENT_MIXED 0xfffffffeu
+#define SESDF_COMPONENT_NONE  0xffffffffu
 sesdf_status sesdf_surface_components( sesdf_handle * h, uint32_t * host_labels, uint32_t * exterior_label );
 /* d_codes: winners stored by sesdf_sample_points_ex (store) or null (recompute: from 8,192 points
  * on, a forward pass in cell order writes them to scratch codes kept with the handle, 12 bytes per

@@ -1246,12 +1246,80 @@ namespace orc
             return normalize( v + t * delta );
         }
 
+        // point x of sphere a outside every neighbour's SAS sphere
+        bool exposedPoint( uint32_t a, V3<double> x ) const
+        {
+            for ( uint32_t l = 0; l < r.nbCount[ a ]; l++ )
+            {
+                const uint32_t j = r.nbIds[ size_t( a ) * r.K + l ];
+                if ( len2( x - c[ j ] ) < R[ j ] * R[ j ] )
+                    return false;
+            }
+            return true;
+        }
+        // atoms bucketed by (y, z) in squares of edge rayCell >= the largest SAS radius: a ray along
+        // +x meets only spheres of the 3 x 3 buckets around it
+        double                               rayCell = 1, rayOy = 0, rayOz = 0;
+        int                                  rayNy = 1, rayNz = 1;
+        std::vector<uint32_t>                rayStart, rayAtoms;
+        void buildRayIndex()
+        {
+            const uint32_t n = r.n;
+            double         Rm = 0, lo[ 2 ] = { 1e300, 1e300 }, hi[ 2 ] = { -1e300, -1e300 };
+            for ( uint32_t a = 0; a < n; a++ )
+            {
+                Rm = std::max( Rm, R[ a ] );
+                lo[ 0 ] = std::min( lo[ 0 ], c[ a ].y ), hi[ 0 ] = std::max( hi[ 0 ], c[ a ].y );
+                lo[ 1 ] = std::min( lo[ 1 ], c[ a ].z ), hi[ 1 ] = std::max( hi[ 1 ], c[ a ].z );
+            }
+            rayCell = std::max( Rm, 0.5 );
+            rayOy = lo[ 0 ], rayOz = lo[ 1 ];
+            rayNy = n ? int( ( hi[ 0 ] - lo[ 0 ] ) / rayCell ) + 1 : 1, rayNz = n ? int( ( hi[ 1 ] - lo[ 1 ] ) / rayCell ) + 1 : 1;
+            rayStart.assign( size_t( rayNy ) * rayNz + 1, 0 );
+            auto bucket = [ & ]( uint32_t a ) {
+                return size_t( std::min( int( ( c[ a ].z - rayOz ) / rayCell ), rayNz - 1 ) ) * rayNy +
+                       size_t( std::min( int( ( c[ a ].y - rayOy ) / rayCell ), rayNy - 1 ) );
+            };
+            for ( uint32_t a = 0; a < n; a++ )
+                rayStart[ bucket( a ) + 1 ]++;
+            for ( size_t q = 0; q + 1 < rayStart.size(); q++ )
+                rayStart[ q + 1 ] += rayStart[ q ];
+            rayAtoms.resize( n );
+            std::vector<uint32_t> f( rayStart.begin(), rayStart.end() - 1 );
+            for ( uint32_t a = 0; a < n; a++ )
+                rayAtoms[ f[ bucket( a ) ]++ ] = a;
+        }
+        // first SAS sphere (other than `skip`) entered by the ray p + t x^, t > 0: atom and entry x
+        uint32_t rayHit( V3<double> p, uint32_t skip, double & hitX ) const
+        {
+            const int by = int( std::floor( ( p.y - rayOy ) / rayCell ) ), bz = int( std::floor( ( p.z - rayOz ) / rayCell ) );
+            uint32_t  best = NoneLabel;
+            hitX           = 1e300;
+            for ( int z = std::max( bz - 1, 0 ); z <= std::min( bz + 1, rayNz - 1 ); z++ )
+                for ( int y = std::max( by - 1, 0 ); y <= std::min( by + 1, rayNy - 1 ); y++ )
+                {
+                    const size_t bk = size_t( z ) * rayNy + y;
+                    for ( uint32_t t = rayStart[ bk ]; t < rayStart[ bk + 1 ]; t++ )
+                    {
+                        const uint32_t a  = rayAtoms[ t ];
+                        const double   dy = c[ a ].y - p.y, dz = c[ a ].z - p.z;
+                        const double   h2 = R[ a ] * R[ a ] - dy * dy - dz * dz;
+                        if ( a == skip || !( h2 > 0 ) )
+                            continue;
+                        const double x = c[ a ].x - std::sqrt( h2 );
+                        if ( x > p.x && x < hitX )
+                            hitX = x, best = a;
+                    }
+                }
+            return best;
+        }
+
         void buildComponents()
         {
             const uint32_t n = r.n;
             Cf = uint32_t( r.fullCircles.size() / 2 ), G = uint32_t( r.segments.size() / 4 ), I = uint32_t( r.concaveIds.size() / 4 );
             const uint32_t E = Cf + G;
-            std::vector<uint32_t> par( size_t( E ) + I + n );
+            std::vector<uint32_t> par( size_t( E ) + I + n + 1 ); // + the node of the unbounded region
             for ( size_t q = 0; q < par.size(); q++ )
                 par[ q ] = uint32_t( q );
             auto find = [ & ]( uint32_t a ) {
@@ -1372,6 +1440,7 @@ namespace orc
                 for ( uint32_t x = 0; x < m; x++ )
                     unite( incArcs[ b0 + x ], incArcs[ b0 + loopOf[ x ] ] );
             }
+            auto labels = [ & ]() {
             arcLabel.resize( E );
             for ( uint32_t e = 0; e < E; e++ )
                 arcLabel[ e ] = find( e );
@@ -1401,11 +1470,43 @@ namespace orc
                         lab = MixedLabel;
                 atomLabel[ a ] = lab;
             }
+            };
+            labels();
             uint32_t best = 0;
             for ( uint32_t a = 1; a < n; a++ )
                 if ( double( r.atoms[ 4 * a ] ) + double( r.atoms[ 4 * a + 3 ] ) > double( r.atoms[ 4 * best ] ) + double( r.atoms[ 4 * best + 3 ] ) )
                     best = a;
-            extComp = n ? labelAt( best, V3<double> { 1, 0, 0 } ) : 0;
+            (void)best;
+            // Regions of the complement: several components may bound one region (the outer
+            // surfaces of separate pieces bound the unbounded one; a piece floating in a cavity
+            // shares the cavity's region). A ray from an exposed +x pole runs in the region of the
+            // pole's face until it enters the boundary of the same region, or escapes (the region is
+            // unbounded): the two components are joined, or the pole's joins the unbounded node. The
+            // component whose +x extreme point is a pole has such a ray, and along these rays x
+            // grows, so every region's components end up joined.
+            buildRayIndex();
+            const uint32_t extNode = E + I + n;
+            for ( uint32_t a = 0; a < n; a++ )
+            {
+                const V3<double> pole = c[ a ] + V3<double> { R[ a ], 0, 0 };
+                if ( !exposedPoint( a, pole ) )
+                    continue;
+                const uint32_t l0 = labelAt( a, V3<double> { 1, 0, 0 } );
+                if ( l0 == NoneLabel )
+                    continue;
+                double         hx;
+                const uint32_t h = rayHit( pole, a, hx );
+                uint32_t       l1 = extNode;
+                if ( h != NoneLabel )
+                {
+                    const V3<double> hp { hx, pole.y, pole.z };
+                    l1 = labelAt( h, normalize( hp - c[ h ] ) );
+                }
+                if ( l1 != NoneLabel )
+                    unite( l0, l1 );
+            }
+            labels();
+            extComp = find( extNode );
             // exterior status per atom: 0 no exterior boundary, 1 all of it, 2 part of it
             extStatus.assign( n, 0 );
             for ( uint32_t a = 0; a < n; a++ )
@@ -1667,6 +1768,16 @@ namespace orc
             if ( gA < 0 || g >= D )
             {
                 code = int64_t( 5 ) << 40;
+                if ( exterior )
+                {
+                    // outside A, farther than rp + band from it: the region holding q is the one
+                    // the ray q + t x^ runs in until it enters A (none: unbounded)
+                    double         hx;
+                    const uint32_t h = rayHit( q, NoneLabel, hx );
+                    const uint32_t lh = h == NoneLabel ? extComp : labelAt( h, normalize( V3<double> { hx, q.y, q.z } - c[ h ] ) );
+                    if ( lh != NoneLabel && lh != extComp )
+                        return -band;
+                }
                 return band;
             }
             // outside A the point lies in the complement region bounded by the face that holds its

@@ -19,9 +19,10 @@ HIP_SOURCES = [
     "circles.hip",
     "intersections.hip",
     "sdf.hip",
+    "components.hip",
     "capi.cpp",
 ]
-HIP_HEADERS = ["common.h", "kernels.h", "handle.h"]
+HIP_HEADERS = ["common.h", "kernels.h", "handle.h", "components.h"]
 
 
 def _run(cmd, cwd=None):

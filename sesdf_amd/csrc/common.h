@@ -97,6 +97,7 @@ namespace sesdf
         TicketCircleScan,
         TicketTorus,
         TicketSdf,
+        TicketComp,
         TicketCount = 8
     };
     enum Overflow : uint32_t

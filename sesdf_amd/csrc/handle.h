@@ -3,6 +3,7 @@
 
 #include <vector>
 
+#include "components.h"
 #include "kernels.h"
 #include "sesdf_c.h"
 
@@ -16,8 +17,9 @@ namespace sesdf
     // Device state of the SDF evaluator (allocated on first use, refreshed once per build).
     struct SdfState
     {
-        uint64_t   builtSerial = ~0ull;   // build the component labels belong to
-        uint32_t * component   = nullptr;  // N: surface component label (exterior classification)
+        Components comp {};                 // surface components (exterior variant), see components.h
+        void *     compAlloc  = nullptr;
+        uint64_t   compSerial = ~0ull;      // build the component labels belong to
         float4 *   gradScratch = nullptr;  // N: gradient in sorted order (backward without a caller buffer)
         float4 *   packAtoms     = nullptr; // N: centre, R
         float4 *   surfAtoms     = nullptr; // surface atoms, compacted in cell order
@@ -99,4 +101,5 @@ namespace sesdf
 
     sesdf_status allocate( Handle & h );
     sesdf_status build_resident( Handle & h );
+    sesdf_status prepare_components( Handle & h ); // once per build, on the handle's stream
 } // namespace sesdf

@@ -7,8 +7,9 @@
 //   arcs     : full circles and segments (A -> B counter-clockwise about n(i -> j));
 //   vertices : concave patch (probe) positions.
 // Outside A, f = min_a |q - c_a| - R_a. The field is 1-Lipschitz and its zero set is the SES.
-// Exterior variant: only elements of the exterior surface component count, and points outside A
-// whose nearest sphere lies in another component (probe-accessible cavities) are inside.
+// Exterior variant: only elements of the exterior boundary component count (components over arcs,
+// vertices and faces, components.h), and points outside A in another region of the complement
+// (probe-accessible cavities) are inside.
 //
 // Gradient (topology fixed): with p the nearest boundary point, e = (p - q) / |p - q| and nu
 // solving sum_m nu_m (p - c_m) = e over the sphere constraints |p - c_m| = R_m of the winning
@@ -45,8 +46,7 @@ namespace sesdf
         uint64_t         n;
         float            band;
         int              exterior;
-        const uint32_t * comp;       // surface component per sorted atom (exterior)
-        const uint32_t * extComp;    // label of the exterior component (device)
+        Components       comp;       // surface components (exterior variant)
         float *          out;
         uint64_t *       codes;      // optional winner codes
         const float *    weights;    // backward
@@ -82,26 +82,6 @@ namespace sesdf
         }
         const uint64_t ix = q % s.dx, iy = ( q / s.dx ) % s.dy, iz = q / ( uint64_t( s.dx ) * s.dy );
         return mk( ox + sx * float( ix ), oy + sy * float( iy ), oz + sz * float( iz ) );
-    }
-
-    // circle (i, j) in world coordinates: centre, unit normal (i -> j), radius
-    __device__ __forceinline__ void circle_world( const Buffers & b, uint32_t i, uint32_t j, f3 & ctr, f3 & nrm, float & rad )
-    {
-        const float4 ai = b.atoms[ i ], aj = b.atoms[ j ];
-        const float  Ri = ai.w + b.probeRadius, Rj = aj.w + b.probeRadius;
-        const f3     dj = xyz( aj ) - xyz( ai );
-        const float  d2 = len2( dj );
-        const float  t  = ( Ri * Ri - Rj * Rj + d2 ) / ( 2.f * d2 );
-        const f3     cl = dj * t;
-        rad             = sqrtf( fmaxf( Ri * Ri - len2( cl ), 0.f ) );
-        nrm             = dj * ( 1.0f / sqrtf( d2 ) );
-        ctr             = xyz( ai ) + cl;
-    }
-
-    __device__ __forceinline__ float angle_ccw( f3 A, f3 u, f3 n )
-    {
-        float t = atan2f( dot( cross( A, u ), n ), dot( A, u ) );
-        return t < 0.f ? t + TwoPi : t;
     }
 
     // In a wave-cooperative evaluation (COOP) all lanes walk the same cells and atoms, so a test
@@ -288,7 +268,7 @@ namespace sesdf
             const float  sd = fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w;
             if ( sd < L.g || ( sd == L.g && a < L.gA ) )
                 L.g = sd, L.gA = a;
-            if ( s.exterior && s.comp[ a ] == extC )
+            if ( s.exterior && s.comp.extStatus[ a ] == 1 )
                 L.gExt = fminf( L.gExt, sd );
         }
     }
@@ -301,26 +281,32 @@ namespace sesdf
     {
         for ( uint32_t k = k0; elems && k < k1; k++ )
         {
-            const uint32_t a = s.surfIds[ k ] & 0x7fffffffu;
-            if ( s.exterior && s.comp[ a ] != extC )
+            const uint32_t a  = s.surfIds[ k ] & 0x7fffffffu;
+            const uint32_t st = s.exterior ? s.comp.extStatus[ a ] : 1u; // 2: element by element
+            if ( st == 0 )
                 continue;
             const float4 aa = s.surfAtoms[ k ];
             if ( !( fabsf( fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w ) < L.best ) )
                 continue;
             const int4 r0 = s.surfRanges[ 2 * k ], r1 = s.surfRanges[ 2 * k + 1 ];
             for ( int f = r0.x; f < r0.y; f++ )
-                full_circle_test( b, L, uint32_t( f ) );
+                if ( st == 1 || s.comp.label[ f ] == extC )
+                    full_circle_test( b, L, uint32_t( f ) );
             for ( int sg = r0.z; sg < r0.w; sg++ )
-                segment_test( L, s.segGeo[ 4 * sg ], s.segGeo[ 4 * sg + 1 ], s.segGeo[ 4 * sg + 2 ], s.segGeo[ 4 * sg + 3 ], uint32_t( sg ) );
+                if ( st == 1 || s.comp.label[ s.comp.capFull + uint32_t( sg ) ] == extC )
+                    segment_test( L, s.segGeo[ 4 * sg ], s.segGeo[ 4 * sg + 1 ], s.segGeo[ 4 * sg + 2 ], s.segGeo[ 4 * sg + 3 ], uint32_t( sg ) );
             for ( int v = r1.x; v < r1.y; v++ )
-                vertex_test( L, b.concavePos[ v ], uint32_t( v ) );
+                if ( st == 1 || s.comp.label[ vertex_node( s.comp, uint32_t( v ) ) ] == extC )
+                    vertex_test( L, b.concavePos[ v ], uint32_t( v ) );
             for ( int v = int( Iw ) + r1.z; v < int( Iw ) + r1.w; v++ )
-                vertex_test( L, b.concavePos[ v ], uint32_t( v ) );
+                if ( st == 1 || s.comp.label[ vertex_node( s.comp, uint32_t( v ) ) ] == extC )
+                    vertex_test( L, b.concavePos[ v ], uint32_t( v ) );
         }
         for ( uint32_t k = k0; faces && k < k1; k++ )
         {
             const uint32_t id = s.surfIds[ k ], a = id & 0x7fffffffu;
-            if ( !( id >> 31 ) || ( s.exterior && s.comp[ a ] != extC ) )
+            const uint32_t st = s.exterior ? s.comp.extStatus[ a ] : 1u;
+            if ( !( id >> 31 ) || st == 0 )
                 continue;
             const float4 aa   = s.surfAtoms[ k ];
             const f3     dq   = L.q - xyz( aa );
@@ -336,6 +322,8 @@ namespace sesdf
                 const float4 cap = s.caps[ c ]; // axis, cos(half angle)
                 exposed          = !( dot( u, xyz( cap ) ) > cap.w );
             }
+            if ( exposed && st == 2 )
+                exposed = label_at( b, s.comp, a, u ) == extC;
             if ( exposed )
                 L.best = fabsf( sd ), L.bc = code_of( SdfFace, a );
         }
@@ -360,7 +348,7 @@ namespace sesdf
             const uint32_t a    = base + lane;
             const bool     ok   = a < a1;
             const float4   mine = ok ? s.sAtoms[ a ] : make_float4( 0.f, 0.f, 0.f, 0.f );
-            const uint32_t ext  = ok && s.exterior && s.comp[ a ] == extC ? 1u : 0u;
+            const uint32_t ext  = ok && s.exterior && s.comp.extStatus[ a ] == 1 ? 1u : 0u;
             const int      cnt  = int( min( uint32_t( WaveSize ), a1 - base ) );
             for ( int j = 0; j < cnt; j++ )
             {
@@ -394,7 +382,9 @@ namespace sesdf
             const bool     ok     = k < k1;
             const float4   myAtom = ok ? s.surfAtoms[ k ] : make_float4( 0.f, 0.f, 0.f, 0.f );
             const uint32_t myId   = ok ? s.surfIds[ k ] : 0u;
-            const uint32_t myExt  = ok && ( !s.exterior || s.comp[ myId & 0x7fffffffu ] == extC ) ? 1u : 0u;
+            const uint32_t mySt   = !ok ? 0u : s.exterior ? s.comp.extStatus[ myId & 0x7fffffffu ] : 1u;
+            const uint32_t myExt  = mySt != 0 ? 1u : 0u;
+            const uint64_t chkMask = ballot( mySt == 2 ); // atoms whose elements are checked one by one
             const int      cnt    = int( min( uint32_t( WaveSize ), k1 - base ) );
             const uint32_t last   = min( k, k1 - 1 );
             if ( elems )
@@ -439,17 +429,23 @@ namespace sesdf
                         staged = true;
                     }
                     const int4 r0 = rl( myR0, j ), r1 = rl( myR1, j );
+                    const bool chk = ( chkMask >> j ) & 1;
                     for ( int f = r0.x; f < r0.y; f++ )
-                        full_circle_test( b, L, uint32_t( f ) );
+                        if ( !chk || s.comp.label[ f ] == extC )
+                            full_circle_test( b, L, uint32_t( f ) );
                     for ( int sg = r0.z; sg < r0.w; sg++ )
                     {
+                        if ( chk && s.comp.label[ s.comp.capFull + uint32_t( sg ) ] != extC )
+                            continue;
                         const float4 * g = segIn ? stage + 4 * ( sg - s0 ) : s.segGeo + 4 * sg;
                         segment_test( L, g[ 0 ], g[ 1 ], g[ 2 ], g[ 3 ], uint32_t( sg ) );
                     }
                     for ( int v = r1.x; v < r1.y; v++ )
-                        vertex_test( L, wIn ? stage[ 256 + v - w0 ] : b.concavePos[ v ], uint32_t( v ) );
+                        if ( !chk || s.comp.label[ vertex_node( s.comp, uint32_t( v ) ) ] == extC )
+                            vertex_test( L, wIn ? stage[ 256 + v - w0 ] : b.concavePos[ v ], uint32_t( v ) );
                     for ( int v = int( Iw ) + r1.z; v < int( Iw ) + r1.w; v++ )
-                        vertex_test( L, nIn ? stage[ 320 + v - n0 ] : b.concavePos[ v ], uint32_t( v ) );
+                        if ( !chk || s.comp.label[ vertex_node( s.comp, uint32_t( v ) ) ] == extC )
+                            vertex_test( L, nIn ? stage[ 320 + v - n0 ] : b.concavePos[ v ], uint32_t( v ) );
                 }
             }
             if ( elems && !faces && keepMask )
@@ -507,6 +503,8 @@ namespace sesdf
                                 exposed          = exposed && !( dot( u, xyz( cap ) ) > cap.w );
                             }
                     }
+                    if ( exposed && ( ( chkMask >> j ) & 1 ) )
+                        exposed = label_at( b, s.comp, id & 0x7fffffffu, u ) == extC;
                     if ( exposed )
                         L.best = fabsf( sd ), L.bc = code_of( SdfFace, id & 0x7fffffffu );
                 }
@@ -531,7 +529,7 @@ namespace sesdf
         const float    D     = rp + s.band;
         const Molecule M     = b.mols[ mol ];
         const uint32_t Iw    = b.ctl->intersectionWithNbNb;
-        const uint32_t extC  = s.exterior ? *s.extComp : 0u;
+        const uint32_t extC  = s.exterior ? s.comp.extComp[ 0 ] : 0u;
         const int      reach = max( int( ceilf( ( D + Rmax + 1e-3f ) / M.cell ) ), 1 );
         int            gx, gy, gz;
         cell_of( M, L.q.x, L.q.y, L.q.z, gx, gy, gz );
@@ -586,8 +584,26 @@ namespace sesdf
 
         // 0: +band (nothing within the band), 1: outside A, 2: inside (pass B)
         int state = 0;
-        if ( L.gA != 0xffffffffu && L.g < D )
-            state = L.g >= 0.f && ( !s.exterior || L.gExt <= L.g + 1e-4f ) ? 1 : 2;
+        if ( L.gA == 0xffffffffu || !( L.g < D ) )
+        {
+            // farther than rp + band from A: +band in the unbounded region, -band in a cavity (pass B
+            // then finds no element within the band)
+            if ( s.exterior && L.active && far_point_in_cavity( b, s.comp, L.q, D, extC ) )
+                state = 2;
+        }
+        else
+        {
+            // outside A the point lies in the region of the complement bounded by the face that holds
+            // its projection onto the nearest sphere; ties within 1e-4 A count as exterior
+            bool outside = L.g >= 0.f && ( !s.exterior || L.gExt <= L.g + 1e-4f );
+            if ( s.exterior && L.g >= 0.f && !outside && s.comp.extStatus[ L.gA ] == 2 )
+            {
+                const f3    dq = L.q - xyz( s.sAtoms[ L.gA ] );
+                const float dl = fsqrt( len2( dq ) );
+                outside        = dl > 0.f && label_at( b, s.comp, L.gA, dq * ( 1.f / dl ) ) == extC;
+            }
+            state = outside ? 1 : 2;
+        }
         const bool needB = L.active && state == 2;
         L.best = D, L.bc = code_of( SdfClamped, 0 );
         if ( any_lane<COOP>( needB ) )
@@ -1015,123 +1031,12 @@ namespace sesdf
             gradInput[ s2i[ a ] ] = gradSorted[ a ];
     }
 
-    // ---- surface components (exterior variant): union-find over the arcs --------------------------
-    // parent[] is read with agent-scope atomic loads: other XCDs hook through atomics and a plain
-    // load could be served from this XCD's L2 with a stale value.
-    __device__ __forceinline__ uint32_t uf_load( uint32_t * parent, uint32_t a )
-    {
-        return __hip_atomic_load( parent + a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT );
-    }
-    __device__ __forceinline__ uint32_t uf_find( uint32_t * parent, uint32_t a )
-    {
-        uint32_t p = uf_load( parent, a );
-        while ( p != a )
-        {
-            a = p;
-            p = uf_load( parent, a );
-        }
-        return a;
-    }
-
-    __global__ void comp_init_kernel( uint32_t * parent, uint32_t n, unsigned long long * best, const float4 * atoms )
-    {
-        const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-        if ( a >= n )
-            return;
-        parent[ a ] = a;
-        // key: largest x + r, ties to the lowest index; float made order-preserving as unsigned
-        const float    v    = atoms[ a ].x + atoms[ a ].w;
-        uint32_t       bits = __float_as_uint( v );
-        bits                = ( bits & 0x80000000u ) ? ~bits : ( bits | 0x80000000u );
-        atomicMax( best, ( (unsigned long long)bits << 32 ) | ( 0xffffffffu - a ) );
-    }
-
-    __global__ void comp_hook_kernel( Buffers b, uint32_t * parent, uint32_t * changed )
-    {
-        const uint32_t Cf = b.ctl->fullCircleNb, G = b.ctl->segmentNb;
-        for ( uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < Cf + G; e += gridDim.x * blockDim.x )
-        {
-            uint32_t u, v;
-            if ( e < Cf )
-                u = b.fullCircles[ e ].x, v = b.fullCircles[ e ].y;
-            else
-                u = b.segments[ e - Cf ].x, v = b.segments[ e - Cf ].y;
-            for ( ;; )
-            {
-                u = uf_find( parent, u );
-                v = uf_find( parent, v );
-                if ( u == v )
-                    break;
-                const uint32_t hi = max( u, v ), lo = min( u, v );
-                if ( atomicCAS( parent + hi, hi, lo ) == hi )
-                {
-                    *changed = 1u;
-                    break;
-                }
-            }
-        }
-    }
-
-    __global__ void comp_compress_kernel( uint32_t * parent, uint32_t n, const unsigned long long * best, uint32_t * extComp )
-    {
-        const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-        if ( a < n )
-            parent[ a ] = uf_find( parent, a );
-        if ( a == 0 )
-        {
-            const uint32_t top = 0xffffffffu - uint32_t( *best & 0xffffffffu );
-            extComp[ 0 ]       = uf_find( parent, top );
-        }
-    }
 } // namespace sesdf
 
 using namespace sesdf;
 
 namespace
 {
-    sesdf_status prepare_components( Handle & h )
-    {
-        if ( h.sdf.component && h.sdf.builtSerial == h.buildSerial )
-            return SESDF_OK;
-        const uint32_t n    = h.n;
-        const size_t   nPad = ( size_t( n ) + 2 + 1 ) & ~size_t( 1 ); // 8-byte aligned slot for the argmax key
-        if ( !h.sdf.component )
-        {
-            void * p = nullptr;
-            if ( hipMalloc( &p, nPad * 4 + 16 ) != hipSuccess )
-                return SESDF_ERR_HIP;
-            h.sdfAllocs.push_back( p );
-            h.sdfBytes += nPad * 4 + 16;
-            h.sdf.component = static_cast<uint32_t *>( p );
-        }
-        uint32_t *           parent  = h.sdf.component;
-        uint32_t *           extComp = parent + n;
-        uint32_t *           changed = parent + n + 1;
-        unsigned long long * best    = reinterpret_cast<unsigned long long *>( parent + nPad );
-        hipStream_t          s       = h.stream;
-        // small fill of the reduction slot with a kernel-free path: host copy
-        const unsigned long long zero64 = 0;
-        if ( hipMemcpyAsync( best, &zero64, 8, hipMemcpyHostToDevice, s ) != hipSuccess )
-            return SESDF_ERR_HIP;
-        comp_init_kernel<<<( n + 255 ) / 256, 256, 0, s>>>( parent, n, best, h.b.atoms );
-        for ( int it = 0; it < 64; it++ )
-        {
-            uint32_t one = 0, flag = 0;
-            if ( hipMemcpyAsync( changed, &one, 4, hipMemcpyHostToDevice, s ) != hipSuccess )
-                return SESDF_ERR_HIP;
-            comp_hook_kernel<<<1024, 256, 0, s>>>( h.b, parent, changed );
-            if ( hipMemcpyAsync( &flag, changed, 4, hipMemcpyDeviceToHost, s ) != hipSuccess || hipStreamSynchronize( s ) != hipSuccess )
-                return SESDF_ERR_HIP;
-            if ( !flag )
-                break;
-        }
-        comp_compress_kernel<<<( n + 255 ) / 256, 256, 0, s>>>( parent, n, best, extComp );
-        if ( hipGetLastError() != hipSuccess )
-            return SESDF_ERR_HIP;
-        h.sdf.builtSerial = h.buildSerial;
-        return SESDF_OK;
-    }
-
     // Packed atoms and the compacted surface atoms of the current build (once per build).
     sesdf_status prepare_surface( Handle & h )
     {
@@ -1240,8 +1145,7 @@ namespace
             sesdf_status st = prepare_components( h );
             if ( st != SESDF_OK )
                 return st;
-            s.comp    = h.sdf.component;
-            s.extComp = h.sdf.component + h.n;
+            s.comp = h.sdf.comp;
         }
         return SESDF_OK;
     }
@@ -1375,8 +1279,8 @@ extern "C"
             *exterior_label = 0;
             return SESDF_OK;
         }
-        if ( hipMemcpyAsync( host_labels, h.sdf.component, size_t( h.n ) * 4, hipMemcpyDeviceToHost, h.stream ) != hipSuccess ||
-             hipMemcpyAsync( exterior_label, h.sdf.component + h.n, 4, hipMemcpyDeviceToHost, h.stream ) != hipSuccess ||
+        if ( hipMemcpyAsync( host_labels, h.sdf.comp.atomLabel, size_t( h.n ) * 4, hipMemcpyDeviceToHost, h.stream ) != hipSuccess ||
+             hipMemcpyAsync( exterior_label, h.sdf.comp.extComp, 4, hipMemcpyDeviceToHost, h.stream ) != hipSuccess ||
              hipStreamSynchronize( h.stream ) != hipSuccess )
             return SESDF_ERR_HIP;
         return SESDF_OK;

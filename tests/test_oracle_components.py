@@ -7,8 +7,11 @@ hold several faces of different components). Pinned here by:
 
 * a voxel flood fill of the complement of A (0.2 A voxels, face connectivity): points at least
   two voxels from A are classified by the flood-fill label and by the oracle's field (outside A the
-  exterior field is rp + g in the unbounded region and at most rp - g in a cavity, g the distance to
-  A), on globules with carved voids and on the many small cavities their packing leaves;
+  exterior field is min(rp + g, band) in the unbounded region and at most rp - g in a cavity, g the
+  distance to A), on globules with carved voids and on the many small cavities their packing
+  leaves; with a narrow band, so that cavity points farther than rp + band from A occur; and on a
+  system of two separate globules with an atom floating in one of the voids (the outer surfaces of
+  both globules bound the unbounded region, the floating atom's sphere bounds the void's region);
 * the voids of the 100,000-atom globule: the complete field is positive at every void centre and
   the exterior field negative;
 * teeth: the rule the first version used (components of atoms joined by arcs, a point taking the
@@ -42,6 +45,21 @@ def flood_fill(atoms, h=0.2, pad=4.0):
         np.minimum(dist[sl], dd.astype(np.float32), out=dist[sl])
     lab, _ = ndimage.label(dist > 0)
     return lo, h, dist, lab, lab[0, 0, 0]
+
+
+def two_globules_and_a_floating_atom(seed):
+    """Two globules 40 A apart (the second with voids) and one atom at the centre of its largest void."""
+    a, _ = synth.globule_voids(1500, 1, synth.SEED_BASE + 300 + seed)
+    b, voids = synth.globule_voids(2500, 2, synth.SEED_BASE + 400 + seed)
+    shift = np.array([a[:, 0].max() - b[:, 0].min() + 40.0, 0, 0, 0], np.float32)
+    v = voids[np.argmax(voids[:, 3])]
+    floating = np.array([[v[0], v[1], v[2], 1.0]], np.float32) + shift
+    return np.concatenate([a, b + shift, floating]).astype(np.float32)
+
+
+def exterior_class(values, band, h=0.2):
+    """Unbounded region iff the field reaches min(band, rp + 2h) (cavity values stay below rp - 2h)."""
+    return values >= min(band, RP + 2 * h) - 1e-6
 
 
 def sample_complement(atoms, seed, n_any=3000, n_cavity=1500):
@@ -79,17 +97,30 @@ def atom_level_rule(res, pts):
     return comp[np.argmin(d, axis=1)] == ext
 
 
-@pytest.mark.parametrize("n,n_voids,seed", [(1500, 2, 5), (3000, 3, 6), (800, 1, 7)])
-def test_exterior_rule_matches_flood_fill(n, n_voids, seed):
-    atoms, _ = synth.globule_voids(n, n_voids, synth.SEED_BASE + 100 + seed)
+def flood_case(n, n_voids, seed):
+    if n == "two":
+        return two_globules_and_a_floating_atom(seed)
+    return synth.globule_voids(n, n_voids, synth.SEED_BASE + 100 + seed)[0]
+
+
+FLOOD_CASES = [(1500, 2, 5, 10.0), (3000, 3, 6, 10.0), (800, 1, 7, 10.0), (3000, 3, 6, 1.2), ("two", 0, 8, 1.2)]
+
+
+@pytest.mark.parametrize("n,n_voids,seed,band", FLOOD_CASES)
+def test_exterior_rule_matches_flood_fill(n, n_voids, seed, band):
+    atoms = flood_case(n, n_voids, seed)
     pts, want = sample_complement(atoms, seed)
     res = O.build(atoms, mode=0, precision=64, track=True)
-    sd = res.sdf(pts, band=10.0, exterior=True, threads=8)
-    got = sd > RP + 1e-9
+    sd = res.sdf(pts, band=band, exterior=True, threads=8)
+    got = exterior_class(sd, band)
     assert (~want).sum() > 500 and want.sum() > 500
+    if band < 2:
+        # cavity points beyond rp + band from A are classified too (by their ray)
+        g = np.sqrt(((pts[:, None, :] - atoms[None, :, :3]) ** 2).sum(-1)) - (atoms[:, 3] + RP)[None, :]
+        assert ((~want) & (g.min(1) > RP + band)).sum() >= 5
     assert np.count_nonzero(got != want) == 0
     # the complete field does not tell cavities apart: positive everywhere outside A
-    assert (res.sdf(pts, band=10.0, exterior=False, threads=8) > RP).all()
+    assert (res.sdf(pts, band=band, exterior=False, threads=8) >= min(band, RP) - 1e-9).all()
 
 
 def test_void_centres_of_the_100k_globule():
