@@ -92,15 +92,13 @@ def test_headline_sdf_grid_256(gpu, headline, exterior):
     surf.sample_grid_ptr(lo, sp, (dim,) * 3, out.data_ptr(), band=band, exterior=exterior)
     torch.cuda.synchronize()
     f = out.cpu().numpy()
-    # 1-Lipschitz along each axis, over the whole grid (the complete surface: the exterior field
-    # jumps where a void's lining meets the outer surface, by its definition)
+    # 1-Lipschitz along each axis, over the whole grid (both fields are signed distances, shifted
+    # by rp and clamped)
     grid = f.reshape(dim, dim, dim)
     for ax, h in ((2, sp[0]), (1, sp[1]), (0, sp[2])):
         jumps = np.abs(np.diff(grid, axis=ax)) > h * (1 + 1e-4) + 1e-5
-        if exterior:
-            _say(f"axis {ax}: {int(jumps.sum())} steps above the grid spacing")
-        else:
-            assert not jumps.any()
+        _say(f"axis {ax}: {int(jumps.sum())} steps above the grid spacing")
+        assert not jumps.any()
     # 500,000 vertices, at the points exactly as the kernel forms them; the point path must give
     # the grid's values, and its winner codes name the elements
     rng = np.random.default_rng(17)
@@ -129,7 +127,8 @@ def test_headline_sdf_grid_256(gpu, headline, exterior):
             _say(f"vertex {int(ii[q])} at {pts[q]}: GPU {pv[q]:.6f} (code {int(pc[q]):#x}) oracle {o[q]:.6f} (code {int(oc[q]):#x})")
     assert err[~excused].max() <= SDF_TOL
     assert excused.mean() < 1e-3
-    assert (o < 0).mean() > 0.2 and (np.abs(o) < band).mean() > 0.1
+    # (the exterior field fills the voids and the packing's small cavities: fewer band vertices)
+    assert (o < 0).mean() > 0.2 and (np.abs(o) < band).mean() > (0.05 if exterior else 0.1)
     surf.close()
 
 
