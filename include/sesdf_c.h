@@ -113,6 +113,10 @@ typedef struct
     uint32_t rebuild_nb;               /* capacity-growth rebuilds performed by the last build */
     uint32_t max_atom_records;
     uint64_t device_bytes;             /* device memory held by the handle */
+    uint64_t build_serial;             /* stage sequences enqueued on the handle so far: changes with every build */
+    uint32_t grid_relayout_nb;         /* grid lay-outs redone because atoms moved outside the cell bricks */
+    uint32_t component_overflow_atoms; /* exterior SDF: atoms whose arcs exceeded the per-atom face grouping
+                                          capacity (their loops were joined) */
 } sesdf_counts;
 
 /* Device pointers of the internal arrays exposed by the reference's analysis getters

@@ -47,7 +47,8 @@ class Counts(ctypes.Structure):
                 ("concave_with_neighbors_nb", u32), ("segment_nb", u32), ("torus_nb", u32),
                 ("neighbor_overflow_atoms", u32), ("probe_neighbor_overflow", u32),
                 ("circle_intersection_overflow", u32), ("jk_anomaly_nb", u32), ("fp64_resolve_nb", u32),
-                ("rebuild_nb", u32), ("max_atom_records", u32), ("device_bytes", u64)]
+                ("rebuild_nb", u32), ("max_atom_records", u32), ("device_bytes", u64), ("build_serial", u64),
+                ("grid_relayout_nb", u32), ("component_overflow_atoms", u32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

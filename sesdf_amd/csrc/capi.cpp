@@ -45,6 +45,41 @@ namespace
             return hip_fail( _e, #expr );                                                                                        \
     } while ( 0 )
 
+// A bounding box the grid can be laid out on: finite, min <= max on each axis (an infinite or
+// inverted box would give a brick of no cells, or none that fits).
+static bool box_ok( const float * bx )
+{
+    for ( int d = 0; d < 3; d++ )
+        if ( !std::isfinite( bx[ d ] ) || !std::isfinite( bx[ d + 3 ] ) || !( bx[ d ] <= bx[ d + 3 ] ) )
+            return false;
+    return true;
+}
+
+// Every coordinate and radius of atoms [b, e) finite.
+static bool atoms_finite( const float * xyzr, uint64_t b, uint64_t e )
+{
+    for ( uint64_t i = 4 * b; i < 4 * e; i++ )
+        if ( !std::isfinite( xyzr[ i ] ) )
+            return false;
+    return true;
+}
+
+static void box_of( const float * xyzr, uint64_t b, uint64_t e, float * out )
+{
+    float mn[ 3 ] = { 3.4e38f, 3.4e38f, 3.4e38f }, mx[ 3 ] = { -3.4e38f, -3.4e38f, -3.4e38f };
+    for ( uint64_t i = b; i < e; i++ )
+        for ( int d = 0; d < 3; d++ )
+        {
+            mn[ d ] = std::min( mn[ d ], xyzr[ 4 * i + d ] - xyzr[ 4 * i + 3 ] );
+            mx[ d ] = std::max( mx[ d ], xyzr[ 4 * i + d ] + xyzr[ 4 * i + 3 ] );
+        }
+    if ( e == b )
+        for ( int d = 0; d < 3; d++ )
+            mn[ d ] = mx[ d ] = 0.f;
+    std::memcpy( out, mn, sizeof( mn ) );
+    std::memcpy( out + 3, mx, sizeof( mx ) );
+}
+
 namespace sesdf
 {
     void Handle::freeAll()
@@ -80,8 +115,9 @@ namespace sesdf
             return _s;                                                                                                           \
     } while ( 0 )
 
-    // Dense per-molecule cell bricks from each molecule's vdW bounding box.
-    static void layout_grid( Handle & h )
+    // Dense per-molecule cell bricks from each molecule's vdW bounding box. Cell indices are u32:
+    // a layout of 2^32 cells or more is refused.
+    static sesdf_status layout_grid( Handle & h )
     {
         uint64_t cellBase = 0;
         for ( size_t m = 0; m < h.mols.size(); m++ )
@@ -108,8 +144,11 @@ namespace sesdf
             M.dz       = int( std::floor( ext[ 2 ] / cell ) ) + 1;
             M.cellBase = uint32_t( cellBase );
             cellBase += uint64_t( M.dx ) * M.dy * M.dz;
+            if ( cellBase >= 0xfffffff0ull )
+                return set_error( SESDF_ERR_CAPACITY, "grid bricks of all molecules exceed 2^32 cells (use a larger cell size)" );
         }
         h.cellNb = uint32_t( cellBase );
+        return SESDF_OK;
     }
 
     static void initial_capacities( Handle & h )
@@ -324,7 +363,7 @@ namespace sesdf
 
     sesdf_status build_resident( Handle & h )
     {
-        h.rebuilds = 0;
+        h.rebuilds = 0, h.relayouts = 0;
         for ( int attempt = 0; attempt < 4; attempt++ )
         {
             sesdf_status st = enqueue( h );
@@ -338,10 +377,25 @@ namespace sesdf
                 h.built = true;
                 return SESDF_OK;
             }
-            // capacities too small: grow, keep the uploaded input, rebuild
+            // capacities too small, or atoms outside the grid: grow / lay the grid out again over the
+            // current atoms, keep the uploaded input, rebuild
             std::vector<float4> keep( h.n );
             HIP_TRY( hipMemcpy( keep.data(), h.input, size_t( h.n ) * sizeof( float4 ), hipMemcpyDeviceToHost ) );
-            if ( !grow( h ) )
+            const bool relaid = ( h.hctl->overflow & OverflowGrid ) != 0;
+            if ( relaid )
+            {
+                const float * xyzr = reinterpret_cast<const float *>( keep.data() );
+                if ( !atoms_finite( xyzr, 0, h.n ) )
+                    return set_error( SESDF_ERR_INVALID, "atom coordinates and radii must be finite" );
+                for ( size_t m = 0; m < h.mols.size(); m++ )
+                    box_of( xyzr, h.mols[ m ].atomBase, uint64_t( h.mols[ m ].atomBase ) + h.mols[ m ].atomNb, h.boxes.data() + 6 * m );
+                const sesdf_status lt = layout_grid( h );
+                if ( lt != SESDF_OK )
+                    return lt;
+                h.freeSdf(); // per-cell SDF arrays follow the grid
+                h.relayouts++;
+            }
+            if ( !grow( h ) && !relaid )
             {
                 g_lastError = "capacity overflow without growth";
                 return SESDF_ERR_CAPACITY;
@@ -356,41 +410,6 @@ namespace sesdf
         return SESDF_ERR_CAPACITY;
     }
 } // namespace sesdf
-
-// A bounding box the grid can be laid out on: finite, min <= max on each axis (an infinite or
-// inverted box would give a brick of no cells, or none that fits).
-static bool box_ok( const float * bx )
-{
-    for ( int d = 0; d < 3; d++ )
-        if ( !std::isfinite( bx[ d ] ) || !std::isfinite( bx[ d + 3 ] ) || !( bx[ d ] <= bx[ d + 3 ] ) )
-            return false;
-    return true;
-}
-
-// Every coordinate and radius of atoms [b, e) finite.
-static bool atoms_finite( const float * xyzr, uint64_t b, uint64_t e )
-{
-    for ( uint64_t i = 4 * b; i < 4 * e; i++ )
-        if ( !std::isfinite( xyzr[ i ] ) )
-            return false;
-    return true;
-}
-
-static void box_of( const float * xyzr, uint64_t b, uint64_t e, float * out )
-{
-    float mn[ 3 ] = { 3.4e38f, 3.4e38f, 3.4e38f }, mx[ 3 ] = { -3.4e38f, -3.4e38f, -3.4e38f };
-    for ( uint64_t i = b; i < e; i++ )
-        for ( int d = 0; d < 3; d++ )
-        {
-            mn[ d ] = std::min( mn[ d ], xyzr[ 4 * i + d ] - xyzr[ 4 * i + 3 ] );
-            mx[ d ] = std::max( mx[ d ], xyzr[ 4 * i + d ] + xyzr[ 4 * i + 3 ] );
-        }
-    if ( e == b )
-        for ( int d = 0; d < 3; d++ )
-            mn[ d ] = mx[ d ] = 0.f;
-    std::memcpy( out, mn, sizeof( mn ) );
-    std::memcpy( out + 3, mx, sizeof( mx ) );
-}
 
 static sesdf_status common_setup( Handle & h, int device, void * stream, uint32_t K, uint32_t maxProbeNb, float rp, int mode, float cell )
 {
@@ -485,9 +504,10 @@ extern "C"
         h->mols[ 0 ]          = Molecule {};
         h->mols[ 0 ].atomBase = 0;
         h->mols[ 0 ].atomNb   = h->n;
-        layout_grid( *h );
+        st = layout_grid( *h );
         initial_capacities( *h );
-        st = allocate( *h );
+        if ( st == SESDF_OK )
+            st = allocate( *h );
         if ( st == SESDF_OK && info->build_surface )
             st = sesdf_build( reinterpret_cast<sesdf_handle *>( h ) );
         if ( st != SESDF_OK )
@@ -546,9 +566,10 @@ extern "C"
                 return set_error( SESDF_ERR_INVALID, "atom coordinates must be finite" );
             }
         }
-        layout_grid( *h );
+        st = layout_grid( *h );
         initial_capacities( *h );
-        st = allocate( *h );
+        if ( st == SESDF_OK )
+            st = allocate( *h );
         if ( st != SESDF_OK )
         {
             sesdf_destroy( reinterpret_cast<sesdf_handle *>( h ) );
@@ -603,9 +624,16 @@ extern "C"
             std::vector<float4> keep( h.n );
             if ( h.n )
                 HIP_TRY( hipMemcpy( keep.data(), h.input, size_t( h.n ) * sizeof( float4 ), hipMemcpyDeviceToHost ) );
-            h.cellSize = t->cell_size;
-            layout_grid( h );
-            sesdf_status st = allocate( h );
+            const float oldCell = h.cellSize;
+            h.cellSize          = t->cell_size;
+            sesdf_status st     = layout_grid( h );
+            if ( st != SESDF_OK )
+            {
+                h.cellSize = oldCell;
+                (void)layout_grid( h );
+                return st;
+            }
+            st = allocate( h );
             if ( st != SESDF_OK )
                 return st;
             if ( h.n )
@@ -738,6 +766,13 @@ extern "C"
         out->rebuild_nb                   = h.rebuilds;
         out->max_atom_records             = c.maxAtomRecords;
         out->device_bytes                 = h.deviceBytes + h.sdfBytes;
+        out->build_serial                 = h.buildSerial;
+        out->grid_relayout_nb             = h.relayouts;
+        uint32_t compOverflow             = 0;
+        if ( h.sdf.compAlloc && h.sdf.compSerial == h.buildSerial &&
+             hipMemcpy( &compOverflow, h.sdf.comp.extComp + 1, 4, hipMemcpyDeviceToHost ) != hipSuccess )
+            compOverflow = 0;
+        out->component_overflow_atoms = compOverflow;
         return SESDF_OK;
     }
 

@@ -111,6 +111,7 @@ namespace sesdf
         OverflowSegments       = 1u << 6,
         OverflowTorus          = 1u << 7,
         OverflowProbeNeighbors = 1u << 8,
+        OverflowGrid           = 1u << 9, // an atom lies outside its molecule's cell brick (moved atoms)
     };
     struct Control
     {

@@ -36,6 +36,12 @@ namespace sesdf
             const Molecule M = b.mols[ m ];
             if ( b.molNb > 1 )
                 b.atomMol[ a ] = m; // molecule-major order: the same for the sorted index a
+            // an atom outside the brick (atoms moved since the grid was laid out, or not finite)
+            // would be clamped into a border cell, where the stencil bounds no longer hold: the
+            // host lays the grid out again and rebuilds
+            const float fx = ( p.x - M.ox ) / M.cell, fy = ( p.y - M.oy ) / M.cell, fz = ( p.z - M.oz ) / M.cell;
+            if ( !( fx >= 0.f && fx < float( M.dx ) && fy >= 0.f && fy < float( M.dy ) && fz >= 0.f && fz < float( M.dz ) ) )
+                set_overflow( b.ctl, OverflowGrid );
             int            gx, gy, gz;
             cell_of( M, p.x, p.y, p.z, gx, gy, gz );
             const uint32_t c = cell_index( M, gx, gy, gz );

@@ -64,6 +64,7 @@ namespace sesdf
         bool        built     = false;
         bool        inputValid = false;
         uint32_t    rebuilds  = 0;
+        uint32_t    relayouts = 0;           // grid lay-outs redone by the last build (moved atoms)
         uint64_t    buildSerial = 0;     // stage sequences enqueued so far (keys the SDF caches)
         uint32_t    stageStride = 32;    // staged concave records per atom (grows on overflow)
         uint32_t    wavesCircles = 4;    // block shapes (autotuned)

@@ -7,6 +7,10 @@ Its backward gives d(sum_q g_q SDF(q_q))/d(atoms) with the patch topology of the
 fixed (the hand-written HIP backward kernel behind the C ABI). ``store=True`` keeps the winning
 element of every query from the forward pass (8 bytes per query); ``store=False`` re-derives it in
 the backward pass.
+
+The handle may be rebuilt between forward and backward (two forwards on one handle before one
+backward, several conformers in one loss): the backward then builds the forward's atoms again
+first. Builds are deterministic, so the stored winners index the same elements.
 """
 import torch
 
@@ -34,13 +38,19 @@ class SurfaceSDF(torch.autograd.Function):
                                   d_codes=codes.data_ptr() if store else None)
         _sync(surface)
         ctx.surface, ctx.band, ctx.exterior, ctx.codes = surface, band, exterior, codes
-        ctx.save_for_backward(pts)
+        ctx.serial = surface.counts()["build_serial"]
+        ctx.save_for_backward(pts, atoms)
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
-        (pts,) = ctx.saved_tensors
+        pts, atoms = ctx.saved_tensors
         surface = ctx.surface
+        if surface.counts()["build_serial"] != ctx.serial:
+            # the handle holds another build now: build the forward's atoms again
+            torch.cuda.current_stream().synchronize()
+            surface.build_device(atoms.data_ptr())
+            _sync(surface)
         w = grad_out.detach().contiguous().float()
         n = surface.atom_nb
         grad = torch.zeros(n, 4, device=pts.device, dtype=torch.float32)

@@ -43,7 +43,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(L.SesdfData) == 120
     assert ctypes.sizeof(L.SsesdfData) == 88
     assert ctypes.sizeof(L.GridDesc) == 36
-    assert ctypes.sizeof(L.Counts) == 80
+    assert ctypes.sizeof(L.Counts) == 96
     assert ctypes.sizeof(L.Internal) == 56
     assert ctypes.sizeof(L.BatchInfo) == 64
 
