@@ -1,14 +1,19 @@
-// Value types of the host interface: a non-owning span, the atom vector, the bounding box and a
-// non-owning device-array view. Same roles as the reference's ConstSpan / Vec4f / Aabb /
-// DeviceBuffer (sesdf/include/bcs/core/type.hpp:21-36, sesdf/include/bcs/core/math.hpp:18-22,
-// sesdf/include/bcs/cuda/memory.cuh), written for this library without glm or CUDA.
+// Value types of the host interface: spans, the atom vector, the bounding box, handle spans,
+// ranges and a non-owning view of a device array. Same names, members and meaning as the
+// reference's Span / ConstSpan / HandleSpan / Range (sesdf/include/bcs/core/type.hpp:9-53),
+// Vec4f / Aabb (sesdf/include/bcs/core/math.hpp) and DeviceBuffer's accessors
+// (sesdf/include/bcs/cuda/memory.cuh:119-160: get(), get<Type>(byteOffset), size<Type>(),
+// toHost<Type>()), written for this library without glm or CUDA.
 #ifndef BCS_CORE_TYPE_HPP
 #define BCS_CORE_TYPE_HPP
 
 #include <algorithm>
+#include <cassert>
 #include <cstddef>
 #include <cstdint>
 #include <vector>
+
+#include "sesdf_c.h"
 
 namespace bcs
 {
@@ -25,6 +30,22 @@ namespace bcs
     };
 
     template<class Type>
+    struct Span
+    {
+        Type *      ptr  = nullptr;
+        std::size_t size = 0;
+
+        Span() = default;
+        Span( Type * p, std::size_t n ) : ptr( p ), size( n ) {}
+        Span( std::vector<Type> & v ) : ptr( v.data() ), size( v.size() ) {}
+
+        Type * data() const { return ptr; }
+        Type * begin() const { return ptr; }
+        Type * end() const { return ptr + size; }
+        Type & operator[]( std::size_t i ) const { return ptr[ i ]; }
+    };
+
+    template<class Type>
     struct ConstSpan
     {
         const Type * ptr  = nullptr;
@@ -33,12 +54,36 @@ namespace bcs
         ConstSpan() = default;
         ConstSpan( const Type * p, std::size_t n ) : ptr( p ), size( n ) {}
         ConstSpan( const std::vector<Type> & v ) : ptr( v.data() ), size( v.size() ) {}
+        ConstSpan( const Span<Type> s ) : ptr( s.ptr ), size( s.size ) {}
 
         const Type * data() const { return ptr; }
         const Type * begin() const { return ptr; }
         const Type * end() const { return ptr + size; }
         const Type & operator[]( std::size_t i ) const { return ptr[ i ]; }
     };
+
+    // Handle of a graphics buffer with an offset and a size in bytes (the *Graphics structs).
+    template<class HandleType>
+    struct HandleSpan
+    {
+        HandleType  handle {};
+        std::size_t offset = 0;
+        std::size_t size   = 0;
+
+        HandleSpan() = default;
+        HandleSpan( HandleType h, std::size_t o, std::size_t s ) : handle( h ), offset( o ), size( s ) {}
+        HandleSpan( HandleType h, std::size_t s ) : handle( h ), offset( 0 ), size( s ) {}
+    };
+
+    template<class Type>
+    struct RangeBase
+    {
+        Type start {};
+        Type end {};
+
+        Type size() const { return end - start; }
+    };
+    using Range = RangeBase<std::size_t>;
 
     struct Aabb
     {
@@ -63,32 +108,50 @@ namespace bcs
         return box;
     }
 
-    // Non-owning view of a device array held by a builder (valid until its next build).
+    // Non-owning view of a device array held by a builder (valid until its next build). The
+    // accessors are the reference DeviceBuffer's: byte offsets in get<Type>, element counts in
+    // size<Type>, and a blocking copy of the whole array in toHost<Type>.
     class DeviceBuffer
     {
       public:
         DeviceBuffer() = default;
-        DeviceBuffer( void * ptr, std::size_t bytes ) : m_ptr( ptr ), m_size( bytes ) {}
+        DeviceBuffer( void * ptr, std::size_t bytes ) : m_ptr( static_cast<uint8_t *>( ptr ) ), m_size( bytes ) {}
+
+        uint8_t *       get() { return m_ptr; }
+        const uint8_t * get() const { return m_ptr; }
+        template<class Type>
+        Type * get( std::size_t byteOffset = 0 )
+        {
+            return reinterpret_cast<Type *>( m_ptr + byteOffset );
+        }
+        template<class Type>
+        const Type * get( std::size_t byteOffset = 0 ) const
+        {
+            return reinterpret_cast<const Type *>( m_ptr + byteOffset );
+        }
+
+        std::size_t size() const { return m_size; }
+        template<class Type>
+        std::size_t size() const
+        {
+            return m_size / sizeof( Type );
+        }
 
         template<class Type>
-        Type * get() const
+        std::vector<Type> toHost() const
         {
-            return static_cast<Type *>( m_ptr );
+            assert( m_size % sizeof( Type ) == 0 && "the array is not a whole number of these elements" );
+            std::vector<Type> out( m_size / sizeof( Type ) );
+            if ( !out.empty() )
+                (void)sesdf_device_to_host( out.data(), m_ptr, out.size() * sizeof( Type ) );
+            return out;
         }
-        std::size_t size() const { return m_size; }
+
+        explicit operator bool() const { return m_ptr != nullptr; }
 
       private:
-        void *      m_ptr  = nullptr;
+        uint8_t *   m_ptr  = nullptr;
         std::size_t m_size = 0;
-    };
-
-    // OpenGL buffer handle span of the reference's *Graphics structs (sesdf/include/bcs/core/type.hpp).
-    template<class Handle>
-    struct HandleSpan
-    {
-        Handle      id     = 0;
-        std::size_t offset = 0;
-        std::size_t size   = 0;
     };
 } // namespace bcs
 

@@ -3,8 +3,9 @@
 // (sesdf/include/bcs/sesdf/sesdf.hpp:15-88); header-only, link with -lsesdf_hip.
 //
 // Differences from the reference, by design:
-//   - graphics = true does not create OpenGL buffers (no GL interop): getGraphics() returns sizes
-//     and offsets with zero ids, getData() returns the device pointers;
+//   - graphics = true does not create OpenGL buffers (no GL interop): getGraphics() returns the
+//     reference's buffer layout (offsets, sizes; ids given by the caller or 0) and packGraphics()
+//     fills caller-owned device buffers in that layout; getData() returns the device pointers;
 //   - the device and stream are explicit (BuildOptions) instead of the max-GFLOPS device;
 //   - SDF sampling and its gradient are added (sampleGrid / samplePoints / sdfBackward).
 #ifndef BCS_SESDF_SESDF_HPP
@@ -36,34 +37,47 @@ namespace bcs
         // Re-reads the molecule span given at construction and rebuilds every output.
         void build() { m_builder.build(); }
 
-        sesdf::SesdfGraphics getGraphics() const
+        // Field layout of the reference's shared GL buffers (sesdf/src/sesdf/sesdf.cu:266-299): one
+        // handle per buffer (ids[0..4]: atoms, convex patches, full circles + sectors, segments,
+        // concave patches; 0 without ids), byte offsets and sizes. packGraphics() fills device
+        // buffers laid out so.
+        sesdf::SesdfGraphics getGraphics( const GLuint * ids = nullptr ) const
         {
-            sesdf_data d {};
-            m_builder.check( sesdf_get_data( m_builder.handle(), &d ), "get_data" );
-            sesdf::SesdfGraphics g;
-            auto span = []( std::size_t bytes ) {
-                HandleSpan<GLuint> h;
-                h.size = bytes;
-                return h;
+            sesdf_graphics_layout L {};
+            m_builder.check( sesdf_get_graphics_layout( m_builder.handle(), &L ), "get_graphics_layout" );
+            const sesdf_counts   c = m_builder.counts();
+            auto span = [ & ]( int q ) {
+                const sesdf_graphics_span & sp = L.spans[ q ];
+                return HandleSpan<GLuint>( ids ? ids[ sp.buffer ] : 0u, std::size_t( sp.offset ), std::size_t( sp.size ) );
             };
-            g.atomNb                  = d.atom_nb;
-            g.atoms                   = span( size_t( d.atom_nb ) * 16 );
-            g.convexPatchNb           = d.convex_patch_nb;
-            g.convexPatches           = span( size_t( d.convex_patch_nb ) * 8 );
-            g.sectorNb                = d.sector_nb;
-            g.sectors                 = span( size_t( d.sector_nb ) * 16 );
-            g.segmentPatchNb          = d.segment_patch_nb;
-            g.segmentPatches          = span( size_t( d.segment_patch_nb ) * 16 );
-            g.circlePatchNb           = d.circle_patch_nb;
-            g.circlePatches           = span( size_t( d.circle_patch_nb ) * 8 );
-            g.concavePatchNb          = d.concave_patch_nb;
-            g.concavePatchesPosition  = span( size_t( d.concave_patch_nb ) * 16 );
-            g.concavePatchesId        = span( size_t( d.concave_patch_nb ) * 16 );
-            const sesdf_counts c      = m_builder.counts();
-            g.concavePatchesNeighbors = span( size_t( c.concave_with_neighbors_nb ) * d.max_concave_neighbors * 16 );
-            g.probeRadius             = d.probe_radius;
-            g.maxConcaveNeighbors     = d.max_concave_neighbors;
+            sesdf::SesdfGraphics g;
+            g.atomNb                  = c.atom_nb;
+            g.atoms                   = span( 0 );
+            g.convexPatchNb           = c.atom_nb;
+            g.convexPatches           = span( 1 );
+            g.sectorNb                = c.sector_nb;
+            g.sectors                 = span( 2 );
+            g.segmentPatchNb          = c.segment_nb;
+            g.segmentPatches          = span( 3 );
+            g.circlePatchNb           = c.full_circle_nb;
+            g.circlePatches           = span( 4 );
+            g.concavePatchNb          = c.concave_patch_nb;
+            g.concavePatchesPosition  = span( 5 );
+            g.concavePatchesId        = span( 6 );
+            g.concavePatchesNeighbors = span( 7 );
+            g.probeRadius             = m_builder.probeRadius();
+            g.maxConcaveNeighbors     = uint16_t( m_builder.internal().max_intersection_neighbors );
             return g;
+        }
+        sesdf_graphics_layout getGraphicsLayout() const
+        {
+            sesdf_graphics_layout L {};
+            m_builder.check( sesdf_get_graphics_layout( m_builder.handle(), &L ), "get_graphics_layout" );
+            return L;
+        }
+        sesdf_status packGraphics( void * const * dBuffers ) const
+        {
+            return m_builder.check( sesdf_pack_graphics( m_builder.handle(), dBuffers ), "pack_graphics" );
         }
 
         sesdf::SesdfData getData()
@@ -99,9 +113,19 @@ namespace bcs
         DeviceBuffer &        getDAtomNeighborsCount() { return refresh().m_dNeighborsCount; }
         DeviceBuffer &        getDAtomNeighborsIndices() { return refresh().m_dNeighborsIndices; }
         DeviceBuffer &        getDCircleVisibilityStatus() { return refresh().m_dStatus; }
-        // The builder keeps the inverse of the reference's globalToTrimmed map: for every intersected
-        // circle (trimmed id) its global slot i * maxNeighbors + l.
-        DeviceBuffer &        getDGlobalToTrimmedId() { return refresh().m_dTrimmedToGlobal; }
+        // For every neighbour slot i * maxNeighbors + l (and one past the last), the number of
+        // intersected circles in earlier slots: the reference's globalToTrimmed map, computed on
+        // request (sesdf/include/bcs/sesdf/operations.inl:46-70).
+        DeviceBuffer & getDGlobalToTrimmedId()
+        {
+            uint32_t * d     = nullptr;
+            uint64_t   count = 0;
+            m_builder.check( sesdf_get_global_to_trimmed( m_builder.handle(), &d, &count ), "get_global_to_trimmed" );
+            m_dGlobalToTrimmed = DeviceBuffer( d, std::size_t( count ) * 4 );
+            return m_dGlobalToTrimmed;
+        }
+        // Its inverse, kept by the builder: the slot of every intersected circle (trimmed id).
+        DeviceBuffer & getDTrimmedToGlobalId() { return refresh().m_dTrimmedToGlobal; }
 
         // Counts of the last build (sizes, overflow and epsilon-band counters, rebuilds, memory).
         sesdf_counts getCounts() const { return m_builder.counts(); }
@@ -146,7 +170,7 @@ namespace bcs
         }
 
         detail::Builder m_builder;
-        DeviceBuffer    m_dAtomIndices, m_dNeighborsCount, m_dNeighborsIndices, m_dStatus, m_dTrimmedToGlobal;
+        DeviceBuffer    m_dAtomIndices, m_dNeighborsCount, m_dNeighborsIndices, m_dStatus, m_dTrimmedToGlobal, m_dGlobalToTrimmed;
     };
 } // namespace bcs
 

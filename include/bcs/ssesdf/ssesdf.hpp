@@ -1,8 +1,9 @@
 // bcs::Ssesdf: exterior solvent-excluded surface (torus records + concave patches) of one molecule
 // on an MI355X, over the C ABI of libsesdf_hip.so. Same constructor, build() and getters as the
 // reference class (sesdf/include/bcs/ssesdf/ssesdf.hpp:14-77); header-only, link with -lsesdf_hip.
-// As for bcs::Sesdf, graphics = true reports buffer sizes only (no OpenGL interop) and the device
-// and stream are explicit.
+// As for bcs::Sesdf, graphics = true creates no OpenGL buffers (getGraphics() describes the
+// reference's buffer layout, packGraphics() fills device buffers in it) and the device and stream
+// are explicit.
 #ifndef BCS_SSESDF_SSESDF_HPP
 #define BCS_SSESDF_SSESDF_HPP
 
@@ -31,30 +32,35 @@ namespace bcs
 
         void build() { m_builder.build(); }
 
-        ssesdf::SsesdfGraphics getGraphics() const
+        // Field layout of the reference's shared GL buffers (sesdf/src/ssesdf/ssesdf.cu:189-210): ids[0..2]
+        // for the atom, circle and concave-patch buffers (0 without ids), byte offsets and sizes.
+        ssesdf::SsesdfGraphics getGraphics( const GLuint * ids = nullptr ) const
         {
-            ssesdf_data d {};
-            m_builder.check( sesdf_get_exterior_data( m_builder.handle(), &d ), "get_exterior_data" );
-            ssesdf::SsesdfGraphics g;
-            auto span = []( std::size_t bytes ) {
-                HandleSpan<GLuint> h;
-                h.size = bytes;
-                return h;
+            sesdf_graphics_layout L {};
+            m_builder.check( sesdf_get_graphics_layout( m_builder.handle(), &L ), "get_graphics_layout" );
+            const sesdf_counts c = m_builder.counts();
+            auto span = [ & ]( int q ) {
+                const sesdf_graphics_span & sp = L.spans[ q ];
+                return HandleSpan<GLuint>( ids ? ids[ sp.buffer ] : 0u, std::size_t( sp.offset ), std::size_t( sp.size ) );
             };
-            g.atomNb                  = d.atom_nb;
-            g.atoms                   = span( size_t( d.atom_nb ) * 16 );
-            g.circlePatchNb           = d.circle_patch_nb;
-            g.circlePosition          = span( size_t( d.circle_patch_nb ) * 16 );
-            g.circleAxis              = span( size_t( d.circle_patch_nb ) * 16 );
-            g.circleVs                = span( size_t( d.circle_patch_nb ) * 16 );
-            g.concavePatchNb          = d.concave_patch_nb;
-            g.concavePatchesPosition  = span( size_t( d.concave_patch_nb ) * 16 );
-            g.concavePatchesId        = span( size_t( d.concave_patch_nb ) * 16 );
-            const sesdf_counts c      = m_builder.counts();
-            g.concavePatchesNeighbors = span( size_t( c.concave_with_neighbors_nb ) * d.max_concave_neighbors * 16 );
-            g.probeRadius             = d.probe_radius;
-            g.maxConcaveNeighbors     = d.max_concave_neighbors;
+            ssesdf::SsesdfGraphics g;
+            g.atomNb                  = c.atom_nb;
+            g.atoms                   = span( 0 );
+            g.circlePatchNb           = c.torus_nb;
+            g.circlePosition          = span( 1 );
+            g.circleAxis              = span( 2 );
+            g.circleVs                = span( 3 );
+            g.concavePatchNb          = c.concave_patch_nb;
+            g.concavePatchesPosition  = span( 4 );
+            g.concavePatchesId        = span( 5 );
+            g.concavePatchesNeighbors = span( 6 );
+            g.probeRadius             = m_builder.probeRadius();
+            g.maxConcaveNeighbors     = uint16_t( m_builder.internal().max_intersection_neighbors );
             return g;
+        }
+        sesdf_status packGraphics( void * const * dBuffers ) const
+        {
+            return m_builder.check( sesdf_pack_graphics( m_builder.handle(), dBuffers ), "pack_graphics" );
         }
 
         ssesdf::SsesdfData getData()

@@ -75,7 +75,7 @@ typedef struct
     uint16_t  max_concave_neighbors;
 } sesdf_data;
 
-/* Mirror of bcs::ssesdf::SsesdfData (sesdf/include/bcs/ssesdf/graphics.hpp:100-119). */
+/* Mirror of bcs::ssesdf::SsesdfData (sesdf/include/bcs/ssesdf/graphics.hpp:31-50). */
 typedef struct
 {
     uint32_t  atom_nb;
@@ -169,6 +169,71 @@ sesdf_status sesdf_get_internal( sesdf_handle * h, sesdf_internal * out );
 sesdf_status sesdf_get_atom_indices( sesdf_handle * h, uint32_t * host_out );
 /* Stream of the handle (hipStream_t). */
 sesdf_status sesdf_get_stream( sesdf_handle * h, void ** stream );
+
+/* -------- Stage entry points (app/detailed/sesdf_benchmark.inl:13-266 drives the reference's stages one
+ * by one). Each enqueues one stage of the build on the handle's stream, on the atoms of the input
+ * buffer (sesdf_upload copies the host atoms there; sesdf_input_buffer gives it for device atoms);
+ * sesdf_sync ends the sequence, reads the counts back and reports SESDF_ERR_CAPACITY when a
+ * capacity overflowed (sesdf_build grows capacities; these entry points do not). Stages run in
+ * this order, each once per sequence: find_circles, find_intersections, build_conic_patches
+ * (complete surface), handle_intersection_singularities, fill_circles (exterior path); otherwise
+ * SESDF_ERR_INVALID, or SESDF_ERR_UNSUPPORTED for a stage of the other surface. -------- */
+/* #1 CPU => GPU of bcs::Sesdf::build (sesdf/src/sesdf/sesdf.cu:211-213): host atoms -> input buffer. */
+sesdf_status sesdf_upload( sesdf_handle * h );
+/* bcs::sesdf::findCircles (sesdf/include/bcs/sesdf/operations.cuh:11-15; ssesdf operations.cuh:11):
+ * grid, neighbour circles, visibility status, compaction, convex sectors. */
+sesdf_status sesdf_find_circles( sesdf_handle * h );
+/* bcs::sesdf::findIntersections (operations.cuh:17-18; ssesdf operations.cuh:13-14): circle-circle
+ * intersections (concave patches), record placement, per-circle lists and probe positions. */
+sesdf_status sesdf_find_intersections( sesdf_handle * h );
+/* bcs::sesdf::buildConicPatches (operations.cuh:20-23): segments. Complete surface only. */
+sesdf_status sesdf_build_conic_patches( sesdf_handle * h );
+/* bcs::sesdf::handleIntersectionSingularities (operations.cuh:25-29; ssesdf operations.cuh:16-20):
+ * probe neighbours of the with-neighbour concave patches. */
+sesdf_status sesdf_handle_intersection_singularities( sesdf_handle * h );
+/* bcs::ssesdf::fillCircles (ssesdf/include/bcs/ssesdf/operations.cuh:29-34): torus records.
+ * Exterior path only. */
+sesdf_status sesdf_fill_circles( sesdf_handle * h );
+
+/* bcs::Sesdf::getDGlobalToTrimmedId (sesdf/include/bcs/sesdf/sesdf.hpp:42; filled at
+ * sesdf/include/bcs/sesdf/operations.inl:46-70): for each of the atom_nb * max_neighbors + 1
+ * neighbour slots, the number of intersected circles (status 1, j > i) in earlier slots. Computed on
+ * request into a device array owned by the handle (valid until the next build). */
+sesdf_status sesdf_get_global_to_trimmed( sesdf_handle * h, uint32_t ** d_map, uint64_t * count );
+
+/* -------- Render-facing output: the shared-buffer layouts of bcs::Sesdf::getGraphics /
+ * bcs::Ssesdf::getGraphics (sesdf/src/sesdf/sesdf.cu:266-299, sesdf/src/ssesdf/ssesdf.cu:189-210)
+ * without OpenGL. The reference keeps its outputs in a few GL buffers and describes every field as
+ * (buffer, byte offset, byte size); sesdf_get_graphics_layout gives the same description for the
+ * last build, and sesdf_pack_graphics copies the outputs into caller-owned device buffers laid out
+ * that way (for a renderer, or GL / Vulkan buffers imported into HIP). Buffers, complete surface:
+ *   0 atoms [N float4]; 1 convex patches [N uint2]; 2 full circles [Cf uint2] then sectors [V float4]
+ *   at Cf * 8 + (Cf * 8) % 16; 3 segments [G uint4]; 4 concave ids [I int4], positions [I float4],
+ *   neighbours [Iw * max_concave_neighbors float4].
+ * Exterior path: 0 atoms; 1 circle positions, axes, visibility spheres [T float4 each];
+ *   2 concave positions [I float4], ids [I int4], neighbours.
+ * Spans, complete: atoms, convex_patches, sectors, segment_patches, circle_patches,
+ *   concave_patches_position, concave_patches_id, concave_patches_neighbors;
+ * exterior: atoms, circle_position, circle_axis, circle_vs, concave_patches_position,
+ *   concave_patches_id, concave_patches_neighbors (span 7 unused). -------- */
+typedef struct
+{
+    uint32_t buffer;
+    uint64_t offset;
+    uint64_t size;
+} sesdf_graphics_span;
+typedef struct
+{
+    uint32_t            buffer_nb;
+    uint64_t            buffer_bytes[ 5 ];
+    sesdf_graphics_span spans[ 8 ];
+} sesdf_graphics_layout;
+sesdf_status sesdf_get_graphics_layout( sesdf_handle * h, sesdf_graphics_layout * out );
+/* d_buffers: buffer_nb device pointers, buffer b holding at least buffer_bytes[b] bytes. Blocking. */
+sesdf_status sesdf_pack_graphics( sesdf_handle * h, void * const * d_buffers );
+
+/* Blocking device -> host copy (DeviceBuffer::toHost, sesdf/include/bcs/cuda/memory.cuh:155). */
+sesdf_status sesdf_device_to_host( void * host, const void * device, size_t bytes );
 
 /* Per-stage device times of the last timed build, milliseconds (app/detailed/sesdf_benchmark.cu:50-232:
  * Circle / Intersection / Segment / Neighborhood). stage_ms receives up to SESDF_STAGE_COUNT values:

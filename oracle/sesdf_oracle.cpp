@@ -194,7 +194,7 @@ namespace orc
     }
 
     // Output of one build, mirroring SesdfData / SsesdfData (sesdf/include/bcs/sesdf/graphics.hpp:39-66,
-    // ssesdf/graphics.hpp:100-119) plus the internal arrays the reference exposes through getters.
+    // ssesdf/graphics.hpp:31-50) plus the internal arrays the reference exposes through getters.
     struct Result
     {
         int      mode = 0; // 0 complete (Sesdf), 1 exterior path (Ssesdf)

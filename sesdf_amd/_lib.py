@@ -98,6 +98,16 @@ SIGNATURES = {
     "sesdf_set_tuning": (i32, [vp, ctypes.c_void_p]),
     "sesdf_batch_create": (i32, [ctypes.POINTER(BatchInfo), ctypes.POINTER(vp)]),
     "sesdf_batch_offsets": (i32, [vp, vp]),
+    "sesdf_upload": (i32, [vp]),
+    "sesdf_find_circles": (i32, [vp]),
+    "sesdf_find_intersections": (i32, [vp]),
+    "sesdf_build_conic_patches": (i32, [vp]),
+    "sesdf_handle_intersection_singularities": (i32, [vp]),
+    "sesdf_fill_circles": (i32, [vp]),
+    "sesdf_get_global_to_trimmed": (i32, [vp, ctypes.POINTER(vp), ctypes.POINTER(u64)]),
+    "sesdf_get_graphics_layout": (i32, [vp, vp]),
+    "sesdf_pack_graphics": (i32, [vp, vp]),
+    "sesdf_device_to_host": (i32, [vp, vp, ctypes.c_size_t]),
     "sesdf_status_string": (ctypes.c_char_p, [i32]),
     "sesdf_last_error": (ctypes.c_char_p, []),
 }

@@ -78,13 +78,21 @@ def build_hip(jobs=4):
     return out
 
 
+CPP_PROGRAMS = ["host_api", "circles_census", "detailed_stages"]
+
+
 def build_host_api():
-    """The C++ host-class example (bcs::Sesdf / bcs::Ssesdf over the C ABI), used by the tests."""
-    src = os.path.join(ROOT, "tests", "cpp", "host_api.cpp")
-    out = os.path.join(ROOT, "tests", "cpp", "host_api")
-    _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
-          "-I", os.path.join(ROOT, "include"), src, "-o", out, "-L", PKG, "-lsesdf_hip", "-Wl,-rpath,$ORIGIN/../../sesdf_amd"])
-    return out
+    """The C++ programs over the host classes (bcs::Sesdf / bcs::Ssesdf over the C ABI) that the tests
+    run: tests/cpp/<name>.cpp -> tests/cpp/<name>."""
+    outs = []
+    for name in CPP_PROGRAMS:
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        out = os.path.join(ROOT, "tests", "cpp", name)
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+              "-I", os.path.join(ROOT, "include"), src, "-o", out, "-L", PKG, "-lsesdf_hip",
+              "-Wl,-rpath,$ORIGIN/../../sesdf_amd"])
+        outs.append(out)
+    return outs
 
 
 def build_oracle():

@@ -87,6 +87,10 @@ namespace sesdf
         if ( graphExec )
             (void)hipGraphExecDestroy( graphExec );
         graphExec = nullptr;
+        if ( g2t )
+            (void)hipFree( g2t );
+        g2t       = nullptr;
+        g2tSerial = ~0ull;
         for ( void * p : allocs )
             (void)hipFree( p );
         allocs.clear();
@@ -282,6 +286,7 @@ namespace sesdf
     {
         HIP_TRY( hipSetDevice( h.device ) );
         h.buildSerial++; // outputs of the previous build are stale from here on
+        h.stageNext = 0;
         if ( h.n == 0 )
             return SESDF_OK;
         if ( h.useGraph )
@@ -856,6 +861,194 @@ extern "C"
         HIP_TRY( hipSetDevice( h.device ) );
         HIP_TRY( hipMemcpyAsync( hostOut, h.b.sortedToInitial, size_t( h.n ) * 4, hipMemcpyDeviceToHost, h.stream ) );
         HIP_TRY( hipStreamSynchronize( h.stream ) );
+        return SESDF_OK;
+    }
+
+    sesdf_status sesdf_get_graphics_layout( sesdf_handle * hh, sesdf_graphics_layout * out )
+    {
+        if ( !hh || !out )
+            return SESDF_ERR_INVALID;
+        const Handle & h = *reinterpret_cast<Handle *>( hh );
+        if ( !h.built )
+            return SESDF_ERR_NOT_BUILT;
+        const Control & c = *h.hctl;
+        const uint64_t  N = h.n, I = uint64_t( c.intersectionWithNbNb ) + c.intersectionNoNbNb;
+        const uint64_t  nbBytes = uint64_t( c.intersectionWithNbNb ) * h.maxProbeNb * 16;
+        std::memset( out, 0, sizeof( *out ) );
+        auto span = [ & ]( int q, uint32_t buffer, uint64_t offset, uint64_t size ) { out->spans[ q ] = { buffer, offset, size }; };
+        if ( h.mode == 0 )
+        {
+            const uint64_t Cf = c.fullCircleNb, V = c.sectorNb, G = c.segmentNb;
+            const uint64_t sectorOffset = Cf * 8 + ( Cf * 8 ) % 16;
+            out->buffer_nb = 5;
+            out->buffer_bytes[ 0 ] = N * 16, out->buffer_bytes[ 1 ] = N * 8, out->buffer_bytes[ 2 ] = sectorOffset + V * 16;
+            out->buffer_bytes[ 3 ] = G * 16, out->buffer_bytes[ 4 ] = I * 32 + nbBytes;
+            span( 0, 0, 0, N * 16 );             // atoms
+            span( 1, 1, 0, N * 8 );              // convex patches
+            span( 2, 2, sectorOffset, V * 16 );  // sectors
+            span( 3, 3, 0, G * 16 );             // segments
+            span( 4, 2, 0, Cf * 8 );             // full circles
+            span( 5, 4, I * 16, I * 16 );        // concave positions
+            span( 6, 4, 0, I * 16 );             // concave ids
+            span( 7, 4, I * 32, nbBytes );       // concave neighbours
+        }
+        else
+        {
+            const uint64_t T = c.torusNb;
+            out->buffer_nb = 3;
+            out->buffer_bytes[ 0 ] = N * 16, out->buffer_bytes[ 1 ] = T * 48, out->buffer_bytes[ 2 ] = I * 32 + nbBytes;
+            span( 0, 0, 0, N * 16 );
+            span( 1, 1, 0, T * 16 );
+            span( 2, 1, T * 16, T * 16 );
+            span( 3, 1, T * 32, T * 16 );
+            span( 4, 2, 0, I * 16 );
+            span( 5, 2, I * 16, I * 16 );
+            span( 6, 2, I * 32, nbBytes );
+        }
+        return SESDF_OK;
+    }
+
+    sesdf_status sesdf_pack_graphics( sesdf_handle * hh, void * const * d )
+    {
+        if ( !hh || !d )
+            return SESDF_ERR_INVALID;
+        Handle &              h = *reinterpret_cast<Handle *>( hh );
+        sesdf_graphics_layout L;
+        const sesdf_status    st = sesdf_get_graphics_layout( hh, &L );
+        if ( st != SESDF_OK )
+            return st;
+        for ( uint32_t q = 0; q < L.buffer_nb; q++ )
+            if ( L.buffer_bytes[ q ] && !d[ q ] )
+                return SESDF_ERR_INVALID;
+        HIP_TRY( hipSetDevice( h.device ) );
+        hipStream_t s   = h.stream;
+        auto        put = [ & ]( const sesdf_graphics_span & sp, const void * src ) -> hipError_t {
+            if ( !sp.size )
+                return hipSuccess;
+            return hipMemcpyAsync( static_cast<char *>( d[ sp.buffer ] ) + sp.offset, src, sp.size, hipMemcpyDeviceToDevice, s );
+        };
+        const Buffers & b = h.b;
+        if ( h.mode == 0 )
+        {
+            const void * src[ 8 ] = { b.atoms, b.convexPatches, b.sectors, b.segments, b.fullCircles, b.concavePos, b.concaveIds, b.concaveNb };
+            for ( int q = 0; q < 8; q++ )
+                HIP_TRY( put( L.spans[ q ], src[ q ] ) );
+            // the alignment gap between full circles and sectors
+            const uint64_t gap = L.spans[ 2 ].offset - L.spans[ 4 ].size;
+            if ( gap )
+                HIP_TRY( hipMemsetAsync( static_cast<char *>( d[ 2 ] ) + L.spans[ 4 ].size, 0, gap, s ) );
+        }
+        else
+        {
+            const void * src[ 7 ] = { b.atoms, b.torusPos, b.torusAxis, b.torusVs, b.concavePos, b.concaveIds, b.concaveNb };
+            for ( int q = 0; q < 7; q++ )
+                HIP_TRY( put( L.spans[ q ], src[ q ] ) );
+        }
+        HIP_TRY( hipStreamSynchronize( s ) );
+        return SESDF_OK;
+    }
+
+    sesdf_status sesdf_device_to_host( void * host, const void * device, size_t bytes )
+    {
+        if ( bytes && ( !host || !device ) )
+            return SESDF_ERR_INVALID;
+        if ( bytes )
+            HIP_TRY( hipMemcpy( host, device, bytes, hipMemcpyDeviceToHost ) );
+        return SESDF_OK;
+    }
+
+    sesdf_status sesdf_upload( sesdf_handle * hh )
+    {
+        if ( !hh )
+            return SESDF_ERR_INVALID;
+        Handle & h = *reinterpret_cast<Handle *>( hh );
+        HIP_TRY( hipSetDevice( h.device ) );
+        if ( h.n )
+        {
+            if ( !h.hostAtoms )
+                return SESDF_ERR_INVALID;
+            HIP_TRY( hipMemcpyAsync( h.input, h.hostAtoms, size_t( h.n ) * sizeof( float4 ), hipMemcpyHostToDevice, h.stream ) );
+        }
+        return SESDF_OK;
+    }
+
+    // Stage entry points: the build's stage sequence in its order, one call per stage of the
+    // reference's operations (stream-ordered, no readback; sesdf_sync ends the sequence).
+    static sesdf_status run_stage( sesdf_handle * hh, uint32_t stage )
+    {
+        if ( !hh )
+            return SESDF_ERR_INVALID;
+        Handle & h = *reinterpret_cast<Handle *>( hh );
+        if ( ( stage == 2 && h.mode != 0 ) || ( stage == 4 && h.mode != 1 ) )
+            return set_error( SESDF_ERR_UNSUPPORTED, stage == 2 ? "conic patches (segments) belong to the complete surface"
+                                                                 : "torus records belong to the exterior path" );
+        if ( stage != 0 && stage < h.stageNext )
+            return set_error( SESDF_ERR_INVALID, "stages must run in order, starting with sesdf_find_circles" );
+        if ( stage != 0 && h.stageNext == 0 )
+            return set_error( SESDF_ERR_INVALID, "sesdf_find_circles must run first" );
+        HIP_TRY( hipSetDevice( h.device ) );
+        const Buffers & b = h.b;
+        hipStream_t     s = h.stream;
+        if ( h.n )
+            switch ( stage )
+            {
+            case 0:
+                launch_begin( b, s );
+                launch_grid( b, s );
+                launch_circles( b, s );
+                launch_circles_finish( b, s );
+                break;
+            case 1:
+                launch_intersections( b, s );
+                launch_intersections_finish( b, s );
+                launch_circle_scan( b, s );
+                launch_fill( b, s );
+                break;
+            case 2: launch_segments( b, s ); break;
+            case 3: launch_probe_neighbors( b, s ); break;
+            case 4: launch_torus( b, s ); break;
+            }
+        if ( stage == 0 )
+            h.buildSerial++;
+        h.stageNext = stage + 1;
+        HIP_TRY( hipGetLastError() );
+        return SESDF_OK;
+    }
+    sesdf_status sesdf_find_circles( sesdf_handle * h ) { return run_stage( h, 0 ); }
+    sesdf_status sesdf_find_intersections( sesdf_handle * h ) { return run_stage( h, 1 ); }
+    sesdf_status sesdf_build_conic_patches( sesdf_handle * h ) { return run_stage( h, 2 ); }
+    sesdf_status sesdf_handle_intersection_singularities( sesdf_handle * h ) { return run_stage( h, 3 ); }
+    sesdf_status sesdf_fill_circles( sesdf_handle * h ) { return run_stage( h, 4 ); }
+
+    sesdf_status sesdf_get_global_to_trimmed( sesdf_handle * hh, uint32_t ** d_map, uint64_t * count )
+    {
+        if ( !hh || !d_map || !count )
+            return SESDF_ERR_INVALID;
+        Handle & h = *reinterpret_cast<Handle *>( hh );
+        if ( !h.built )
+            return SESDF_ERR_NOT_BUILT;
+        if ( h.mode != 0 )
+            return set_error( SESDF_ERR_UNSUPPORTED, "globalToTrimmed belongs to the complete surface" );
+        HIP_TRY( hipSetDevice( h.device ) );
+        const uint64_t slots = uint64_t( h.n ) * h.K + 1;
+        if ( !h.g2t )
+        {
+            void * p = nullptr;
+            HIP_TRY( hipMalloc( &p, slots * 4 ) );
+            h.g2t = static_cast<uint32_t *>( p );
+        }
+        if ( h.g2tSerial != h.buildSerial )
+        {
+            if ( h.n )
+                launch_global_to_trimmed( h.b, h.g2t, h.stream );
+            else
+                HIP_TRY( hipMemsetAsync( h.g2t, 0, 4, h.stream ) );
+            HIP_TRY( hipGetLastError() );
+            HIP_TRY( hipStreamSynchronize( h.stream ) );
+            h.g2tSerial = h.buildSerial;
+        }
+        *d_map = h.g2t;
+        *count = slots;
         return SESDF_OK;
     }
 

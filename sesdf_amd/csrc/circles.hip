@@ -11,6 +11,8 @@
 // flattened across the 64 lanes, and neighbours are appended with a ballot in ascending sorted
 // index (the reference's order). Output offsets come from a scan of per-atom counts, so outputs
 // are deterministic and no N x K mask scan is needed.
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace sesdf
@@ -604,5 +606,34 @@ namespace sesdf
             a.in[ 1 ] = b.atomCnt + 3 * size_t( b.n ); // neighbour counts (summed)
             launch_scan<1>( b, a, s );
         }
+    }
+
+    // The reference's globalToTrimmedId (sesdf/include/bcs/sesdf/operations.inl:46-70): for every
+    // neighbour slot g = i * K + l of the N * K + 1 slots, the exclusive prefix count of the
+    // intersected circles (status 1, j > i) before it. trimmedToGlobal lists those circles' slots in
+    // ascending order, so the count is a lower bound in it.
+    __global__ void global_to_trimmed_kernel( Buffers b, uint32_t * out, uint64_t slots )
+    {
+        const uint32_t Ci = min( b.ctl->intersectedCircleNb, b.capTrimmed );
+        for ( uint64_t g = uint64_t( blockIdx.x ) * blockDim.x + threadIdx.x; g < slots; g += uint64_t( gridDim.x ) * blockDim.x )
+        {
+            uint32_t lo = 0, hi = Ci;
+            while ( lo < hi )
+            {
+                const uint32_t mid = ( lo + hi ) >> 1;
+                if ( uint64_t( b.trimmedToGlobal[ mid ] ) < g )
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            out[ g ] = lo;
+        }
+    }
+
+    void launch_global_to_trimmed( const Buffers & b, uint32_t * out, hipStream_t s )
+    {
+        const uint64_t slots  = uint64_t( b.n ) * b.K + 1;
+        const uint32_t blocks = uint32_t( std::min<uint64_t>( ( slots + 255 ) / 256, uint64_t( device_cu_count() ) * 16 ) );
+        global_to_trimmed_kernel<<<blocks, 256, 0, s>>>( b, out, slots );
     }
 } // namespace sesdf

@@ -82,6 +82,10 @@ namespace sesdf
         float4 *            input = nullptr;
         Control *           hctl  = nullptr;  // pinned host copy of the control block
 
+        uint32_t stageNext = 0;          // stage entry points: the next stage allowed (they run in order)
+        uint32_t * g2t     = nullptr;    // globalToTrimmed (N * K + 1), computed on request
+        uint64_t g2tSerial = ~0ull;
+
         hipGraphExec_t graphExec = nullptr;
         hipEvent_t     events[ SESDF_STAGE_COUNT + 1 ] = {};
 

@@ -115,6 +115,7 @@ namespace sesdf
     void launch_segments( const Buffers & b, hipStream_t s );
     void launch_probe_neighbors( const Buffers & b, hipStream_t s );
     void launch_torus( const Buffers & b, hipStream_t s );
+    void launch_global_to_trimmed( const Buffers & b, uint32_t * out, hipStream_t s ); // N * K + 1 prefix counts
 
     int device_cu_count();
 } // namespace sesdf

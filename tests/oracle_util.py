@@ -14,6 +14,7 @@ def as_build_arrays(res):
         "concave_pos": res["concave_pos"],
         "concave_nb": res["concave_nb"],
         "max_nb": 32,
+        "counts": {"concave_with_neighbors_nb": res.iw},
     }
     if res.mode == 0:
         out.update(full_circles=res["full_circles"], sectors=res["sectors"], convex_patches=res["convex_patches"],

@@ -4,9 +4,10 @@
 Covered: neighbour pairs, circle statuses (unordered in the exterior path), full circles, sector
 values and segments (complete surface), torus records keyed by circle (exterior path; values where
 both sides built a record from the same head atom), concave patches keyed by (sorted input triple,
-side) with their probe positions and hasNeighbors flags. Probe-neighbour sets are left to ``parity.compare``
-on the smaller globules. Every difference must fall in the oracle's epsilon band; ``band_atoms``
-lists the atoms of the differences that do (input indices).
+side) with their probe positions and hasNeighbors flags, and the probe-neighbour set of every
+with-neighbour probe whose list is shorter than the capacity (neighbours identified by their
+positions, which are copies of the probes'). Every difference must fall in the oracle's epsilon
+band; ``band_atoms`` lists the atoms of the differences that do (input indices).
 
 ``keep`` (input indices) restricts neighbour lists to kept atoms and circles to those whose two atoms
 are kept, and concave patches to those whose three atoms are all in ``keep_patches``: a build on a ball cut out of
@@ -178,6 +179,46 @@ def compare_fast(gpu, orc, atoms_in, mode, orc_to_input=None, keep=None, keep_pa
         rep["band"]["has_neighbors"] = int(((hg != ho) & amb_h).sum())
     else:  # exterior path: the flag is not part of its records
         rep["mismatch"]["has_neighbors"] = rep["band"]["has_neighbors"] = 0
+
+    # -- probe neighbours: the set of each with-neighbour probe (lists shorter than the capacity) ---------
+    maxnb = int(gpu["max_nb"])
+    iw_g = int(gpu["counts"]["concave_with_neighbors_nb"])
+    iw_o = int(orc.iw)
+    allk = np.unique(np.concatenate([cg, co]))
+
+    def probe_pairs(ids, pos32, nb, keys, iw, sel):
+        cnt = ids[:iw, 3].astype(np.int64)
+        q = np.repeat(np.arange(iw), cnt)
+        e = np.arange(int(cnt.sum())) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+        ent = np.ascontiguousarray(nb.reshape(-1, 4)[q * maxnb + e, :3], np.float32)
+        pv = np.ascontiguousarray(pos32[:iw, :3], np.float32).view(np.dtype((np.void, 12))).ravel()
+        ev = ent.view(np.dtype((np.void, 12))).ravel()
+        order = np.argsort(pv)
+        loc = np.minimum(np.searchsorted(pv[order], ev), max(iw - 1, 0))
+        found = pv[order][loc] == ev if iw else np.zeros(0, bool)
+        r = order[loc]
+        kid = np.searchsorted(allk, keys[:iw])
+        code = kid[q] * len(allk) + np.where(found, kid[r], len(allk) - 1)
+        comparable = (cnt < maxnb) & sel[:iw]
+        return code, kid, comparable
+
+    if iw_g and iw_o:
+        pc_g, kid_g, cmp_g = probe_pairs(gpu["concave_ids"], gpu["concave_pos"], gpu["concave_nb"], cg, iw_g, sel_g)
+        pc_o, kid_o, cmp_o = probe_pairs(orc["concave_ids"], orc["concave_pos"], orc["concave_nb"], co, iw_o, sel_o)
+        amb_p = (orc["amb_probe_nb"][:iw_o] | orc["amb_concave"][:iw_o]).astype(bool)
+        probes = np.intersect1d(kid_g[cmp_g], kid_o[cmp_o & ~amb_p])
+        band_probes = np.intersect1d(kid_g[cmp_g], kid_o[cmp_o & amb_p])
+        pg_ = pc_g[np.isin(pc_g // len(allk), probes)]
+        po_ = pc_o[np.isin(pc_o // len(allk), probes)]
+        rep["mismatch"]["probe_neighbors"] = int(len(np.unique(np.setxor1d(pg_, po_) // len(allk))))
+        bg_ = pc_g[np.isin(pc_g // len(allk), band_probes)]
+        bo_ = pc_o[np.isin(pc_o // len(allk), band_probes)]
+        rep["band"]["probe_neighbors"] = int(len(np.unique(np.setxor1d(bg_, bo_) // len(allk))))
+        rep["probe_sets_compared"] = int(len(probes))
+    else:
+        rep["mismatch"]["probe_neighbors"] = int(iw_g != iw_o)
+        rep["band"]["probe_neighbors"] = 0
+        rep["probe_sets_compared"] = 0
 
     # -- complete surface: full circles and sectors ------------------------------------------------------
     if mode == 0:

@@ -34,18 +34,22 @@ def test_status_strings_without_gpu():
     assert lib.sesdf_status_string(3) == b"unsupported"
 
 
-def test_struct_layouts_match_header():
-    """ctypes mirrors agree with the C struct sizes compiled from the header."""
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors agree with the C struct sizes the C compiler gives the header's structs."""
+    import subprocess
+
     import sesdf_amd._lib as L
 
-    # sizes of the C structs as laid out by the C compiler (x86-64 SysV)
-    assert ctypes.sizeof(L.CreateInfo) == 88
-    assert ctypes.sizeof(L.SesdfData) == 120
-    assert ctypes.sizeof(L.SsesdfData) == 88
-    assert ctypes.sizeof(L.GridDesc) == 36
-    assert ctypes.sizeof(L.Counts) == 96
-    assert ctypes.sizeof(L.Internal) == 56
-    assert ctypes.sizeof(L.BatchInfo) == 64
+    names = {"sesdf_create_info": L.CreateInfo, "sesdf_data": L.SesdfData, "ssesdf_data": L.SsesdfData,
+             "sesdf_grid_desc": L.GridDesc, "sesdf_counts": L.Counts, "sesdf_internal": L.Internal,
+             "sesdf_batch_info": L.BatchInfo}
+    prog = tmp_path / "sizes.c"
+    prog.write_text('#include <stdio.h>\n#include "sesdf_c.h"\nint main(void) {\n' +
+                    "".join(f'    printf("%zu\\n", sizeof({n}));\n' for n in names) + "    return 0;\n}\n")
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(prog), "-o", str(exe)], check=True)
+    sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert sizes == [ctypes.sizeof(t) for t in names.values()], dict(zip(names, sizes))
 
 
 def test_synthetic_generators_deterministic():

@@ -8,7 +8,9 @@
   20 A cut around surface atoms of the large system, each built by the oracle on its own; the
   neighbour lists and circles of atoms 12 A inside a ball and the patches of atoms 11 A inside do
   not see the cut (checked on the CPU in test_parity_fast.py), so there they must match the GPU
-  build of the whole system. The topology must also not depend on the cell edge at this size.
+  build of the whole system; and the whole system against the FP64 oracle run on all of it. The
+  topology must also not depend on the cell edge at this size. The exterior SDF of the system (a
+  complete-surface build) at 1M points against the FP64 SDF oracle.
 """
 import numpy as np
 import pytest
@@ -164,6 +166,61 @@ def test_large_exterior_balls_match_fp64(gpu, large):
         total_patches += rep["concave_nb"]
     assert total_pairs > 5000 and total_patches > 300
     surf.close()
+
+
+def test_large_exterior_whole_system_matches_fp64(gpu, large):
+    """The whole 2.5M-atom exterior build against the FP64 oracle on all atoms (threaded): lists,
+    statuses, concave patches, probe positions, probe-neighbour sets and torus records."""
+    atoms = large
+    _say("Ssesdf 2.5M atoms, whole build against the FP64 oracle on the whole system")
+    surf = Ssesdf(atoms)
+    g = parity.gpu_arrays(surf)
+    surf.close()
+    orc = O.build(atoms, mode=1, precision=64, track=True, threads=16)
+    rep = PF.compare_fast(g, orc, atoms, 1)
+    _say({k: rep[k] for k in ("neighbor_pairs", "concave_nb", "probe_pos_max_err", "torus_nb", "probe_sets_compared")},
+         rep["mismatch"], rep["band"])
+    PF.assert_parity(rep)
+    assert rep["neighbor_pairs"] > 100_000_000 and rep["concave_nb"] > 2_000_000 and rep["torus_nb"] > 2_500_000
+    assert rep["probe_sets_compared"] > 500_000
+
+
+def test_large_exterior_sdf_at_1m_points(gpu, large):
+    """The exterior SDF of the 2.5M-atom system (a complete-surface build; 500 separate globules,
+    whose outer surfaces all bound the unbounded region) at 1M points within 2 A of atom spheres,
+    against the FP64 SDF oracle."""
+    import torch
+
+    atoms = large
+    _say("Sesdf 2.5M atoms, exterior SDF at 1M points")
+    rng = np.random.default_rng(29)
+    a = rng.integers(0, len(atoms), 1_000_000)
+    u = rng.standard_normal((1_000_000, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    r = atoms[a, 3] + 1.4 + rng.uniform(-2.0, 2.0, 1_000_000)
+    pts = (atoms[a, :3] + u * r[:, None]).astype(np.float32)
+    surf = Sesdf(atoms)
+    P = torch.from_numpy(np.c_[pts, np.zeros(len(pts), np.float32)]).cuda()
+    out = torch.empty(len(pts), device="cuda")
+    codes = torch.empty(len(pts), device="cuda", dtype=torch.int64)
+    surf.sample_points_ptr(P.data_ptr(), len(pts), out.data_ptr(), band=2.0, exterior=True, d_codes=codes.data_ptr())
+    torch.cuda.synchronize()
+    got, pc = out.cpu().numpy(), codes.cpu().numpy()
+    g = parity.gpu_arrays(surf)
+    surf.close()
+    orc = O.build(atoms, mode=0, precision=64, track=True, threads=16)
+    rep = PF.compare_fast(g, orc, atoms, 0)
+    PF.assert_parity(rep)
+    o, oc = orc.sdf(pts.astype(np.float64), band=2.0, exterior=True, threads=16, codes=True)
+    s2i_g, s2i_o = g["s2i"].astype(np.int64), orc["sorted_to_initial"].astype(np.int64)
+    wa_g = _winner_atoms(pc, s2i_g, g["full_circles"], g["segments"], g["concave_ids"])
+    wa_o = _winner_atoms(oc, s2i_o, orc["full_circles"], orc["segments"], orc["concave_ids"])
+    excused = (np.isin(wa_g, rep["band_atoms"]) | np.isin(wa_o, rep["band_atoms"])).any(axis=1)
+    err = np.abs(got - o)
+    _say(f"max |GPU - FP64| {err[~excused].max():.2e} over {int((~excused).sum())} points, {int(excused.sum())} decided by band "
+         f"elements; inside {np.mean(o < 0):.3f}, in band {np.mean(np.abs(o) < 2.0):.3f}")
+    assert err[~excused].max() <= SDF_TOL
+    assert excused.mean() < 1e-3 and (o < 0).mean() > 0.2 and (np.abs(o) < 2.0).mean() > 0.3
 
 
 def test_large_exterior_topology_independent_of_cell(gpu, large):

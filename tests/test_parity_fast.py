@@ -9,7 +9,7 @@ import parity_fast as PF
 from oracle_util import as_build_arrays
 from sesdf_amd import synth
 
-COVERED = ("neighbors", "status", "concave", "has_neighbors", "full_circles", "segments")
+COVERED = ("neighbors", "status", "concave", "has_neighbors", "full_circles", "segments", "probe_neighbors")
 
 
 @pytest.mark.parametrize("mode", [0, 1])
@@ -53,6 +53,15 @@ def test_planted_mistakes_are_reported():
     st[slot] = 2
     arr["status"] = st
     assert PF.compare_fast(arr, f64, atoms, 0)["mismatch"]["status"] >= 1
+
+    arr = dict(base)  # a probe-neighbour list one entry short (a probe away from the band)
+    ids = arr["concave_ids"].copy()
+    amb_p = f64["amb_probe_nb"].astype(bool)
+    q = next(q for q in range(f64.iw) if 1 <= ids[q, 3] < 32 and not amb_p[q])
+    ids[q, 3] -= 1
+    arr["concave_ids"] = ids
+    rep = PF.compare_fast(arr, f64, atoms, 0)
+    assert rep["mismatch"]["probe_neighbors"] == 1 and rep["probe_sets_compared"] > 20
 
     arr = dict(base)  # a probe moved by 1e-3 A
     pos = arr["concave_pos"].copy()
