@@ -475,6 +475,24 @@ namespace orc
                     const Geometry<T> g       = circleGeometry( c[ i ], R[ i ], c[ j ], R[ j ] );
                     uint32_t          visible = 1, complete = 1;
                     bool              hiddenRobust = false, hiddenAmb = false, nearRobust = false, nearAmb = false;
+                    // No circle where the SAS spheres do not meet in one: one contains the other
+                    // (|c_i - c_j| < |R_i - R_j|) or they are tangent or apart (|c_i - c_j| >=
+                    // R_i + R_j, kept by the neighbour test's slack). The reference's radius is then
+                    // the square root of a negative number, NaN, which passes no test (status 2
+                    // with NaN geometry); defined here as hidden. In band within delta of either.
+                    {
+                        const T d2 = len2( c[ j ] - c[ i ] ), dR = R[ i ] - R[ j ], dS = R[ i ] + R[ j ];
+                        const bool edge = inBand( d2 - dR * dR ) || inBand( d2 - dS * dS );
+                        if ( !( d2 > dR * dR && d2 < dS * dS ) )
+                        {
+                            res.status[ size_t( i ) * K + a ] = 0;
+                            if ( track )
+                                res.ambCircle[ size_t( i ) * K + a ] = edge;
+                            continue;
+                        }
+                        if ( track && edge )
+                            hiddenAmb = true;
+                    }
                     for ( uint32_t kk = 0; kk < ni; kk++ )
                     {
                         if ( kk == a )
@@ -782,7 +800,17 @@ namespace orc
                     recCircles[ 3 * q + 1 ] = globalToTrimmedScan[ gjk ];
                     recCircles[ 3 * q + 2 ] = globalToTrimmedScan[ gik ];
                     if ( res.status[ gjk ] != 1 )
+                    {
+                        // The jk circle is not intersected (its status rounded the other way on
+                        // j's side; near-degenerate only). The reference's exclusive scan then
+                        // names j's next intersected circle, which depends on the order of j's
+                        // neighbour slots; the patch is kept out of every jk list instead, and
+                        // the record is ambiguous.
                         res.anomalyJk++;
+                        recCircles[ 3 * q + 1 ] = 0xffffffffu;
+                        if ( track )
+                            res.ambConcave[ q ] = 1;
+                    }
                 }
                 else
                 {
@@ -1207,6 +1235,42 @@ namespace orc
             }
             return NoneLabel;
         }
+        // first arc of atom a crossed by the geodesic from u towards target (unit vectors)
+        uint32_t firstHitTowards( uint32_t a, V3<double> u, V3<double> target ) const
+        {
+            V3<double> w;
+            double     Phi;
+            if ( !frame( u, target, w, Phi ) )
+                return NoneLabel;
+            double   bestPhi = 1e300;
+            uint32_t bestArc = NoneLabel;
+            for ( uint32_t t2 = incStart[ a ]; t2 < incStart[ a + 1 ]; t2++ )
+            {
+                double    ph[ 2 ];
+                const int k = crossArc( a, u, w, Phi, incArcs[ t2 ], ph );
+                for ( int q = 0; q < k; q++ )
+                    if ( ph[ q ] < bestPhi )
+                        bestPhi = ph[ q ], bestArc = incArcs[ t2 ];
+            }
+            return bestArc;
+        }
+        // an inside point of arc e (see insidePoint) that lies in the face e bounds: the geodesic
+        // from it back towards e's cap pole crosses e first; the offset shrinks until it does
+        V3<double> insidePointChecked( uint32_t a, uint32_t e, bool & ok ) const
+        {
+            V3<double> pole;
+            double     cosT;
+            capOf( a, arcOther( e, a ), pole, cosT );
+            V3<double> x { 0, 0, 0 };
+            ok = false;
+            for ( double delta = 1e-4; delta > 5e-10 && !ok; delta *= 0.1 )
+            {
+                x  = insidePoint( a, e, delta );
+                ok = firstHitTowards( a, x, pole ) == e;
+            }
+            return x;
+        }
+
         // component of the exposed point c_a + R_a u of sphere a
         uint32_t labelAt( uint32_t a, V3<double> u ) const
         {
@@ -1399,8 +1463,16 @@ namespace orc
                 // points just inside each loop's face; two loops bound the same face iff every loop
                 // of the atom is crossed an even number of times between them
                 std::vector<V3<double>> pts;
-                for ( uint32_t x : reps )
-                    pts.push_back( insidePoint( a, incArcs[ b0 + x ], 1e-6 ) );
+                for ( uint32_t r0 : reps )
+                {
+                    // any arc of the loop bounds the same face: the first whose inside point checks out
+                    bool       ok = false;
+                    V3<double> pt { 0, 0, 0 };
+                    for ( uint32_t x = 0; x < m && !ok; x++ )
+                        if ( loopOf[ x ] == r0 )
+                            pt = insidePointChecked( a, incArcs[ b0 + x ], ok );
+                    pts.push_back( pt );
+                }
                 for ( size_t p = 0; p < reps.size(); p++ )
                     for ( size_t q = p + 1; q < reps.size(); q++ )
                     {

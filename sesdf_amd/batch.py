@@ -104,6 +104,9 @@ def molecule_slice(data, offs, m):
     ids[:, :3] -= a0
     out["concave_patches_id"] = ids
     out["concave_patches_position"] = data["concave_patches_position"][conc]
+    maxnb = int(data["max_concave_neighbors"])
+    out["concave_patches_neighbors"] = data["concave_patches_neighbors"][w0 * maxnb:w1 * maxnb]
+    out["concave_with_neighbors_nb"] = w1 - w0
     if "sectors" in data:
         out["convex_patches"] = data["convex_patches"][a0:int(offs["atoms"][m + 1])] - int(offs["sectors"][m])
         out["sectors"] = data["sectors"][int(offs["sectors"][m]):int(offs["sectors"][m + 1])]

@@ -59,6 +59,12 @@ namespace sesdf
         return c;
     }
 
+    // The SAS spheres of i and j meet in no circle when R^2 <= 0: one contains the other, or they
+    // are tangent or apart (pairs within the neighbour test's slack of R_i + R_j). The pair then has
+    // no circle: status 0. (The reference takes a square root of R^2 < 0 there, and its NaN radius
+    // passes no test: status 2, a full circle with NaN geometry.)
+    __device__ __forceinline__ bool no_circle( const CircleTest & c ) { return !( c.radius2 > 0.f ); }
+
     // orthogonalVector (sesdf/include/bcs/cuda/math.cuh:32-38)
     __device__ __forceinline__ f3 orthogonal_vector( f3 n )
     {
@@ -287,6 +293,7 @@ namespace sesdf
             const uint64_t nearC = nearM | ( nearM >> 32 );
             bool           hidden  = ( hidC >> c ) & 1u;
             const bool     near    = ( nearC >> c ) & 1u;
+            const bool     none    = no_circle( cg );
             const uint32_t twUpper = __shfl_xor( tw, 32, WaveSize );
             const uint64_t T       = upper ? 0ull : ( uint64_t( tw ) | ( uint64_t( twUpper ) << hk ) ); // slots [0, 2 hk)
             uint32_t       wd[ 4 ] = { uint32_t( T ), uint32_t( T >> 32 ), 0u, 0u };
@@ -295,6 +302,7 @@ namespace sesdf
                 if ( !upper && act && near )
                     circle_exact( sAtom[ w ], sAux[ w ], ni, l, cg, twoRho, hidden, touched, wd );
             const bool     own = !upper && act;
+            hidden             = hidden || none;
             const uint32_t st  = own ? ( hidden ? 0u : ( touched ? 1u : 2u ) ) : 0u;
             for ( uint32_t z = lane; z < s0; z += WaveSize ) // exterior: circles with j < i are not evaluated
                 b.status[ size_t( i ) * K + z ] = 0u;
@@ -395,6 +403,7 @@ namespace sesdf
                 }
                 touch0 = uint64_t( w0 ) | ( uint64_t( w1 ) << 32 );
                 touch1 = uint64_t( w2 ) | ( uint64_t( w3 ) << 32 );
+                hidden = hidden || no_circle( c );
                 st     = active ? ( hidden ? 0u : ( touched ? 1u : 2u ) ) : 0u;
             }
             if ( l < ni )

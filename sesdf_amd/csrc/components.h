@@ -144,6 +144,31 @@ namespace sesdf
         return cnt;
     }
 
+    // First arc of atom a crossed by the geodesic from u towards `target` (unit vectors from c_a);
+    // LabelNone if none is crossed.
+    __device__ __forceinline__ uint32_t first_hit_towards( const Buffers & b, const Components & c, uint32_t a, f3 u, f3 target )
+    {
+        const float4   A  = b.atoms[ a ];
+        const f3       ca = xyz( A );
+        const float    Ra = A.w + b.probeRadius;
+        f3             w;
+        float          Phi;
+        if ( !geo_frame( u, target, w, Phi ) )
+            return LabelNone;
+        float    best = 3.0e38f;
+        uint32_t arc  = LabelNone;
+        for ( uint32_t t2 = c.incStart[ a ]; t2 < c.incStart[ a + 1 ]; t2++ )
+        {
+            float          ph[ 2 ];
+            const uint32_t e = c.incArcs[ t2 ];
+            const int      k = cross_arc( b, c, a, ca, Ra, u, w, Phi, e, ph );
+            for ( int q = 0; q < k; q++ )
+                if ( ph[ q ] < best )
+                    best = ph[ q ], arc = e;
+        }
+        return arc;
+    }
+
     // First arc of atom a crossed from the exposed point u (unit vector from c_a) towards a cap
     // pole; LabelNone if none is found (u not exposed).
     __device__ __forceinline__ uint32_t first_hit( const Buffers & b, const Components & c, uint32_t a, f3 u )

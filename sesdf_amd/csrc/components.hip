@@ -115,7 +115,11 @@ namespace sesdf
     }
 
     // A point of arc e just off it on the exposed side of sphere a (unit vector from c_a): the arc's
-    // midpoint moved by `delta` radians away from the cap the arc bounds.
+    // midpoint moved by `delta` radians away from the cap the arc bounds. The point lies in the face
+    // the arc bounds only if no other arc passes between it and the arc (a small loop, or another
+    // arc close by): then the geodesic from it back towards the cap's pole crosses e first. The
+    // offset is shrunk until that holds (1e-3, 1e-4, 1e-5 rad); `ok` tells whether it did.
+    __device__ f3 inside_point_checked( const Buffers & b, const Components & c, uint32_t a, uint32_t e, bool & ok );
     __device__ f3 inside_point( const Buffers & b, const Components & c, uint32_t a, uint32_t e, float delta )
     {
         const uint2 uv = arc_atoms( b, c, e );
@@ -144,6 +148,21 @@ namespace sesdf
         const f3 v = unit( x - xyz( b.atoms[ a ] ) );
         const f3 t = unit( v * dot( v, pole ) - pole ); // tangent, away from the cap
         return unit( v + t * delta );
+    }
+    __device__ f3 inside_point_checked( const Buffers & b, const Components & c, uint32_t a, uint32_t e, bool & ok )
+    {
+        const uint2 uv = arc_atoms( b, c, e );
+        f3          pole;
+        float       cosT;
+        cap_of( b, a, uv.x == a ? uv.y : uv.x, pole, cosT );
+        f3 x = mk( 0.f, 0.f, 0.f );
+        ok   = false;
+        for ( float delta = 1e-3f; delta > 5e-6f && !ok; delta *= 0.1f )
+        {
+            x  = inside_point( b, c, a, e, delta );
+            ok = first_hit_towards( b, c, a, x, pole ) == e;
+        }
+        return x;
     }
 
     // Loops and faces of each atom. Arcs of atom a chain at shared vertices into loops; two loops
@@ -233,7 +252,15 @@ namespace sesdf
         const float  Ra = A.w + b.probeRadius;
         f3           pts[ MaxAtomLoops ];
         for ( uint32_t p = 0; p < L; p++ )
-            pts[ p ] = inside_point( b, c, a, arcs[ reps[ p ] ], 1e-3f );
+        {
+            // any arc of the loop bounds the same face: the first whose inside point checks out
+            bool ok = false;
+            for ( uint32_t x = 0; x < m && !ok; x++ )
+                if ( root0[ x ] == reps[ p ] )
+                    pts[ p ] = inside_point_checked( b, c, a, arcs[ x ], ok );
+            if ( !ok )
+                atomicAdd( c.extComp + 1, 1u ); // counted with the atoms over capacity
+        }
         for ( uint32_t p = 0; p < L; p++ )
             for ( uint32_t q = p + 1; q < L; q++ )
             {

@@ -21,6 +21,8 @@ namespace sesdf
         void *     compAlloc  = nullptr;
         uint64_t   compSerial = ~0ull;      // build the component labels belong to
         float4 *   gradScratch = nullptr;  // N: gradient in sorted order (backward without a caller buffer)
+        unsigned long long * gradFixed = nullptr; // 4 N + 2: fixed-point accumulators, then the scale words
+        uint32_t   gradFixedCap = 0;
         float4 *   packAtoms     = nullptr; // N: centre, R
         float4 *   surfAtoms     = nullptr; // surface atoms, compacted in cell order
         int4 *     surfRanges    = nullptr; // 2 per surface atom: element ranges

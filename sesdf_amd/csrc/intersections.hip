@@ -57,6 +57,10 @@ namespace sesdf
     // Ill-conditioned triplets (near-collinear centres or discriminant near zero) are re-solved in
     // FP64: the circle-circle intersection point xc and discriminant t of the probe trilateration.
     __device__ __forceinline__ bool ill_conditioned( float den, float t, float Ri2 ) { return den < 1e-3f || fabsf( t ) < 1e-3f * Ri2; }
+    // Probe positions (the fill) take a stricter flag: an FP32 solve errs by about its rounding
+    // (~1e-6 A in i's frame) times 1 / sqrt(den), and the discriminant's root by its rounding over
+    // 2 sqrt(t); below these thresholds that would approach the 1e-4 A the positions are held to.
+    __device__ __forceinline__ bool ill_conditioned_position( float den, float t, float Ri2 ) { return den < 3e-2f || fabsf( t ) < 3e-2f * Ri2; }
 
     // c_b - c_a in FP64 from the stored coordinates: exact, where the FP32 difference rounds (its
     // operands differ by more than a factor of two) and an ill-conditioned solve would amplify that.
@@ -123,7 +127,7 @@ namespace sesdf
         const float     t   = dot( ik.centre - ij.centre, ik.normal ) / den;
         const f3        x1  = ij.centre + u * t;
         const float     c   = Ri2 - len2( x1 );
-        ill                 = ill_conditioned( den, c, Ri2 );
+        ill                 = ill_conditioned_position( den, c, Ri2 );
         const f3 n          = unit( cross( ij.normal, ik.normal ) );
         return x1 + n * ( float( side ) * sqrtf( fmaxf( c, 0.f ) ) );
     }
@@ -135,6 +139,11 @@ namespace sesdf
         for ( uint32_t w = 0; w < slot / 32; w++ )
             acc += __popc( m[ w ] );
         return acc + __popc( m[ slot / 32 ] & ( ( 1u << ( slot % 32 ) ) - 1u ) );
+    }
+
+    __device__ __forceinline__ bool is_intersected( const Buffers & b, uint32_t atom, uint32_t slot )
+    {
+        return ( b.intMask[ size_t( atom ) * ( b.K / 32 ) + slot / 32 ] >> ( slot % 32 ) ) & 1u;
     }
 
     __device__ __forceinline__ int find_in_list( const uint32_t * list, uint32_t n, uint32_t key )
@@ -465,10 +474,16 @@ namespace sesdf
                     set_overflow( b.ctl, OverflowIntersections );
                 if ( MODE == 0 )
                 {
-                    // per-circle intersection counts on (ij, jk, ik) (operations.inl:476-482)
-                    const uint32_t t0 = trimmed_id( b, i, la ), t1 = trimmed_id( b, j, jk ), t2 = trimmed_id( b, i, lk );
-                    if ( !( b.intMask[ size_t( j ) * ( K / 32 ) + jk / 32 ] & ( 1u << ( jk % 32 ) ) ) )
-                        atomicAdd( &b.ctl->jkAnomalyNb, 1u ); // near-degenerate only
+                    // per-circle intersection counts on (ij, jk, ik) (operations.inl:476-482). A jk
+                    // circle that is not intersected (its status rounded the other way on the j
+                    // side; near-degenerate only) has no trimmed slot: the reference's popcount
+                    // then names the next intersected circle of j, which depends on j's slot order,
+                    // so such a patch is left out of every jk list here.
+                    const bool     jkInt = is_intersected( b, j, jk );
+                    const uint32_t t0 = trimmed_id( b, i, la ), t1 = jkInt ? trimmed_id( b, j, jk ) : ci;
+                    const uint32_t t2 = trimmed_id( b, i, lk );
+                    if ( !jkInt )
+                        atomicAdd( &b.ctl->jkAnomalyNb, 1u );
                     if ( t0 < ci )
                         atomicAdd( b.circleIntNb + t0, 1u );
                     if ( t1 < ci )
@@ -532,7 +547,9 @@ namespace sesdf
             if ( MODE == 0 )
             {
                 const uint32_t Ci     = min( b.ctl->intersectedCircleNb, b.capTrimmed );
-                const uint32_t ids[ 3 ] = { trimmed_id( b, i, la ), trimmed_id( b, j, jk ), trimmed_id( b, i, lk ) };
+                // a jk circle that is not intersected takes no patch (as at counting)
+                const uint32_t ids[ 3 ] = { trimmed_id( b, i, la ), is_intersected( b, j, jk ) ? trimmed_id( b, j, jk ) : Ci,
+                                            trimmed_id( b, i, lk ) };
                 for ( int e = 0; e < 3; e++ )
                 {
                     if ( ids[ e ] >= Ci )

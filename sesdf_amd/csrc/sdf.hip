@@ -52,6 +52,7 @@ namespace sesdf
         const float *    weights;    // backward
         int              recompute;  // backward: re-derive the winners instead of reading codes
         float4 *         gradSorted; // backward: per sorted atom (dx, dy, dz, dr)
+        unsigned long long * gradFixed; // backward: 4 fixed-point accumulators per sorted atom; [4 n] max |w| bits
         const float4 *   molGrids;   // batches: per molecule (origin, 0), (spacing, 0); grid dx^3 each
         const uint32_t * perm;       // points given in cell order: result q goes to perm[q] (null: q)
         const float4 *   sAtoms;        // per sorted atom: centre, R = r + rp
@@ -929,17 +930,47 @@ namespace sesdf
         }
     }
 
-    __device__ __forceinline__ void grad_add( float4 * g, uint32_t a, d3 dc, double dr, float w )
+    // Gradient accumulation in fixed point: each contribution w * d is rounded to a multiple of
+    // 2^-k, with 2^k = 2^30 / (max |w| rounded up to a power of two), and added as a 64-bit integer.
+    // Integer sums do not depend on the order of the additions, so the gradient is the same in
+    // every run; the resolution is 2^-30 of max |w| (finer than FP32 for gradients of order
+    // max |w|), and sums stay below 2^63 up to 2^32 max |w| per atom.
+    __device__ __forceinline__ double grad_scale( const unsigned long long * gf, uint32_t n )
     {
-        atomicAdd( &g[ a ].x, float( w * dc.x ) );
-        atomicAdd( &g[ a ].y, float( w * dc.y ) );
-        atomicAdd( &g[ a ].z, float( w * dc.z ) );
-        atomicAdd( &g[ a ].w, float( w * dr ) );
+        const float maxW = __uint_as_float( uint32_t( gf[ 4 * size_t( n ) ] ) );
+        int         e;
+        (void)frexpf( maxW, &e ); // maxW < 2^e
+        return ldexp( 1.0, 30 - e );
+    }
+    __device__ __forceinline__ void grad_add( unsigned long long * g, uint32_t a, d3 dc, double dr, float w, double scale )
+    {
+        const double v[ 4 ] = { double( w ) * dc.x, double( w ) * dc.y, double( w ) * dc.z, double( w ) * dr };
+        for ( int c = 0; c < 4; c++ )
+            atomicAdd( g + 4 * size_t( a ) + c, (unsigned long long)( long long )rint( v[ c ] * scale ) );
+    }
+
+    // max |w| over the queries (bits of a non-negative float order as unsigned integers)
+    __global__ void grad_maxw_kernel( const float * w, uint64_t n, unsigned long long * gf, uint32_t atoms )
+    {
+        float m = 0.f;
+        for ( uint64_t q = uint64_t( blockIdx.x ) * blockDim.x + threadIdx.x; q < n; q += uint64_t( gridDim.x ) * blockDim.x )
+            m = fmaxf( m, fabsf( w[ q ] ) );
+        for ( int o = 32; o > 0; o >>= 1 )
+            m = fmaxf( m, __shfl_xor( m, o, WaveSize ) );
+        if ( lane_id() == 0 && m > 0.f )
+            atomicMax( gf + 4 * size_t( atoms ), (unsigned long long)__float_as_uint( m ) );
+    }
+
+    __global__ void grad_clear_kernel( unsigned long long * gf, uint32_t atoms )
+    {
+        for ( size_t a = size_t( blockIdx.x ) * blockDim.x + threadIdx.x; a < 4 * size_t( atoms ) + 2; a += size_t( gridDim.x ) * blockDim.x )
+            gf[ a ] = 0ull;
     }
 
     __global__ void __launch_bounds__( 256 ) sdf_backward_kernel( Buffers b, SdfArgs s )
     {
-        const double rp = b.probeRadius;
+        const double rp    = b.probeRadius;
+        const double scale = grad_scale( s.gradFixed, b.n );
         for ( uint64_t q = uint64_t( blockIdx.x ) * blockDim.x + threadIdx.x; q < s.n; q += uint64_t( gridDim.x ) * blockDim.x )
         {
             const float w = s.weights[ q ];
@@ -962,7 +993,7 @@ namespace sesdf
             {
                 const d3     v  = Q - C( idx );
                 const double dn = sqrt( dot( v, v ) );
-                grad_add( s.gradSorted, idx, v * ( -1.0 / dn ), -1.0, w );
+                grad_add( s.gradFixed, idx, v * ( -1.0 / dn ), -1.0, w, scale );
                 continue;
             }
             uint32_t at[ 3 ];
@@ -1013,22 +1044,23 @@ namespace sesdf
                 av[ u ] = p - C( at[ u ] );
             solve_nu( m, av, e * ( 1.0 / D ), nu );
             for ( int u = 0; u < m; u++ )
-                grad_add( s.gradSorted, at[ u ], av[ u ] * ( -nu[ u ] ), -nu[ u ] * R( at[ u ] ), w );
+                grad_add( s.gradFixed, at[ u ], av[ u ] * ( -nu[ u ] ), -nu[ u ] * R( at[ u ] ), w, scale );
         }
     }
 
-    __global__ void zero_grad_kernel( float4 * g, uint32_t n )
+    // fixed point -> float, in sorted order (gradSorted) and in input order (gradInput)
+    __global__ void gather_grad_kernel( const unsigned long long * gf, const uint32_t * s2i, uint32_t n, float4 * gradSorted, float4 * gradInput )
     {
         const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-        if ( a < n )
-            g[ a ] = make_float4( 0.f, 0.f, 0.f, 0.f );
-    }
-
-    __global__ void gather_grad_kernel( const float4 * gradSorted, const uint32_t * s2i, uint32_t n, float4 * gradInput )
-    {
-        const uint32_t a = blockIdx.x * blockDim.x + threadIdx.x;
-        if ( a < n )
-            gradInput[ s2i[ a ] ] = gradSorted[ a ];
+        if ( a >= n )
+            return;
+        const double inv = 1.0 / grad_scale( gf, n );
+        float        v[ 4 ];
+        for ( int c = 0; c < 4; c++ )
+            v[ c ] = float( double( ( long long )gf[ 4 * size_t( a ) + c ] ) * inv );
+        const float4 g = make_float4( v[ 0 ], v[ 1 ], v[ 2 ], v[ 3 ] );
+        gradSorted[ a ]      = g;
+        gradInput[ s2i[ a ] ] = g;
     }
 
 } // namespace sesdf
@@ -1412,12 +1444,36 @@ extern "C"
             s.codes     = d.ptCodes;
             s.recompute = 0;
         }
-        // cleared by a kernel, not a memset node (memset nodes of captured graphs are avoided)
-        zero_grad_kernel<<<( h.n + 255 ) / 256, 256, 0, h.stream>>>( s.gradSorted, h.n );
+        // fixed-point accumulators, kept with the handle (cleared by a kernel, not a memset node:
+        // memset nodes of captured graphs are avoided)
+        SdfState & d = h.sdf;
+        if ( !d.gradFixed || d.gradFixedCap < h.n )
+        {
+            if ( d.gradFixed )
+            {
+                h.sdfAllocs.erase( std::find( h.sdfAllocs.begin(), h.sdfAllocs.end(), static_cast<void *>( d.gradFixed ) ) );
+                (void)hipFree( d.gradFixed );
+                d.gradFixed = nullptr;
+            }
+            void *       p     = nullptr;
+            const size_t bytes = ( 4 * size_t( h.n ) + 2 ) * sizeof( unsigned long long );
+            if ( hipMalloc( &p, bytes ) != hipSuccess )
+                return SESDF_ERR_CAPACITY;
+            h.sdfAllocs.push_back( p );
+            h.sdfBytes += bytes;
+            d.gradFixed    = static_cast<unsigned long long *>( p );
+            d.gradFixedCap = h.n;
+        }
+        s.gradFixed = d.gradFixed;
+        const uint32_t wide = uint32_t( device_cu_count() ) * 8;
+        grad_clear_kernel<<<std::min<uint32_t>( uint32_t( ( 4 * size_t( h.n ) + 2 + 255 ) / 256 ), wide ), 256, 0, h.stream>>>( s.gradFixed, h.n );
         if ( npoints )
+        {
+            grad_maxw_kernel<<<std::min<uint32_t>( grid_blocks( npoints ), wide ), 256, 0, h.stream>>>( d_weights, npoints, s.gradFixed, h.n );
             sdf_backward_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+        }
         if ( h.n )
-            gather_grad_kernel<<<( h.n + 255 ) / 256, 256, 0, h.stream>>>( s.gradSorted, h.b.sortedToInitial, h.n,
+            gather_grad_kernel<<<( h.n + 255 ) / 256, 256, 0, h.stream>>>( s.gradFixed, h.b.sortedToInitial, h.n, s.gradSorted,
                                                                           reinterpret_cast<float4 *>( d_grad ) );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from conftest import ROOT
+from sesdf_amd import synth
 
 BIN = os.path.join(ROOT, "tests", "cpp", "host_api")
 SRC = os.path.join(ROOT, "tests", "cpp", "host_api.cpp")

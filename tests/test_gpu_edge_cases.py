@@ -190,15 +190,18 @@ def test_large_probe_radius_wide_lists(gpu, mode):
 def test_tangent_spheres_within_epsilon(gpu, mode):
     """Spheres farther apart than Ri + Rj by less than the predicate's slack are neighbours whose
     circle has radius^2 < 0. The reference's sector angle is then acos(clamp(NaN)) = acos(1) = 0
-    (helper_math clamp is fmaxf(lo, fminf(v, hi))): a point cap, which hides no part of the face."""
+    (helper_math clamp is fmaxf(lo, fminf(v, hi))): a point cap, which hides no part of the face.
+    Here such a pair has no circle at all (status 0 on both sides, no sector): the same surface."""
     d = np.float32(np.sqrt(np.float64(6.2) ** 2 + 5e-5))
     atoms = np.array([[0, 0, 0, 1.7], [d, 0, 0, 1.7]], np.float32)
     surf, orc, rep = _check(atoms, mode)
     if mode == 1:
+        assert surf.counts()["torus_nb"] == 0
         return
     data = surf.get_data()
-    assert data["sectors"].shape == (2, 4) and (data["sectors"][:, 3] == 0.0).all()
-    np.testing.assert_array_equal(orc["sectors"][:, 3], [0.0, 0.0])
+    assert surf.counts()["neighbor_nb"] == 2
+    assert data["sectors"].shape == (0, 4) and orc["sectors"].size == 0
+    assert len(data["circle_patches"]) == 0 and len(data["segment_patches"]) == 0
     import torch
 
     # the face of each atom stays exposed: inside the SAS ball of atom 0, on its far side, the field

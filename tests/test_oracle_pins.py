@@ -152,7 +152,9 @@ def test_torus_records_invariants(n):
     res = O.build(atoms, mode=1, precision=64, track=False, threads=8)
     rep = torus_report(res)
     _assert_torus(rep)
-    assert rep["records"] > n // 3 and rep["tangent"] < 1e-3 * rep["records"]
+    # pairs without a real circle (tangent within the neighbour slack, or contained) have status 0:
+    # no record
+    assert rep["records"] > n // 3 and rep["tangent"] == 0
 
 
 def test_torus_check_detects_planted_mistakes():
