@@ -20,9 +20,10 @@ HIP_SOURCES = [
     "intersections.hip",
     "sdf.hip",
     "components.hip",
+    "sort.hip",
     "capi.cpp",
 ]
-HIP_HEADERS = ["common.h", "kernels.h", "handle.h", "components.h"]
+HIP_HEADERS = ["common.h", "kernels.h", "handle.h", "components.h", "sort.h"]
 
 
 def _run(cmd, cwd=None):

@@ -37,6 +37,8 @@ namespace
     }
 } // namespace
 
+sesdf_status sesdf::report_error( sesdf_status st, const char * what ) { return set_error( st, what ); }
+
 #define HIP_TRY( expr )                                                                                                          \
     do                                                                                                                           \
     {                                                                                                                            \

@@ -446,7 +446,7 @@ namespace sesdf
         c.extStatus[ a ] = st;
     }
 
-    sesdf_status prepare_components( Handle & h )
+    sesdf_status prepare_components( Handle & h, bool mayAlloc )
     {
         SdfState & d = h.sdf;
         if ( d.compAlloc && d.compSerial == h.buildSerial )
@@ -455,6 +455,8 @@ namespace sesdf
         if ( !d.compAlloc || d.comp.capFull < h.cap.full || d.comp.capSeg < h.cap.segments || d.comp.capRec < h.cap.records ||
              d.comp.cellNb != h.cellNb )
         {
+            if ( !mayAlloc )
+                return SESDF_ERR_CAPACITY;
             if ( d.compAlloc )
             {
                 h.sdfAllocs.erase( std::find( h.sdfAllocs.begin(), h.sdfAllocs.end(), d.compAlloc ) );

@@ -37,11 +37,8 @@ namespace sesdf
         uint32_t   geoSectors    = 0;
         uint32_t   geoSegments   = 0;
         uint64_t   packSerial    = ~0ull; // build the packed / surface arrays belong to
-        float4 *   ptSorted    = nullptr;  // scattered points in cell order (ptCap)
-        uint32_t * ptWords     = nullptr;  // 4 x ptCap: cell keys and point indices, double-buffered
-        uint32_t   ptCap       = 0;
-        void *     ptTemp      = nullptr;  // radix sort workspace
-        size_t     ptTempBytes = 0;
+        float4 *   ptSorted    = nullptr;  // point sort space (ptCap): points in cell order, keys and
+        uint32_t   ptCap       = 0;        // indices double-buffered, the sort's histograms
         uint64_t * ptCodes     = nullptr;  // backward without stored winners: codes of a forward pass
         float *    ptVals      = nullptr;  //   and its values (ptEvalCap each)
         uint64_t   ptEvalCap   = 0;
@@ -108,5 +105,9 @@ namespace sesdf
 
     sesdf_status allocate( Handle & h );
     sesdf_status build_resident( Handle & h );
-    sesdf_status prepare_components( Handle & h ); // once per build, on the handle's stream
+    // Once per build, on the handle's stream. With mayAlloc false, SESDF_ERR_CAPACITY instead of
+    // allocating when the component buffers are missing or too small for the build.
+    sesdf_status prepare_components( Handle & h, bool mayAlloc = true );
+    // Records the message that sesdf_last_error returns; returns st.
+    sesdf_status report_error( sesdf_status st, const char * what );
 } // namespace sesdf

@@ -16,13 +16,13 @@
 // element (1 for a face, 2 for an arc, 3 for a vertex):
 //   dSDF/dc_m = -nu_m (p - c_m),  dSDF/dr_m = -nu_m R_m   (envelope theorem);
 // outside A: dSDF/dc_a = -(q - c_a) / |q - c_a|, dSDF/dr_a = -1.
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstring>
 #include <vector>
 
 #include "handle.h"
+#include "sort.h"
 
 namespace sesdf
 {
@@ -1070,12 +1070,15 @@ using namespace sesdf;
 namespace
 {
     // Packed atoms and the compacted surface atoms of the current build (once per build).
-    sesdf_status prepare_surface( Handle & h )
+    // With mayAlloc false, SESDF_ERR_CAPACITY instead of allocating missing or too small buffers.
+    sesdf_status prepare_surface( Handle & h, bool mayAlloc )
     {
         SdfState &     d = h.sdf;
         const uint32_t n = h.n;
         if ( d.packAtoms && d.packSerial == h.buildSerial )
             return SESDF_OK;
+        if ( !mayAlloc && ( !d.packAtoms || !d.geo || d.geoSectors < h.cap.sectors || d.geoSegments < h.cap.segments ) )
+            return SESDF_ERR_CAPACITY;
         if ( !d.packAtoms )
         {
             const size_t n1    = size_t( n ) + 1;
@@ -1144,7 +1147,7 @@ namespace
         return SESDF_OK;
     }
 
-    sesdf_status sdf_common( Handle & h, SdfArgs & s, int flags )
+    sesdf_status sdf_common( Handle & h, SdfArgs & s, int flags, bool mayAlloc = true )
     {
         if ( !h.built )
             return SESDF_ERR_NOT_BUILT;
@@ -1161,7 +1164,7 @@ namespace
             s.empty = 1; // no surface: nothing to prepare, every sample reads +band
             return SESDF_OK;
         }
-        sesdf_status pst = prepare_surface( h );
+        sesdf_status pst = prepare_surface( h, mayAlloc );
         if ( pst != SESDF_OK )
             return pst;
         s.sAtoms        = h.sdf.packAtoms;
@@ -1174,7 +1177,7 @@ namespace
         s.segGeo        = h.sdf.geo + h.sdf.geoSectors;
         if ( s.exterior )
         {
-            sesdf_status st = prepare_components( h );
+            sesdf_status st = prepare_components( h, mayAlloc );
             if ( st != SESDF_OK )
                 return st;
             s.comp = h.sdf.comp;
@@ -1211,53 +1214,67 @@ namespace
     constexpr uint64_t PointSortMin = 8192;
     constexpr uint64_t PointSortMax = 0x7fffffffull; // the sort counts items in int
 
-    sesdf_status sort_points( Handle & h, SdfArgs & s )
+    // Buffers of the point sort: the points in cell order, two key and two index buffers of cap
+    // words each, and the sort's histogram workspace.
+    struct PointSortSpace
     {
-        SdfState &     d = h.sdf;
-        const uint32_t n = uint32_t( s.n );
+        float4 *   sorted;
+        uint32_t * words;
+        uint32_t * hist;
+        uint64_t   cap;
+    };
+
+    size_t point_sort_bytes( uint64_t cap ) { return cap * ( sizeof( float4 ) + 4 * sizeof( uint32_t ) ) + sort_hist_words( cap ) * 4; }
+
+    PointSortSpace point_sort_space( void * p, uint64_t cap )
+    {
+        PointSortSpace w;
+        w.sorted = static_cast<float4 *>( p );
+        w.words  = reinterpret_cast<uint32_t *>( w.sorted + cap );
+        w.hist   = w.words + 4 * cap;
+        w.cap    = cap;
+        return w;
+    }
+
+    void sort_points_in( Handle & h, SdfArgs & s, const PointSortSpace & w )
+    {
+        const uint32_t n  = uint32_t( s.n );
         hipStream_t    st = h.stream;
         int            bits = 1;
         while ( bits < 32 && ( uint64_t( 1 ) << bits ) < uint64_t( h.cellNb ) )
             bits++;
-        if ( n > d.ptCap )
+        uint32_t *k0 = w.words, *k1 = w.words + w.cap, *v0 = w.words + 2 * w.cap, *v1 = w.words + 3 * w.cap;
+        sdf_point_keys_kernel<<<( n + 255 ) / 256, 256, 0, st>>>( h.b, s.points, n, k0, v0 );
+        const int where = radix_sort_pairs( k0, v0, k1, v1, w.hist, n, bits, st );
+        const uint32_t * perm = where ? v1 : v0;
+        sdf_point_gather_kernel<<<( n + 255 ) / 256, 256, 0, st>>>( s.points, perm, n, w.sorted );
+        s.points = w.sorted;
+        s.perm   = perm;
+    }
+
+    sesdf_status sort_points( Handle & h, SdfArgs & s )
+    {
+        SdfState & d = h.sdf;
+        if ( s.n > d.ptCap )
         {
             // grown on demand, kept with the other sampling buffers
-            const uint64_t cap   = std::min<uint64_t>( std::max<uint64_t>( n, 2 * uint64_t( d.ptCap ) ), PointSortMax );
-            const size_t   words = size_t( 4 ) * cap; // two key and two index buffers
+            const uint64_t cap   = std::min<uint64_t>( std::max<uint64_t>( s.n, 2 * uint64_t( d.ptCap ) ), PointSortMax );
+            const size_t   bytes = point_sort_bytes( cap );
             void *         p     = nullptr;
-            if ( hipMalloc( &p, words * sizeof( uint32_t ) + cap * sizeof( float4 ) ) != hipSuccess )
+            if ( hipMalloc( &p, bytes ) != hipSuccess )
                 return SESDF_ERR_CAPACITY;
-            h.sdfAllocs.push_back( p );
-            h.sdfBytes += words * sizeof( uint32_t ) + cap * sizeof( float4 );
-            float4 *   sorted = static_cast<float4 *>( p );
-            uint32_t * w      = reinterpret_cast<uint32_t *>( sorted + cap );
-            size_t     need   = 0;
-            hipcub::DoubleBuffer<uint32_t> k0( w, w + cap ), v0( w + 2 * cap, w + 3 * cap );
-            if ( hipcub::DeviceRadixSort::SortPairs( nullptr, need, k0, v0, int( cap ), 0, 32, st ) != hipSuccess )
-                return SESDF_ERR_HIP;
-            if ( need > d.ptTempBytes )
+            if ( d.ptSorted )
             {
-                void * t = nullptr;
-                if ( hipMalloc( &t, need ) != hipSuccess )
-                    return SESDF_ERR_CAPACITY;
-                h.sdfAllocs.push_back( t );
-                h.sdfBytes += need;
-                d.ptTemp      = t;
-                d.ptTempBytes = need;
+                h.sdfAllocs.erase( std::find( h.sdfAllocs.begin(), h.sdfAllocs.end(), static_cast<void *>( d.ptSorted ) ) );
+                h.sdfBytes -= point_sort_bytes( d.ptCap );
+                (void)hipFree( d.ptSorted );
             }
-            d.ptSorted = sorted;
-            d.ptWords  = w;
+            h.sdfAllocs.push_back( p );
+            h.sdfBytes += bytes;
+            d.ptSorted = static_cast<float4 *>( p );
             d.ptCap    = uint32_t( cap );
         }
-        const uint64_t cap = d.ptCap;
-        hipcub::DoubleBuffer<uint32_t> keys( d.ptWords, d.ptWords + cap ), idx( d.ptWords + 2 * cap, d.ptWords + 3 * cap );
-        sdf_point_keys_kernel<<<( n + 255 ) / 256, 256, 0, st>>>( h.b, s.points, n, keys.Current(), idx.Current() );
-        size_t tempBytes = d.ptTempBytes;
-        if ( hipcub::DeviceRadixSort::SortPairs( d.ptTemp, tempBytes, keys, idx, int( n ), 0, bits, st ) != hipSuccess )
-            return SESDF_ERR_HIP;
-        sdf_point_gather_kernel<<<( n + 255 ) / 256, 256, 0, st>>>( s.points, idx.Current(), n, d.ptSorted );
-        s.points = d.ptSorted;
-        s.perm   = idx.Current();
+        sort_points_in( h, s, point_sort_space( d.ptSorted, d.ptCap ) );
         return SESDF_OK;
     }
 } // namespace
