@@ -262,6 +262,19 @@ sesdf_status sesdf_sample_points( sesdf_handle * h, const float * d_points, uint
  * type << 40 | index; types 0 outside, 1 face, 2 full circle, 3 segment, 4 vertex, 5 clamped). */
 sesdf_status sesdf_sample_points_ex( sesdf_handle * h, const float * d_points, uint64_t npoints, float band, int flags, float * d_out,
                                      uint64_t * d_codes );
+/* Allocation-free form of sesdf_sample_points_ex. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The point sort (from 8,192 points on) runs in one device array
+ * of SESDF_NOALLOC_SCRATCH_BYTES that the library holds on every GPU it runs on until the process
+ * ends, so npoints may not exceed SESDF_NOALLOC_MAX_POINTS (SESDF_ERR_INVALID). The sampling state
+ * of the current build (packed atoms and cap geometry; with SESDF_SDF_EXTERIOR the surface
+ * components) must already be sized by one sesdf_sample_points_ex call with the same flags after
+ * a build of this size, else SESDF_ERR_CAPACITY. Every call on one GPU uses the same array: calls
+ * that overlap on a GPU (any handle, any stream) race. */
+#define SESDF_NOALLOC_MAX_POINTS    1048576u
+#define SESDF_NOALLOC_SCRATCH_BYTES 34079744u /* 32 bytes per point + the sort's tile histograms */
+sesdf_status sesdf_sample_points_noalloc( sesdf_handle * h, const float * d_points, uint64_t npoints, float band, int flags,
+                                          float * d_out, uint64_t * d_codes );
 /* SDF on a regular vertex grid (x fastest), written to d_out (float, dims product). */
 sesdf_status sesdf_sample_grid( sesdf_handle * h, const sesdf_grid_desc * grid, float band, float * d_out );
 sesdf_status sesdf_sample_grid_ex( sesdf_handle * h, const sesdf_grid_desc * grid, float band, int flags, float * d_out );

@@ -91,6 +91,7 @@ SIGNATURES = {
     "sesdf_sample_grid": (i32, [vp, ctypes.POINTER(GridDesc), f32, vp]),
     "sesdf_sdf_backward": (i32, [vp, vp, vp, u64, f32, ctypes.c_int, vp]),
     "sesdf_sample_points_ex": (i32, [vp, vp, u64, f32, ctypes.c_int, vp, vp]),
+    "sesdf_sample_points_noalloc": (i32, [vp, vp, u64, f32, ctypes.c_int, vp, vp]),
     "sesdf_sample_grid_ex": (i32, [vp, ctypes.POINTER(GridDesc), f32, ctypes.c_int, vp]),
     "sesdf_sdf_backward_ex": (i32, [vp, vp, vp, u64, f32, ctypes.c_int, vp, vp, vp]),
     "sesdf_surface_components": (i32, [vp, vp, vp]),

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "handle.h"
@@ -1224,7 +1225,7 @@ namespace
         uint64_t   cap;
     };
 
-    size_t point_sort_bytes( uint64_t cap ) { return cap * ( sizeof( float4 ) + 4 * sizeof( uint32_t ) ) + sort_hist_words( cap ) * 4; }
+    constexpr size_t point_sort_bytes( uint64_t cap ) { return cap * ( sizeof( float4 ) + 4 * sizeof( uint32_t ) ) + sort_hist_words( cap ) * 4; }
 
     PointSortSpace point_sort_space( void * p, uint64_t cap )
     {
@@ -1279,8 +1280,79 @@ namespace
     }
 } // namespace
 
+// Device array of the allocation-free sampling entry point: the point sort space for up to
+// SESDF_NOALLOC_MAX_POINTS points. Each GPU holds its own copy; host code reaches it only through
+// the address hipGetSymbolAddress returns on that GPU (the name is a host-side stand-in).
+static __device__ __align__( 256 ) unsigned char scratch[ SESDF_NOALLOC_SCRATCH_BYTES ];
+
+namespace
+{
+    static_assert( point_sort_bytes( SESDF_NOALLOC_MAX_POINTS ) == SESDF_NOALLOC_SCRATCH_BYTES, "scratch size" );
+
+    std::mutex           g_scratchMutex;
+    std::vector<void *>  g_scratchAddr; // per device ordinal
+
+    // Address of `scratch` on the current device (the caller made h.device current), looked up on
+    // the first call per device and kept.
+    sesdf_status scratch_address( int device, void ** out )
+    {
+        std::lock_guard<std::mutex> lock( g_scratchMutex );
+        if ( device < 0 )
+            return SESDF_ERR_INVALID;
+        if ( size_t( device ) >= g_scratchAddr.size() )
+            g_scratchAddr.resize( size_t( device ) + 1, nullptr );
+        if ( !g_scratchAddr[ device ] )
+        {
+            void *           p = nullptr;
+            const hipError_t e = hipGetSymbolAddress( &p, HIP_SYMBOL( scratch ) );
+            if ( e != hipSuccess || !p )
+                return report_error( SESDF_ERR_HIP, "hipGetSymbolAddress(scratch) failed" );
+            g_scratchAddr[ device ] = p;
+        }
+        *out = g_scratchAddr[ device ];
+        return SESDF_OK;
+    }
+} // namespace
+
 extern "C"
 {
+    sesdf_status sesdf_sample_points_noalloc( sesdf_handle * hh, const float * d_points, uint64_t npoints, float band, int flags,
+                                              float * d_out, uint64_t * d_codes )
+    {
+        if ( !hh || ( npoints && ( !d_points || !d_out ) ) )
+            return SESDF_ERR_INVALID;
+        if ( npoints > SESDF_NOALLOC_MAX_POINTS )
+            return report_error( SESDF_ERR_INVALID, "sesdf_sample_points_noalloc: npoints exceeds SESDF_NOALLOC_MAX_POINTS (1048576)" );
+        Handle & h = *reinterpret_cast<Handle *>( hh );
+        SdfArgs  s {};
+        s.points        = reinterpret_cast<const float4 *>( d_points );
+        s.n             = npoints;
+        s.band          = band;
+        s.out           = d_out;
+        s.codes         = d_codes;
+        sesdf_status st = sdf_common( h, s, flags, false );
+        if ( st == SESDF_ERR_CAPACITY )
+            return report_error( st, "sesdf_sample_points_noalloc: the sampling state of this build is not in place "
+                                     "(call sesdf_sample_points_ex once with the same flags after the build)" );
+        if ( st != SESDF_OK || npoints == 0 )
+            return st;
+        if ( s.empty )
+            sdf_fill_empty_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( d_out, d_codes, npoints, band );
+        else
+        {
+            if ( npoints >= PointSortMin )
+            {
+                void * p = nullptr;
+                st       = scratch_address( h.device, &p );
+                if ( st != SESDF_OK )
+                    return st;
+                sort_points_in( h, s, point_sort_space( p, SESDF_NOALLOC_MAX_POINTS ) );
+            }
+            sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+        }
+        return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
+    }
+
     sesdf_status sesdf_sample_points_ex( sesdf_handle * hh, const float * d_points, uint64_t npoints, float band, int flags, float * d_out,
                                          uint64_t * d_codes )
     {
