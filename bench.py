@@ -6,6 +6,7 @@ with 10 empty voids of radius U(4, 8) A, probe 1.4 A, complete surface. A step i
 replayed as one hipGraph. Further legs (same JSON line):
   exterior   the exterior path on the same atoms;
   sdf        SDF of the complete surface on a 256^3 grid over the bounding box + 3 A;
+  protein    both surfaces of 7CGO (335,722 atoms, the reference's largest sample present), from MMTF;
   large      exterior path on 2.5M atoms (500 globules of 5,000 on a 360 A Fibonacci sphere) + memory;
   batch      4,096 molecules (N ~ logU[20, 5000]) in one launch sequence + 32^3 SDF grid each, vs
              one launch sequence per molecule (timed on the first 512 molecules); also the builds
@@ -30,7 +31,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 STAGES = ["grid", "circles_kernel", "circles_offsets_emit", "intersections_kernel", "intersections_offsets_emit",
           "circle_list_scan", "fill", "segments", "probe_neighbors", "torus"]
-ALL_LEGS = "main,exterior,sdf,large,batch,grad,cpu"
+ALL_LEGS = "main,exterior,sdf,protein,large,batch,grad,cpu"
 
 
 def parse():
@@ -202,6 +203,25 @@ def leg_sdf(args, atoms, surf, out):
                   "inside_fraction": float((r < 0).mean()), "surface_band_fraction": float((np.abs(r) < 2).mean()),
                   "tuning": tune}
     log(f"sdf 256^3: {ms:.3f} ms")
+
+
+def leg_protein(args, out):
+    """The largest of the reference's sample proteins present (7CGO, 335,722 atoms), read from its
+    MMTF file with the reference's radii: both surfaces, hipGraph replays."""
+    from sesdf_amd import Sesdf, Ssesdf, load
+
+    atoms = load(os.path.join(ROOT, "tests", "golden", "mmtf", "7CGO.mmtf"))
+    res = {"protein": "7CGO", "atoms": int(atoms.shape[0])}
+    for name, cls in (("complete", Sesdf), ("exterior", Ssesdf)):
+        surf = cls(atoms, probe_radius=1.4, cell_size=args.cell, use_graph=True)
+        tune = tuned(surf, args)
+        ms = time_builds(surf, max(10, args.steps // 2), 3)
+        c = surf.counts()
+        res[name] = {"ms_per_build": ms, "atoms_per_s": atoms.shape[0] / (ms * 1e-3),
+                     "concave": int(c["concave_patch_nb"]), "tuning": tune}
+        log(f"7CGO {name}: {ms:.3f} ms/build")
+        surf.close()
+    out["protein_7CGO"] = res
 
 
 def leg_large(args, out):
@@ -415,6 +435,8 @@ def main():
         leg_sdf(args, atoms, surf, out)
     if surf is not None:
         surf.close()
+    if "protein" in legs:
+        leg_protein(args, out)
     if "large" in legs:
         leg_large(args, out)
     if "batch" in legs:

@@ -137,7 +137,8 @@ typedef struct
 
 /* bcs::Sesdf::Sesdf / bcs::Ssesdf::Ssesdf (sesdf/src/sesdf/sesdf.cu:20-75, sesdf/src/ssesdf/ssesdf.cu:19-66).
  * SESDF_ERR_INVALID, before any device work, for non-finite coordinates or radii and for a given
- * aabb that is not finite or has min > max. No atoms is valid (an empty surface). */
+ * aabb that is not finite or has min > max; SESDF_ERR_CAPACITY for 2^30 atoms or more. No atoms is
+ * valid (an empty surface). */
 sesdf_status sesdf_create( const sesdf_create_info * info, sesdf_handle ** out );
 /* bcs::Sesdf::~Sesdf (sesdf.cu:159-170). */
 sesdf_status sesdf_destroy( sesdf_handle * h );
@@ -234,6 +235,16 @@ sesdf_status sesdf_pack_graphics( sesdf_handle * h, void * const * d_buffers );
 
 /* Blocking device -> host copy (DeviceBuffer::toHost, sesdf/include/bcs/cuda/memory.cuh:155). */
 sesdf_status sesdf_device_to_host( void * host, const void * device, size_t bytes );
+
+/* -------- Molecule loading (replaces bcs::load, sesdf/src/core/loader.cpp:14-71, for MMTF files,
+ * with the radius table of sesdf/src/core/molecule.cpp:9-131) -------- */
+/* The atoms of the first model in file order as xyzr (host memory, 4 floats per atom, freed with
+ * sesdf_free_atoms); radius from the element, upper-cased (unknown elements 1.20 A).
+ * SESDF_ERR_INVALID with a message for unreadable, truncated or inconsistent files. */
+sesdf_status sesdf_load_mmtf( const char * path, float ** xyzr, uint64_t * atom_nb );
+void         sesdf_free_atoms( float * xyzr );
+/* van der Waals radius of an element symbol (any case), as getRadius (molecule.cpp:130). */
+float sesdf_element_radius( const char * element );
 
 /* Per-stage device times of the last timed build, milliseconds (app/detailed/sesdf_benchmark.cu:50-232:
  * Circle / Intersection / Segment / Neighborhood). stage_ms receives up to SESDF_STAGE_COUNT values:

@@ -2,6 +2,7 @@
 
 Public API mirrors the reference host classes bcs::Sesdf / bcs::Ssesdf; see ``surface.py``.
 """
+from .loader import element_radius, load  # noqa: F401
 from .surface import Sesdf, Ssesdf, get_aabb  # noqa: F401
 
-__all__ = ["Sesdf", "Ssesdf", "get_aabb"]
+__all__ = ["Sesdf", "Ssesdf", "get_aabb", "load", "element_radius"]

@@ -92,6 +92,9 @@ SIGNATURES = {
     "sesdf_sdf_backward": (i32, [vp, vp, vp, u64, f32, ctypes.c_int, vp]),
     "sesdf_sample_points_ex": (i32, [vp, vp, u64, f32, ctypes.c_int, vp, vp]),
     "sesdf_sample_points_noalloc": (i32, [vp, vp, u64, f32, ctypes.c_int, vp, vp]),
+    "sesdf_load_mmtf": (i32, [ctypes.c_char_p, ctypes.POINTER(ctypes.POINTER(ctypes.c_float)), ctypes.POINTER(u64)]),
+    "sesdf_free_atoms": (None, [ctypes.POINTER(ctypes.c_float)]),
+    "sesdf_element_radius": (f32, [ctypes.c_char_p]),
     "sesdf_sample_grid_ex": (i32, [vp, ctypes.POINTER(GridDesc), f32, ctypes.c_int, vp]),
     "sesdf_sdf_backward_ex": (i32, [vp, vp, vp, u64, f32, ctypes.c_int, vp, vp, vp]),
     "sesdf_surface_components": (i32, [vp, vp, vp]),

@@ -21,6 +21,7 @@ HIP_SOURCES = [
     "sdf.hip",
     "components.hip",
     "sort.hip",
+    "mmtf.cpp",
     "capi.cpp",
 ]
 HIP_HEADERS = ["common.h", "kernels.h", "handle.h", "components.h", "sort.h"]
@@ -79,7 +80,7 @@ def build_hip(jobs=4):
     return out
 
 
-CPP_PROGRAMS = ["host_api", "circles_census", "detailed_stages"]
+CPP_PROGRAMS = ["host_api", "circles_census", "detailed_stages", "load_molecule"]
 
 
 def build_host_api():

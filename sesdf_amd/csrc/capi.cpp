@@ -35,6 +35,9 @@ namespace
         g_lastError = what;
         return st;
     }
+
+    // Sorted atom indices share a 32-bit word with two flags in the sampling kernels.
+    constexpr uint64_t MaxAtoms = 1ull << 30;
 } // namespace
 
 sesdf_status sesdf::report_error( sesdf_status st, const char * what ) { return set_error( st, what ); }
@@ -473,6 +476,8 @@ extern "C"
         }
         if ( info->atom_nb && !info->xyzr )
             return SESDF_ERR_INVALID;
+        if ( info->atom_nb >= MaxAtoms )
+            return set_error( SESDF_ERR_CAPACITY, "at most 2^30 - 1 atoms per handle" );
         // input checks on the host, before any device work
         if ( !atoms_finite( info->xyzr, 0, info->atom_nb ) )
             return set_error( SESDF_ERR_INVALID, "atom coordinates and radii must be finite" );
@@ -540,6 +545,8 @@ extern "C"
                 return set_error( SESDF_ERR_INVALID, "atom_offsets must not decrease" );
         if ( info->atom_offsets[ info->molecule_nb ] && !info->xyzr )
             return SESDF_ERR_INVALID;
+        if ( info->atom_offsets[ info->molecule_nb ] >= MaxAtoms )
+            return set_error( SESDF_ERR_CAPACITY, "at most 2^30 - 1 atoms per handle" );
         if ( info->atom_offsets[ info->molecule_nb ] < 0xffffffffull && !atoms_finite( info->xyzr, 0, info->atom_offsets[ info->molecule_nb ] ) )
             return set_error( SESDF_ERR_INVALID, "atom coordinates and radii must be finite" );
         auto * h = new Handle();

@@ -39,6 +39,11 @@ namespace sesdf
     };
     __device__ __forceinline__ uint64_t code_of( uint32_t type, uint32_t idx ) { return ( uint64_t( type ) << 40 ) | idx; }
 
+    // Flags of a compacted surface atom (high bits of its sorted index).
+    constexpr uint32_t SurfOpen       = 1u << 31; // its face may be exposed
+    constexpr uint32_t SurfHiddenCaps = 1u << 30; // a hidden circle's cap is not inside one visible cap
+    constexpr uint32_t SurfAtom       = 0x3fffffffu;
+
     struct SdfArgs
     {
         const float4 *   points;     // n points (xyz_) or null: grid
@@ -59,7 +64,7 @@ namespace sesdf
         const float4 *   sAtoms;        // per sorted atom: centre, R = r + rp
         const float4 *   surfAtoms;     // surface atoms (compacted, cell order): centre, R
         const int4 *     surfRanges;    // per surface atom: (full circles, segments), (W vertices, N vertices)
-        const uint32_t * surfIds;       // per surface atom: sorted index | face-may-be-exposed << 31
+        const uint32_t * surfIds;       // per surface atom: sorted index | SurfOpen | SurfHiddenCaps
         const uint32_t * surfCellStart; // cellNb + 1 offsets into the surface atoms
         const int2 *     surfCaps;      // per surface atom: range of its visible circles' caps
         const float4 *   caps;          // per visible circle (sector): axis, cos(half angle)
@@ -84,6 +89,25 @@ namespace sesdf
         }
         const uint64_t ix = q % s.dx, iy = ( q / s.dx ) % s.dy, iz = q / ( uint64_t( s.dx ) * s.dy );
         return mk( ox + sx * float( ix ), oy + sy * float( iy ), oz + sz * float( iz ) );
+    }
+
+    // Point p of sphere a inside the SAS ball of a neighbour whose circle is hidden. Only atoms with
+    // a hidden cap that no single visible cap contains are asked: their face may hold a region
+    // that the visible caps leave open but hidden caps cover (a ring of visible caps around a hole
+    // inside one hidden cap).
+    __device__ __forceinline__ bool in_hidden_ball( const Buffers & b, uint32_t a, f3 p )
+    {
+        const uint32_t na = b.nbCount[ a ];
+        bool           in = false;
+        for ( uint32_t l = 0; l < na && !in; l++ )
+        {
+            if ( b.status[ size_t( a ) * b.K + l ] != 0 )
+                continue;
+            const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
+            const float  Rl = al.w + b.probeRadius;
+            in              = len2( p - xyz( al ) ) < Rl * Rl;
+        }
+        return in;
     }
 
     // In a wave-cooperative evaluation (COOP) all lanes walk the same cells and atoms, so a test
@@ -283,7 +307,7 @@ namespace sesdf
     {
         for ( uint32_t k = k0; elems && k < k1; k++ )
         {
-            const uint32_t a  = s.surfIds[ k ] & 0x7fffffffu;
+            const uint32_t a  = s.surfIds[ k ] & SurfAtom;
             const uint32_t st = s.exterior ? s.comp.extStatus[ a ] : 1u; // 2: element by element
             if ( st == 0 )
                 continue;
@@ -306,9 +330,9 @@ namespace sesdf
         }
         for ( uint32_t k = k0; faces && k < k1; k++ )
         {
-            const uint32_t id = s.surfIds[ k ], a = id & 0x7fffffffu;
+            const uint32_t id = s.surfIds[ k ], a = id & SurfAtom;
             const uint32_t st = s.exterior ? s.comp.extStatus[ a ] : 1u;
-            if ( !( id >> 31 ) || st == 0 )
+            if ( !( id & SurfOpen ) || st == 0 )
                 continue;
             const float4 aa   = s.surfAtoms[ k ];
             const f3     dq   = L.q - xyz( aa );
@@ -324,6 +348,8 @@ namespace sesdf
                 const float4 cap = s.caps[ c ]; // axis, cos(half angle)
                 exposed          = !( dot( u, xyz( cap ) ) > cap.w );
             }
+            if ( exposed && ( id & SurfHiddenCaps ) )
+                exposed = !in_hidden_ball( b, a, xyz( aa ) + u * aa.w );
             if ( exposed && st == 2 )
                 exposed = label_at( b, s.comp, a, u ) == extC;
             if ( exposed )
@@ -384,7 +410,7 @@ namespace sesdf
             const bool     ok     = k < k1;
             const float4   myAtom = ok ? s.surfAtoms[ k ] : make_float4( 0.f, 0.f, 0.f, 0.f );
             const uint32_t myId   = ok ? s.surfIds[ k ] : 0u;
-            const uint32_t mySt   = !ok ? 0u : s.exterior ? s.comp.extStatus[ myId & 0x7fffffffu ] : 1u;
+            const uint32_t mySt   = !ok ? 0u : s.exterior ? s.comp.extStatus[ myId & SurfAtom ] : 1u;
             const uint32_t myExt  = mySt != 0 ? 1u : 0u;
             const uint64_t chkMask = ballot( mySt == 2 ); // atoms whose elements are checked one by one
             const int      cnt    = int( min( uint32_t( WaveSize ), k1 - base ) );
@@ -467,7 +493,7 @@ namespace sesdf
                     cand             = uint64_t( uint32_t( __builtin_amdgcn_readfirstlane( int( uint32_t( m ) ) ) ) ) |
                            ( uint64_t( uint32_t( __builtin_amdgcn_readfirstlane( int( uint32_t( m >> 32 ) ) ) ) ) << 32 );
                 }
-                cand &= ballot( ok && ( myId >> 31 ) && myExt );
+                cand &= ballot( ok && ( myId & SurfOpen ) && myExt );
                 const int2 myCaps = s.surfCaps[ last ];
                 const int  c0 = rl( myCaps.x, 0 ), c1 = rl( myCaps.y, cnt - 1 );
                 const bool capIn  = c1 - c0 <= StageSize;
@@ -505,10 +531,12 @@ namespace sesdf
                                 exposed          = exposed && !( dot( u, xyz( cap ) ) > cap.w );
                             }
                     }
+                    if ( ( id & SurfHiddenCaps ) && any_lane<true>( exposed ) )
+                        exposed = exposed && !in_hidden_ball( b, id & SurfAtom, xyz( aa ) + u * aa.w );
                     if ( exposed && ( ( chkMask >> j ) & 1 ) )
-                        exposed = label_at( b, s.comp, id & 0x7fffffffu, u ) == extC;
+                        exposed = label_at( b, s.comp, id & SurfAtom, u ) == extC;
                     if ( exposed )
-                        L.best = fabsf( sd ), L.bc = code_of( SdfFace, id & 0x7fffffffu );
+                        L.best = fabsf( sd ), L.bc = code_of( SdfFace, id & SurfAtom );
                 }
             }
         }
@@ -661,10 +689,57 @@ namespace sesdf
         return sdf_eval_lanes<false>( b, s, mol, L, code, nullptr, nullptr );
     }
 
+    // Cap of circle (a, slot l) on sphere a in FP64 from the FP32 inputs: pole (unit, towards the
+    // neighbour), cos and sin of the half angle. False when the spheres meet in no circle.
+    __device__ __forceinline__ bool cap64( const Buffers & b, d3 ca, double Ra, uint32_t a, uint32_t l, d3 & pole, double & cs, double & sn )
+    {
+        const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
+        const double Rl = double( al.w ) + double( b.probeRadius );
+        const d3     v  = mkd( xyz( al ) ) - ca;
+        const double d2 = dot( v, v ), d = sqrt( d2 );
+        if ( !( d > fabs( Ra - Rl ) && d < Ra + Rl ) )
+            return false;
+        const double t = ( Ra * Ra - Rl * Rl + d2 ) / ( 2.0 * d );
+        pole           = v * ( 1.0 / d );
+        cs             = t / Ra;
+        sn             = sqrt( fmax( Ra * Ra - t * t, 0.0 ) ) / Ra;
+        return true;
+    }
+
+    // The cap of hidden circle (a, slot h) is not contained in any single visible cap of a (FP64:
+    // angle between the poles + its half angle <= the visible cap's half angle, by the cosine and
+    // sine of the sum). Almost every hidden cap lies inside one visible cap.
+    __device__ bool hidden_cap_open( const Buffers & b, uint32_t a, uint32_t h, d3 ca, double Ra )
+    {
+        d3     ph;
+        double ch, sh;
+        if ( !cap64( b, ca, Ra, a, h, ph, ch, sh ) )
+            return false;
+        const uint32_t na = b.nbCount[ a ];
+        for ( uint32_t l = 0; l < na; l++ )
+        {
+            if ( b.status[ size_t( a ) * b.K + l ] == 0 )
+                continue;
+            d3     pv;
+            double cv, sv;
+            if ( !cap64( b, ca, Ra, a, l, pv, cv, sv ) )
+                continue;
+            const double c = dot( ph, pv );
+            const d3     x = cross( ph, pv );
+            const double s = sqrt( dot( x, x ) );
+            // angle(ph, pv) + half angle of h, as cos and sin of the sum
+            const double cSum = c * ch - s * sh, sSum = s * ch + c * sh;
+            if ( sSum >= 0.0 && cSum >= cv + 1e-12 )
+                return false;
+        }
+        return true;
+    }
+
     // Per sorted atom: centre and R; surface flag (owns an arc or vertex, or its face may be
     // exposed) for the compaction scan. A face can be exposed only if the atom has a visible
     // circle, or no neighbour cuts a cap off its sphere: otherwise the exposed part would be
-    // bounded by arcs of visible circles.
+    // bounded by arcs of visible circles. info: face may be exposed (bit 0); a hidden cap lies in
+    // no single visible cap, so the face test also asks the hidden neighbours' balls (bit 1).
     // One wave per atom, lanes over its neighbours.
     __global__ void __launch_bounds__( 256 ) sdf_pack_kernel( Buffers b, float4 * pa, uint32_t * surfFlag, uint32_t * info )
     {
@@ -674,7 +749,7 @@ namespace sesdf
         const bool     live = a < b.n;
         // inside: the sphere lies in a neighbour's ball; cap: some neighbour robustly cuts a
         // non-empty cap (the caps of visible circles then bound any exposed part)
-        bool           inside = false, cap = false;
+        bool           inside = false, cap = false, hidden = false;
         float4         at     = make_float4( 0.f, 0.f, 0.f, 0.f );
         float          Ra     = 0.f;
         if ( live )
@@ -682,6 +757,8 @@ namespace sesdf
             at                = b.atoms[ a ];
             Ra                = at.w + b.probeRadius;
             const uint32_t na = b.nbCount[ a ];
+            const d3       ca = mkd( xyz( at ) );
+            const double   Rd = double( at.w ) + double( b.probeRadius );
             for ( uint32_t l = sub; l < na; l += 16 )
             {
                 const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
@@ -689,11 +766,14 @@ namespace sesdf
                 const float  d  = sqrtf( len2( xyz( al ) - xyz( at ) ) );
                 inside          = inside || d + Ra <= Rl;
                 cap             = cap || ( d < Ra + Rl - 1e-3f && d + Rl > Ra + 1e-3f );
+                if ( !hidden && b.status[ size_t( a ) * b.K + l ] == 0 )
+                    hidden = hidden_cap_open( b, a, l, ca, Rd );
             }
         }
         const uint64_t mine = uint64_t( 0xffff ) << ( 16 * grp );
         inside              = ( ballot( inside ) & mine ) != 0;
         cap                 = ( ballot( cap ) & mine ) != 0;
+        hidden              = ( ballot( hidden ) & mine ) != 0;
         if ( !live || sub != 0 )
             return;
         pa[ a ]          = make_float4( at.x, at.y, at.z, Ra );
@@ -701,7 +781,7 @@ namespace sesdf
                            b.wStart[ a + 1 ] > b.wStart[ a ] || b.nStart[ a + 1 ] > b.nStart[ a ];
         const bool open  = !inside && ( b.vStart[ a + 1 ] > b.vStart[ a ] || !cap );
         surfFlag[ a ]    = ( elems || open ) ? 1u : 0u;
-        info[ a ]        = open ? 1u : 0u;
+        info[ a ]        = ( open ? 1u : 0u ) | ( open && hidden ? 2u : 0u );
     }
 
     // Compacted surface atoms in cell order and their per-cell offsets.
@@ -718,7 +798,7 @@ namespace sesdf
         sr[ 2 * k ]      = make_int4( int( b.fullBase[ a ] ), int( b.fullBase[ a + 1 ] ), int( b.segStart[ b.trimBase[ a ] ] ),
                                       int( b.segStart[ b.trimBase[ a + 1 ] ] ) );
         sr[ 2 * k + 1 ]  = make_int4( int( b.wStart[ a ] ), int( b.wStart[ a + 1 ] ), int( b.nStart[ a ] ), int( b.nStart[ a + 1 ] ) );
-        sid[ k ]         = a | ( info[ a ] << 31 );
+        sid[ k ]         = a | ( ( info[ a ] & 1u ) ? SurfOpen : 0u ) | ( ( info[ a ] & 2u ) ? SurfHiddenCaps : 0u );
         sc[ k ]          = make_int2( int( b.vStart[ a ] ), int( b.vStart[ a + 1 ] ) );
     }
 
@@ -762,9 +842,9 @@ namespace sesdf
         for ( uint32_t k = xcd_block() * ( blockDim.x / WaveSize ) + uint32_t( wave_in_block() ); k < total; k += stride )
         {
             const uint32_t id = surfIds[ k ];
-            if ( !( id >> 31 ) )
+            if ( !( id & SurfOpen ) )
                 continue;
-            const uint32_t a  = id & 0x7fffffffu;
+            const uint32_t a  = id & SurfAtom;
             const uint32_t c0 = b.vStart[ a ], c1 = min( b.vStart[ a + 1 ], b.capSectors );
             const int      n  = int( c1 > c0 ? c1 - c0 : 0u );
             auto           key = []( float w ) { return w == w ? w : 3.0f; };
@@ -1121,6 +1201,11 @@ namespace
             d.geoSegments = h.cap.segments;
         }
         hipStream_t s = h.stream;
+        // sized by the capacities: the kernel reads the counts of the build from the device
+        const uint32_t geoItems = std::max( h.cap.sectors, h.cap.segments );
+        if ( geoItems )
+            sdf_geometry_kernel<<<std::min<uint32_t>( ( geoItems + 255 ) / 256, uint32_t( device_cu_count() ) * 8 ), 256, 0, s>>>(
+                h.b, d.geo, d.geo + d.geoSectors );
         if ( n )
             sdf_pack_kernel<<<( n + 15 ) / 16, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo );
         ScanArgs a {};
@@ -1134,11 +1219,6 @@ namespace
         const uint32_t items = std::max( n, h.cellNb + 1 );
         sdf_surface_kernel<<<( items + 255 ) / 256, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo, d.surfPrefix, d.surfAtoms,
                                                                  d.surfRanges, d.surfIds, d.surfCaps, d.surfCellStart );
-        // sized by the capacities: the kernel reads the counts of the build from the device
-        const uint32_t geoItems = std::max( h.cap.sectors, h.cap.segments );
-        if ( geoItems )
-            sdf_geometry_kernel<<<std::min<uint32_t>( ( geoItems + 255 ) / 256, uint32_t( device_cu_count() ) * 8 ), 256, 0, s>>>(
-                h.b, d.geo, d.geo + d.geoSectors );
         if ( n )
             sdf_sort_caps_kernel<<<std::min<uint32_t>( ( n + 3 ) / 4, uint32_t( device_cu_count() ) * 16 ), 256, 0, s>>>( h.b, d.geo, d.surfIds,
                                                                                                                       d.surfTotal );

@@ -405,3 +405,18 @@ def torus_values_within_fp32(gpu, orc64, orc32, atoms_in, slack=1e-4):
     e32 = np.where(np.isnan(r32) & np.isnan(r64), 0.0, np.abs(r32 - r64))
     bad = ~nan_ok | ~( eg <= 2 * np.nan_to_num(e32, nan=0.0) + slack )
     return int(len(g)), int(bad.any(axis=1).sum())
+
+
+def winner_atoms(codes, s2i, full_circles, segments, concave_ids):
+    """Input indices of the atoms that carry each winning element ([n, 3], -1 where none)."""
+    t, idx = codes >> 40, (codes & 0xFFFFFFFFFF).astype(np.int64)
+    out = np.full((len(codes), 3), -1, np.int64)
+    m = t == 1  # face of an atom
+    out[m, 0] = s2i[idx[m]]
+    m = t == 2  # full circle
+    out[m, :2] = s2i[full_circles.reshape(-1, 2)[idx[m]].astype(np.int64)]
+    m = t == 3  # arc of a segment: its circle
+    out[m, :2] = s2i[segments.reshape(-1, 4)[idx[m], :2].astype(np.int64)]
+    m = t == 4  # probe vertex
+    out[m] = s2i[concave_ids.reshape(-1, 4)[idx[m], :3].astype(np.int64)]
+    return out

@@ -60,21 +60,6 @@ def test_headline_build_matches_fp64(gpu, headline, mode):
     surf.close()
 
 
-def _winner_atoms(codes, s2i, full_circles, segments, concave_ids):
-    """Input indices of the atoms that carry each winning element ([n, 3], -1 where none)."""
-    t, idx = codes >> 40, (codes & 0xFFFFFFFFFF).astype(np.int64)
-    out = np.full((len(codes), 3), -1, np.int64)
-    m = t == 1  # face of an atom
-    out[m, 0] = s2i[idx[m]]
-    m = t == 2  # full circle
-    out[m, :2] = s2i[full_circles.reshape(-1, 2)[idx[m]].astype(np.int64)]
-    m = t == 3  # arc of a segment: its circle
-    out[m, :2] = s2i[segments.reshape(-1, 4)[idx[m], :2].astype(np.int64)]
-    m = t == 4  # probe vertex
-    out[m] = s2i[concave_ids.reshape(-1, 4)[idx[m], :3].astype(np.int64)]
-    return out
-
-
 @pytest.mark.parametrize("exterior", [False, True])
 def test_headline_sdf_grid_256(gpu, headline, exterior):
     """Within 1e-4 A of the FP64 oracle wherever neither side's winning element belongs to an atom
@@ -118,8 +103,8 @@ def test_headline_sdf_grid_256(gpu, headline, exterior):
     rep = PF.compare_fast(g, orc, atoms, 0)
     PF.assert_parity(rep)
     s2i_g, s2i_o = g["s2i"].astype(np.int64), orc["sorted_to_initial"].astype(np.int64)
-    wa_g = _winner_atoms(pc, s2i_g, g["full_circles"], g["segments"], g["concave_ids"])
-    wa_o = _winner_atoms(oc, s2i_o, orc["full_circles"], orc["segments"], orc["concave_ids"])
+    wa_g = PF.winner_atoms(pc, s2i_g, g["full_circles"], g["segments"], g["concave_ids"])
+    wa_o = PF.winner_atoms(oc, s2i_o, orc["full_circles"], orc["segments"], orc["concave_ids"])
     excused = (np.isin(wa_g, rep["band_atoms"]) | np.isin(wa_o, rep["band_atoms"])).any(axis=1)
     err = np.abs(pv - o)
     _say(f"max |GPU - FP64| {err[~excused].max():.2e} over {int((~excused).sum())} vertices; "
@@ -227,8 +212,8 @@ def test_large_complete_whole_system_and_exterior_sdf_at_1m_points(gpu, large):
     # of such an atom, and such points must be rare.
     band_atoms = np.asarray(rep["band_atoms"], np.int64)
     s2i_g, s2i_o = g["s2i"].astype(np.int64), orc["sorted_to_initial"].astype(np.int64)
-    wa_g = _winner_atoms(pc, s2i_g, g["full_circles"], g["segments"], g["concave_ids"])
-    wa_o = _winner_atoms(oc, s2i_o, orc["full_circles"], orc["segments"], orc["concave_ids"])
+    wa_g = PF.winner_atoms(pc, s2i_g, g["full_circles"], g["segments"], g["concave_ids"])
+    wa_o = PF.winner_atoms(oc, s2i_o, orc["full_circles"], orc["segments"], orc["concave_ids"])
     excused = (np.isin(wa_g, band_atoms) | np.isin(wa_o, band_atoms)).any(axis=1)
     err = np.abs(got - o)
     bad = (err > SDF_TOL) & ~excused
@@ -377,8 +362,8 @@ def test_differentiation_workload(gpu):
     g = parity.gpu_arrays(surf)
     rep = PF.compare_fast(g, orc, atoms, 0)
     PF.assert_parity(rep)
-    wa_g = _winner_atoms(c, g["s2i"].astype(np.int64), g["full_circles"], g["segments"], g["concave_ids"])
-    wa_o = _winner_atoms(oc, orc["sorted_to_initial"].astype(np.int64), orc["full_circles"], orc["segments"], orc["concave_ids"])
+    wa_g = PF.winner_atoms(c, g["s2i"].astype(np.int64), g["full_circles"], g["segments"], g["concave_ids"])
+    wa_o = PF.winner_atoms(oc, orc["sorted_to_initial"].astype(np.int64), orc["full_circles"], orc["segments"], orc["concave_ids"])
     excused = (np.isin(wa_g, rep["band_atoms"]) | np.isin(wa_o, rep["band_atoms"])).any(axis=1)
     verr = np.abs(v - o)[~excused]
     _say(f"values: max |GPU - FP64| {verr.max():.2e}, {int(excused.sum())} decided by band elements")
