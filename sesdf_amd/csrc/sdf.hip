@@ -286,6 +286,7 @@ namespace sesdf
     }
 
     // ---- per-lane evaluation (scattered points): each lane walks its own cells ----------------
+    template<bool EXT>
     __device__ __forceinline__ void lane_nearest( const SdfArgs & s, SdfLane & L, uint32_t a0, uint32_t a1, uint32_t extC )
     {
         for ( uint32_t a = a0; a < a1; a++ )
@@ -294,7 +295,7 @@ namespace sesdf
             const float  sd = fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w;
             if ( sd < L.g || ( sd == L.g && a < L.gA ) )
                 L.g = sd, L.gA = a;
-            if ( s.exterior && s.comp.extStatus[ a ] == 1 )
+            if ( EXT && s.comp.extStatus[ a ] == 1 )
                 L.gExt = fminf( L.gExt, sd );
         }
     }
@@ -302,13 +303,14 @@ namespace sesdf
     // Arcs and vertices of the surface atoms [k0, k1) (each lies on its atom's sphere, at distance
     // >= |sd| from q), then their faces: the radial projection of q is exposed unless it lies in
     // the cap of a visible circle (hidden circles' caps lie inside visible ones).
+    template<bool EXT>
     __device__ __forceinline__ void lane_row( const Buffers & b, const SdfArgs & s, SdfLane & L, uint32_t k0, uint32_t k1, uint32_t extC,
                                               uint32_t Iw, bool elems, bool faces )
     {
         for ( uint32_t k = k0; elems && k < k1; k++ )
         {
             const uint32_t a  = s.surfIds[ k ] & SurfAtom;
-            const uint32_t st = s.exterior ? s.comp.extStatus[ a ] : 1u; // 2: element by element
+            const uint32_t st = EXT ? s.comp.extStatus[ a ] : 1u; // 2: element by element
             if ( st == 0 )
                 continue;
             const float4 aa = s.surfAtoms[ k ];
@@ -331,7 +333,7 @@ namespace sesdf
         for ( uint32_t k = k0; faces && k < k1; k++ )
         {
             const uint32_t id = s.surfIds[ k ], a = id & SurfAtom;
-            const uint32_t st = s.exterior ? s.comp.extStatus[ a ] : 1u;
+            const uint32_t st = EXT ? s.comp.extStatus[ a ] : 1u;
             if ( !( id & SurfOpen ) || st == 0 )
                 continue;
             const float4 aa   = s.surfAtoms[ k ];
@@ -368,6 +370,7 @@ namespace sesdf
     constexpr int StageSize = 384; // float4 per wave: 64 segments (4 each) + 64 + 64 vertices, or caps
     constexpr int MaskRows  = 32;  // near-block rows whose element-gate masks are kept (2 chunks each)
 
+    template<bool EXT>
     __device__ __forceinline__ void coop_nearest( const SdfArgs & s, SdfLane & L, uint32_t a0, uint32_t a1, uint32_t extC )
     {
         const uint32_t lane = uint32_t( lane_id() );
@@ -376,7 +379,7 @@ namespace sesdf
             const uint32_t a    = base + lane;
             const bool     ok   = a < a1;
             const float4   mine = ok ? s.sAtoms[ a ] : make_float4( 0.f, 0.f, 0.f, 0.f );
-            const uint32_t ext  = ok && s.exterior && s.comp.extStatus[ a ] == 1 ? 1u : 0u;
+            const uint32_t ext  = ok && EXT && s.comp.extStatus[ a ] == 1 ? 1u : 0u;
             const int      cnt  = int( min( uint32_t( WaveSize ), a1 - base ) );
             for ( int j = 0; j < cnt; j++ )
             {
@@ -396,6 +399,7 @@ namespace sesdf
     // decreases), so faces are tested for the atoms of the chunk's element-gate mask: from the
     // element loop just run, or, for near-block rows (ord >= 0) whose elements ran in an earlier
     // pass, from the wave's mask store.
+    template<bool EXT>
     __device__ __forceinline__ void coop_row( const Buffers & b, const SdfArgs & s, SdfLane & L, bool need, uint32_t k0, uint32_t k1, uint32_t extC,
                                               uint32_t Iw, bool elems, bool faces, int ord, float4 * stage, uint64_t * masks,
                                               const LaneBox & box )
@@ -410,7 +414,7 @@ namespace sesdf
             const bool     ok     = k < k1;
             const float4   myAtom = ok ? s.surfAtoms[ k ] : make_float4( 0.f, 0.f, 0.f, 0.f );
             const uint32_t myId   = ok ? s.surfIds[ k ] : 0u;
-            const uint32_t mySt   = !ok ? 0u : s.exterior ? s.comp.extStatus[ myId & SurfAtom ] : 1u;
+            const uint32_t mySt   = !ok ? 0u : EXT ? s.comp.extStatus[ myId & SurfAtom ] : 1u;
             const uint32_t myExt  = mySt != 0 ? 1u : 0u;
             const uint64_t chkMask = ballot( mySt == 2 ); // atoms whose elements are checked one by one
             const int      cnt    = int( min( uint32_t( WaveSize ), k1 - base ) );
@@ -550,7 +554,7 @@ namespace sesdf
     //     near block arcs and vertices come first, so that the costlier face tests are pruned by
     //     their distance; further rows are nearly always culled or cheap.
     // A row segment is skipped when the distance bound of its box cannot change the outcome.
-    template<bool COOP>
+    template<bool COOP, bool EXT>
     __device__ float sdf_eval_lanes( const Buffers & b, const SdfArgs & s, uint32_t mol, SdfLane & L, uint64_t & code, float4 * stage,
                                      uint64_t * masks )
     {
@@ -559,7 +563,7 @@ namespace sesdf
         const float    D     = rp + s.band;
         const Molecule M     = b.mols[ mol ];
         const uint32_t Iw    = b.ctl->intersectionWithNbNb;
-        const uint32_t extC  = s.exterior ? s.comp.extComp[ 0 ] : 0u;
+        const uint32_t extC  = EXT ? s.comp.extComp[ 0 ] : 0u;
         const int      reach = max( int( ceilf( ( D + Rmax + 1e-3f ) / M.cell ) ), 1 );
         int            gx, gy, gz;
         cell_of( M, L.q.x, L.q.y, L.q.z, gx, gy, gz );
@@ -594,21 +598,21 @@ namespace sesdf
             if ( !any_lane<COOP>( L.active && L.g >= 0.f ) )
                 return; // every point is inside: g no longer matters
             const float lb   = lower( xa, xb, dyz2 );
-            const float gCut = s.exterior ? L.g + 1e-4f : fminf( L.g, fmaxf( s.band - rp, 0.f ) );
+            const float gCut = EXT ? L.g + 1e-4f : fminf( L.g, fmaxf( s.band - rp, 0.f ) );
             if ( !any_lane<COOP>( L.active && L.g >= 0.f && lb <= gCut ) )
                 return;
             const uint32_t h0 = cell_index( M, xa, y, z );
             const uint32_t a0 = b.cellStart[ h0 ], a1 = b.cellStart[ h0 + uint32_t( xb - xa ) + 1 ];
             if constexpr ( COOP )
-                coop_nearest( s, L, a0, a1, extC );
+                coop_nearest<EXT>( s, L, a0, a1, extC );
             else
-                lane_nearest( s, L, a0, a1, extC );
+                lane_nearest<EXT>( s, L, a0, a1, extC );
         };
         for_block( t, M, L.q, nearest );
         {
             // the rest of the stencil, narrowed to what the bounds found in the near block allow
             const bool  needA = L.active && L.g >= 0.f;
-            const float gCut  = s.exterior ? L.g + 1e-4f : fminf( L.g, fmaxf( s.band - rp, 0.f ) );
+            const float gCut  = EXT ? L.g + 1e-4f : fminf( L.g, fmaxf( s.band - rp, 0.f ) );
             for_rest( narrow<COOP>( t, M, L, needA, gCut + Rmax + 1e-3f ), M, L.q, nearest );
         }
 
@@ -618,15 +622,15 @@ namespace sesdf
         {
             // farther than rp + band from A: +band in the unbounded region, -band in a cavity (pass B
             // then finds no element within the band)
-            if ( s.exterior && L.active && far_point_in_cavity( b, s.comp, L.q, D, extC ) )
+            if ( EXT && L.active && far_point_in_cavity( b, s.comp, L.q, D, extC ) )
                 state = 2;
         }
         else
         {
             // outside A the point lies in the region of the complement bounded by the face that holds
             // its projection onto the nearest sphere; ties within 1e-4 A count as exterior
-            bool outside = L.g >= 0.f && ( !s.exterior || L.gExt <= L.g + 1e-4f );
-            if ( s.exterior && L.g >= 0.f && !outside && s.comp.extStatus[ L.gA ] == 2 )
+            bool outside = L.g >= 0.f && ( !EXT || L.gExt <= L.g + 1e-4f );
+            if ( EXT && L.g >= 0.f && !outside && s.comp.extStatus[ L.gA ] == 2 )
             {
                 const f3    dq = L.q - xyz( s.sAtoms[ L.gA ] );
                 const float dl = fsqrt( len2( dq ) );
@@ -656,9 +660,9 @@ namespace sesdf
                     if ( !elems && ord >= 0 && ord < MaskRows && k1 - k0 <= 2 * WaveSize && ( masks[ 2 * ord ] | masks[ 2 * ord + 1 ] ) == 0 )
                         return;
                 if constexpr ( COOP )
-                    coop_row( b, s, L, needB, k0, k1, extC, Iw, elems, faces, ord, stage, masks, box );
+                    coop_row<EXT>( b, s, L, needB, k0, k1, extC, Iw, elems, faces, ord, stage, masks, box );
                 else
-                    lane_row( b, s, L, k0, k1, extC, Iw, elems, faces );
+                    lane_row<EXT>( b, s, L, k0, k1, extC, Iw, elems, faces );
             };
             for_block( t, M, L.q, [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, true, false ); } );
             for_block( t, M, L.q, [ & ]( int xa, int xb, int y, int z, int ord, float dyz2 ) { surface( xa, xb, y, z, ord, dyz2, false, true ); } );
@@ -681,12 +685,13 @@ namespace sesdf
         return v;
     }
 
+    template<bool EXT>
     __device__ float sdf_eval( const Buffers & b, const SdfArgs & s, f3 q, uint32_t mol, uint64_t & code )
     {
         SdfLane L;
         L.q      = q;
         L.active = true;
-        return sdf_eval_lanes<false>( b, s, mol, L, code, nullptr, nullptr );
+        return sdf_eval_lanes<false, EXT>( b, s, mol, L, code, nullptr, nullptr );
     }
 
     // Cap of circle (a, slot l) on sphere a in FP64 from the FP32 inputs: pole (unit, towards the
@@ -879,6 +884,7 @@ namespace sesdf
 
     // Scattered points: a wave whose points fall in a compact block of cells is evaluated
     // cooperatively, any other wave lane by lane.
+    template<bool EXT>
     __global__ void __launch_bounds__( 256, 5 ) sdf_points_kernel( Buffers b, SdfArgs s ) // five waves per SIMD, as the grids
     {
         __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
@@ -903,9 +909,9 @@ namespace sesdf
             if ( !any_lane<true>( L.active ) )
                 continue;
             if ( ex <= 2 && ey <= 2 && ez <= 2 )
-                v = sdf_eval_lanes<true>( b, s, 0, L, code, stage[ threadIdx.x / WaveSize ], masks[ threadIdx.x / WaveSize ] );
+                v = sdf_eval_lanes<true, EXT>( b, s, 0, L, code, stage[ threadIdx.x / WaveSize ], masks[ threadIdx.x / WaveSize ] );
             else if ( L.active )
-                v = sdf_eval_lanes<false>( b, s, 0, L, code, nullptr, nullptr );
+                v = sdf_eval_lanes<false, EXT>( b, s, 0, L, code, nullptr, nullptr );
             if ( L.active )
             {
                 const uint64_t dst = s.perm ? uint64_t( s.perm[ q ] ) : q;
@@ -940,6 +946,7 @@ namespace sesdf
     // Regular grids: one wave per 4 x 4 x 4 brick of grid vertices (batches: per molecule grid).
     // Latency-bound: five waves per SIMD (96 VGPRs, a few values spilled to scratch outside the
     // inner loops) run about 7% faster than four (128 VGPRs); six spill into the loops.
+    template<bool EXT>
     __global__ void __launch_bounds__( 256, 5 ) sdf_grid_kernel( Buffers b, SdfArgs s, uint32_t molNb )
     {
         __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
@@ -962,7 +969,7 @@ namespace sesdf
             uint32_t       m;
             L.q = query_point( s, L.active ? q : uint64_t( mol ) * per, m );
             uint64_t    code;
-            const float v = sdf_eval_lanes<true>( b, s, mol, L, code, stage[ threadIdx.x / WaveSize ], masks[ threadIdx.x / WaveSize ] );
+            const float v = sdf_eval_lanes<true, EXT>( b, s, mol, L, code, stage[ threadIdx.x / WaveSize ], masks[ threadIdx.x / WaveSize ] );
             if ( L.active )
             {
                 s.out[ q ] = v;
@@ -1048,6 +1055,7 @@ namespace sesdf
             gf[ a ] = 0ull;
     }
 
+    template<bool EXT>
     __global__ void __launch_bounds__( 256 ) sdf_backward_kernel( Buffers b, SdfArgs s )
     {
         const double rp    = b.probeRadius;
@@ -1063,7 +1071,7 @@ namespace sesdf
             if ( !s.recompute )
                 code = s.codes[ q ]; // stored by the forward pass
             else
-                (void)sdf_eval( b, s, qf, mol, code );
+                (void)sdf_eval<EXT>( b, s, qf, mol, code );
             const uint32_t type = uint32_t( code >> 40 ), idx = uint32_t( code & 0xffffffffffull );
             if ( type == SdfClamped )
                 continue;
@@ -1428,7 +1436,10 @@ extern "C"
                     return st;
                 sort_points_in( h, s, point_sort_space( p, SESDF_NOALLOC_MAX_POINTS ) );
             }
-            sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+            if ( s.exterior )
+                sdf_points_kernel<true><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+            else
+                sdf_points_kernel<false><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
         }
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
@@ -1458,7 +1469,10 @@ extern "C"
                 if ( ps != SESDF_OK )
                     return ps;
             }
-            sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+            if ( s.exterior )
+                sdf_points_kernel<true><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+            else
+                sdf_points_kernel<false><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
         }
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
@@ -1509,8 +1523,10 @@ extern "C"
             return st;
         if ( s.empty )
             sdf_fill_empty_kernel<<<grid_blocks( s.n ), 256, 0, h.stream>>>( d_out, nullptr, s.n, band );
+        else if ( s.exterior )
+            sdf_grid_kernel<true><<<brick_blocks( s, 1 ), 256, 0, h.stream>>>( h.b, s, 1u );
         else
-            sdf_grid_kernel<<<brick_blocks( s, 1 ), 256, 0, h.stream>>>( h.b, s, 1u );
+            sdf_grid_kernel<false><<<brick_blocks( s, 1 ), 256, 0, h.stream>>>( h.b, s, 1u );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -1557,8 +1573,10 @@ extern "C"
             return st;
         if ( s.empty )
             sdf_fill_empty_kernel<<<grid_blocks( s.n ), 256, 0, h.stream>>>( d_out, nullptr, s.n, band );
+        else if ( s.exterior )
+            sdf_grid_kernel<true><<<brick_blocks( s, uint32_t( M ) ), 256, 0, h.stream>>>( h.b, s, uint32_t( M ) );
         else
-            sdf_grid_kernel<<<brick_blocks( s, uint32_t( M ) ), 256, 0, h.stream>>>( h.b, s, uint32_t( M ) );
+            sdf_grid_kernel<false><<<brick_blocks( s, uint32_t( M ) ), 256, 0, h.stream>>>( h.b, s, uint32_t( M ) );
         return hipGetLastError() == hipSuccess ? SESDF_OK : SESDF_ERR_HIP;
     }
 
@@ -1609,7 +1627,10 @@ extern "C"
             sesdf_status ps = sort_points( h, f );
             if ( ps != SESDF_OK )
                 return ps;
-            sdf_points_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, f );
+            if ( f.exterior )
+                sdf_points_kernel<true><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, f );
+            else
+                sdf_points_kernel<false><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, f );
             s.codes     = d.ptCodes;
             s.recompute = 0;
         }
@@ -1639,7 +1660,10 @@ extern "C"
         if ( npoints )
         {
             grad_maxw_kernel<<<std::min<uint32_t>( grid_blocks( npoints ), wide ), 256, 0, h.stream>>>( d_weights, npoints, s.gradFixed, h.n );
-            sdf_backward_kernel<<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+            if ( s.exterior )
+                sdf_backward_kernel<true><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
+            else
+                sdf_backward_kernel<false><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
         }
         if ( h.n )
             gather_grad_kernel<<<( h.n + 255 ) / 256, 256, 0, h.stream>>>( s.gradFixed, h.b.sortedToInitial, h.n, s.gradSorted,
