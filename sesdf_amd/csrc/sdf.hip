@@ -285,6 +285,17 @@ namespace sesdf
             L.best = d, L.bc = code_of( SdfVertex, v );
     }
 
+    // A face of sphere a can hold the nearest boundary point of a point q inside A only if q lies
+    // inside ball a: were q outside it, the radial segment from q to the face point would leave
+    // every ball through a boundary point nearer than the face point (the face point is outside
+    // all other balls). Points of the exterior variant outside A (in cavities) may reach faces
+    // from outside.
+    template<bool EXT>
+    __device__ __forceinline__ bool face_side( const SdfLane & L, float sd )
+    {
+        return sd < 0.f || ( EXT && L.g >= 0.f );
+    }
+
     // ---- per-lane evaluation (scattered points): each lane walks its own cells ----------------
     template<bool EXT>
     __device__ __forceinline__ void lane_nearest( const SdfArgs & s, SdfLane & L, uint32_t a0, uint32_t a1, uint32_t extC )
@@ -340,7 +351,7 @@ namespace sesdf
             const f3     dq   = L.q - xyz( aa );
             const float  dist = fsqrt( len2( dq ) );
             const float  sd   = dist - aa.w;
-            if ( !( fabsf( sd ) < L.best ) || !( dist > 0.f ) )
+            if ( !( fabsf( sd ) < L.best ) || !face_side<EXT>( L, sd ) || !( dist > 0.f ) )
                 continue;
             const f3   u       = dq * frcp( dist );
             const int2 cr      = s.surfCaps[ k ];
@@ -511,7 +522,7 @@ namespace sesdf
                     const f3       dq      = L.q - xyz( aa );
                     const float    dist    = fsqrt( len2( dq ) );
                     const float    sd      = dist - aa.w;
-                    bool           exposed = need && fabsf( sd ) < L.best && dist > 0.f;
+                    bool           exposed = need && fabsf( sd ) < L.best && face_side<EXT>( L, sd ) && dist > 0.f;
                     if ( !any_lane<true>( exposed ) )
                         continue;
                     if ( capIn && !staged )
