@@ -13,7 +13,9 @@ replayed as one hipGraph. Further legs (same JSON line):
              alone both ways;
   grad       20,000-atom globule, 1M queries within |SDF| <= 2 A, loss = sum w_i SDF(q_i), w ~ N(0, 1):
              forward (build + sampling) vs backward (stored / recomputed winners);
-  cpu        host C++ build of the same stage sequence (the FP32 oracle restatement), box's cores.
+  cpu        host C++ build of the same stage sequence (the FP32 oracle restatement) on this machine's
+             cores for the workloads above (oracle/cpu_baseline.py); its predicate counts give the
+             compute side of the roofline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs main,exterior,...]
 """
@@ -72,6 +74,49 @@ def time_builds(surf, steps, warmup):
     return e0.elapsed_ms(e1) / steps
 
 
+def replay_distribution(surf, reps):
+    """Device time of each of `reps` graph replays, one hipEvent pair around each: median and spread."""
+    import sesdf_amd._lib as L
+
+    lib = L.lib()
+    stream = surf.stream()
+    ev = [(L.Event(), L.Event()) for _ in range(reps)]
+    for e0, e1 in ev:
+        e0.record(stream)
+        L.check(lib.sesdf_enqueue(surf._h), "enqueue")
+        e1.record(stream)
+    ev[-1][1].sync()
+    L.check(lib.sesdf_sync(surf._h), "sync")
+    t = np.array([e0.elapsed_ms(e1) for e0, e1 in ev])
+    return {"n": reps, "median": float(np.median(t)), "p10": float(np.percentile(t, 10)),
+            "p90": float(np.percentile(t, 90)), "min": float(t.min()), "max": float(t.max())}
+
+
+def host_input_rate(surf, atoms, reps):
+    """The reference's build() as a caller sees it: atoms from host memory (upload), the whole
+    build, and the counts read back (sesdf_set_atoms + sesdf_build, blocking), host clock."""
+    import ctypes
+
+    import sesdf_amd._lib as L
+
+    lib = L.lib()
+    a = np.ascontiguousarray(atoms, np.float32)
+    ptr = ctypes.c_void_p(a.ctypes.data)
+    for _ in range(3):
+        L.check(lib.sesdf_set_atoms(surf._h, ptr), "set_atoms")
+        L.check(lib.sesdf_build(surf._h), "build")
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        L.check(lib.sesdf_set_atoms(surf._h, ptr), "set_atoms")
+        L.check(lib.sesdf_build(surf._h), "build")
+        t.append((time.perf_counter() - t0) * 1e3)
+    t = np.array(t)
+    med = float(np.median(t))
+    return {"n": reps, "median_ms": med, "p10_ms": float(np.percentile(t, 10)), "p90_ms": float(np.percentile(t, 90)),
+            "atoms_per_s": len(atoms) / (med * 1e-3), "input_bytes": int(a.nbytes)}
+
+
 def stage_times(surf, reps=10):
     """Per-stage device times (ms): event pairs around each stage of a directly launched build."""
     import ctypes
@@ -114,6 +159,33 @@ def circles_alg_bytes(c, n, K=128):
     return 16 * n + 5 * c["neighbor_nb"] + 4 * n + 16 * c["intersected_circle_nb"] + (K // 8) * n + 16 * n
 
 
+# Flops per predicate evaluation (F_alg = the oracle's counts in the reference's loop order, early
+# exits included, times these): a circle (i, j) against a sphere k, the reference's test
+# (operations.cu:126-204); a candidate triplet (two circle frames, the probe pair solve); one
+# occluder against one of the two probe positions (squared distance against R^2).
+FLOPS_VISIBILITY, FLOPS_TRIPLET, FLOPS_OCCLUDER = 30, 60, 10
+FP32_PEAK_TFLOPS = 157.3
+
+
+def compute_roofline(counts, main_out):
+    """Compute side of the two O(n^2)-per-atom kernels: F_alg from the predicate counts, achieved
+    TFLOP/s against the FP32 vector peak, and max(B_alg / 8 TB/s, F_alg / P_FP32) / t."""
+    k = main_out["roofline_kernels"]
+    flops = {"circles_count_kernel": counts["visibility_tests"] * FLOPS_VISIBILITY,
+             "intersections_count_kernel": counts["triplet_tests"] * FLOPS_TRIPLET + counts["occluder_tests"] * FLOPS_OCCLUDER}
+    res = {}
+    for name, f in flops.items():
+        t = k[name]["kernel_ms"] * 1e-3
+        b = k[name]["alg_bytes_per_launch"]
+        bound_s = max(b / (HBM_PEAK_GBS * 1e9), f / (FP32_PEAK_TFLOPS * 1e12))
+        res[name] = {"flops_per_launch": int(f), "achieved_tflops": f / t / 1e12, "peak_tflops": FP32_PEAK_TFLOPS,
+                     "frac_fp32": f / t / 1e12 / FP32_PEAK_TFLOPS, "frac_roofline": bound_s / t,
+                     "binding": "fp32" if f / (FP32_PEAK_TFLOPS * 1e12) > b / (HBM_PEAK_GBS * 1e9) else "hbm"}
+    res["predicate_counts"] = counts
+    res["flops_per_test"] = {"visibility": FLOPS_VISIBILITY, "triplet": FLOPS_TRIPLET, "occluder": FLOPS_OCCLUDER}
+    return res
+
+
 def kernel_roofline(kernel, alg, kms):
     achieved = alg / (kms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -140,6 +212,8 @@ def leg_main(args, atoms, out):
     ms = time_builds(surf, args.steps, args.warmup)
     counts = surf.counts()
     stages = stage_times(surf)
+    replays = replay_distribution(surf, 120)
+    host = host_input_rate(surf, atoms, 100)
     # both O(n^2)-per-atom kernels; the headline roofline is the longer one's (the dominant kernel)
     lines = {
         "circles_count_kernel": kernel_roofline("circles_count_kernel", circles_alg_bytes(counts, n), stages["circles_kernel"]),
@@ -152,9 +226,12 @@ def leg_main(args, atoms, out):
         "counts": {k: int(v) for k, v in counts.items()},
         "roofline": lines[dominant],
         "roofline_kernels": lines,
+        "replay_ms": replays,
+        "host_input": host,
         "tuning": tune,
     })
-    log(f"main: {ms:.3f} ms/build, {n / (ms * 1e-3) / 1e6:.1f} M atoms/s")
+    log(f"main: {ms:.3f} ms/build, {n / (ms * 1e-3) / 1e6:.1f} M atoms/s; median of {replays['n']} replays "
+        f"{replays['median']:.3f} ms; from host memory {host['median_ms']:.3f} ms")
     return surf
 
 
@@ -262,17 +339,18 @@ def leg_batch(args, out):
     tune = autotune(batch, log=log, after=lambda: batch.sample_grids_ptr(dim, grids.data_ptr(), margin=margin, band=band),
                     tag=f"+grids{dim}")
     ms_build = time_builds(batch, max(5, args.steps // 5), 2)
-    stream = batch.stream()
     batch.sample_grids_ptr(dim, grids.data_ptr(), margin=margin, band=band)
     L.device_synchronize()
-    e0, e1 = L.Event(), L.Event()
-    e0.record(stream)
-    L.check(L.lib().sesdf_enqueue(batch._h), "enqueue")
-    batch.sample_grids_ptr(dim, grids.data_ptr(), margin=margin, band=band)
-    e1.record(stream)
-    e1.sync()
-    L.check(L.lib().sesdf_sync(batch._h), "sync")
-    ms_batched = e0.elapsed_ms(e1)
+    # both ways on the host clock, from an idle device to an idle device: median of three
+    runs_batched = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        L.check(L.lib().sesdf_enqueue(batch._h), "enqueue")
+        batch.sample_grids_ptr(dim, grids.data_ptr(), margin=margin, band=band)
+        L.device_synchronize()
+        runs_batched.append((time.perf_counter() - t0) * 1e3)
+        L.check(L.lib().sesdf_sync(batch._h), "sync")
+    ms_batched = float(np.median(runs_batched))
     # one launch sequence per molecule (own handle, own graph), same work, first P molecules
     P = 512
     singles = []
@@ -290,12 +368,15 @@ def leg_batch(args, out):
         hi = (mol[:, :3] + mol[:, 3:4]).max(0) + margin
         descs.append((lo.astype(np.float32), ((hi - lo) / (dim - 1)).astype(np.float32)))
     L.device_synchronize()
-    t0 = time.perf_counter()
-    for m, s in enumerate(singles):
-        L.check(L.lib().sesdf_enqueue(s._h), "enqueue")
-        s.sample_grid_ptr(descs[m][0], descs[m][1], (dim,) * 3, outs.data_ptr() + 4 * m * dim ** 3, band=band)
-    L.device_synchronize()
-    ms_single = (time.perf_counter() - t0) * 1e3
+    runs_single = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for m, s in enumerate(singles):
+            L.check(L.lib().sesdf_enqueue(s._h), "enqueue")
+            s.sample_grid_ptr(descs[m][0], descs[m][1], (dim,) * 3, outs.data_ptr() + 4 * m * dim ** 3, band=band)
+        L.device_synchronize()
+        runs_single.append((time.perf_counter() - t0) * 1e3)
+    ms_single = float(np.median(runs_single))
     # the builds alone, both ways (no SDF grids)
     t0 = time.perf_counter()
     for s in singles:
@@ -307,7 +388,8 @@ def leg_batch(args, out):
     batched_rate = M / (ms_batched * 1e-3)
     single_rate = P / (ms_single * 1e-3)
     out["batch"] = {"molecules": M, "atoms": int(atoms.shape[0]), "sdf_grid": f"{dim}^3",
-                    "ms_batched_build_plus_sdf": ms_batched, "ms_batched_build": ms_build,
+                    "ms_batched_build_plus_sdf": ms_batched, "runs_ms_batched": runs_batched, "ms_batched_build": ms_build,
+                    "runs_ms_per_molecule_launches": runs_single, "clock": "host, device idle to idle, median of 3",
                     "molecules_per_s_batched": batched_rate,
                     "per_molecule_launch_molecules": P, "ms_per_molecule_launches": ms_single,
                     "molecules_per_s_per_launch": single_rate, "speedup": batched_rate / single_rate,
@@ -384,23 +466,51 @@ def leg_grad(args, out):
     log(f"grad: fwd {t_f:.3f} ms, bwd stored {t_bs:.3f} ms, recompute {t_br:.3f} ms")
 
 
-def leg_cpu(atoms, budget_s=20.0):
-    """Host C++ build of the same stage sequence (the FP32 oracle restatement) on this box's cores."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+def cpu_model():
+    try:
+        import subprocess
 
-    threads = min(os.cpu_count() or 1, 16)
-    t0 = time.time()
-    O.build(atoms, mode=0, precision=32, track=False, threads=threads)  # warm-up
-    times = []
-    while time.time() - t0 < budget_s and len(times) < 5:
-        t = time.time()
-        O.build(atoms, mode=0, precision=32, track=False, threads=threads)
-        times.append(time.time() - t)
-    med = float(np.median(times))
-    return {"value": atoms.shape[0] / med, "unit": "atoms/s", "cores": threads, "kind": "port",
-            "sample": f"{len(times)} full complete-surface builds of the {atoms.shape[0]}-atom workload, FP32, "
-                      f"{threads} threads, median {med:.3f} s"}
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def leg_cpu():
+    """Host C++ build of the same stage sequence (the oracle's FP32 restatement, threaded over atoms)
+    on this machine's cores, one child process per workload (oracle/cpu_baseline.py)."""
+    import subprocess
+
+    hw = os.cpu_count() or 1
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = hw
+    # at most 16 threads: the share of the host that one GPU's jobs may use on these machines
+    threads = max(1, min(allowed, 16))
+    res = {}
+    for w in ("c0", "c1", "c1x", "c2", "c3"):
+        t0 = time.perf_counter()
+        p = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), w, str(threads)],
+                           capture_output=True, text=True, timeout=900)
+        if p.returncode != 0:
+            raise RuntimeError(f"cpu baseline {w} failed: {p.stderr[-2000:]}")
+        res[w] = json.loads(p.stdout.strip().splitlines()[-1])
+        log(f"cpu {w}: {time.perf_counter() - t0:.1f} s wall")
+    c1 = res["c1"]
+    return {"value": c1["atoms_per_s"], "unit": "atoms/s", "cores": threads, "kind": "port",
+            "sample": f"complete-surface builds of the 100,000-atom workload, FP32, {threads} threads, median of "
+                      f"{len(c1['runs_s'])} ({c1['build_s']:.3f} s) after one warm-up; the other workloads below",
+            "hardware_concurrency": hw, "affinity_cpus": allowed, "cpu_model": cpu_model(),
+            "workloads": res}
 
 
 def main():
@@ -443,7 +553,9 @@ def main():
         leg_batch(args, out)
     if "grad" in legs:
         leg_grad(args, out)
-    out["cpu_baseline"] = leg_cpu(atoms) if "cpu" in legs else None
+    out["cpu_baseline"] = leg_cpu() if "cpu" in legs else None
+    if out["cpu_baseline"] and "roofline_kernels" in out:
+        out["roofline_compute"] = compute_roofline(out["cpu_baseline"]["workloads"]["c1"]["predicate_counts"], out)
     print(json.dumps(out), flush=True)
 
 
