@@ -12,7 +12,8 @@ replayed as one hipGraph. Further legs (same JSON line):
              one launch sequence per molecule (timed on the first 512 molecules); also the builds
              alone both ways;
   grad       20,000-atom globule, 1M queries within |SDF| <= 2 A, loss = sum w_i SDF(q_i), w ~ N(0, 1):
-             forward (build + sampling) vs backward (stored / recomputed winners);
+             forward (build + sampling) vs backward (stored / recomputed winners; or the build
+             released after the forward pass and built again in the backward pass);
   cpu        host C++ build of the same stage sequence (the FP32 oracle restatement) on this machine's
              cores for the workloads above (oracle/cpu_baseline.py); its predicate counts give the
              compute side of the roofline.
@@ -459,11 +460,42 @@ def leg_grad(args, out):
     t_f = timed(fwd)
     t_bs = timed(lambda: bwd(True))
     t_br = timed(lambda: bwd(False))
+
+    # release: the forward frees the whole build, the backward builds it again (allocation
+    # included), on the host clock from an idle device to an idle device
+    def wall(fn, reps=5):
+        times = []
+        for _ in range(reps + 1):
+            L.device_synchronize()
+            t0 = time.perf_counter()
+            fn()
+            L.device_synchronize()
+            times.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(times[1:]))
+
+    fwd_wall = wall(fwd)
+    held = surf.counts()["device_bytes"]
+    fwd_release = wall(lambda: (fwd(), surf.release()))
+    bwd_release = []
+    for _ in range(5):
+        surf.release()
+        L.device_synchronize()
+        t0 = time.perf_counter()
+        surf.build_device(d_atoms.data_ptr())
+        bwd(False)
+        L.device_synchronize()
+        bwd_release.append((time.perf_counter() - t0) * 1e3)
+    t_brel = float(np.median(bwd_release))
     out["grad"] = {"atoms": len(atoms), "queries": nq, "band_A": band, "ms_forward_build_plus_sdf": t_f,
                    "ms_backward_stored": t_bs, "ms_backward_recompute": t_br,
                    "bwd_over_fwd_stored": t_bs / t_f, "bwd_over_fwd_recompute": t_br / t_f,
-                   "stored_winner_bytes": nq * 8}
-    log(f"grad: fwd {t_f:.3f} ms, bwd stored {t_bs:.3f} ms, recompute {t_br:.3f} ms")
+                   "stored_winner_bytes": nq * 8,
+                   "release": {"clock": "host, device idle to idle, median of 5", "ms_forward": fwd_wall,
+                               "ms_forward_then_release": fwd_release, "ms_backward_rebuild_recompute": t_brel,
+                               "bwd_over_fwd": t_brel / fwd_wall, "device_bytes_held_between": 0,
+                               "device_bytes_of_the_build": int(held)}}
+    log(f"grad: fwd {t_f:.3f} ms, bwd stored {t_bs:.3f} ms, recompute {t_br:.3f} ms; release: fwd {fwd_wall:.3f} ms, "
+        f"bwd with rebuild {t_brel:.3f} ms")
 
 
 def cpu_model():

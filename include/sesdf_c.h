@@ -147,6 +147,11 @@ sesdf_status sesdf_destroy( sesdf_handle * h );
  * atoms given at create (or by sesdf_set_atoms) and runs the stage sequence. Blocks until counts
  * are on the host (one readback per build). */
 sesdf_status sesdf_build( sesdf_handle * h );
+/* Frees every device buffer of the handle (the build's neighbour lists, circles, patches, segments
+ * and the sampling state) and keeps its configuration; the next build, upload or enqueue
+ * allocates them again, and getters return SESDF_ERR_NOT_BUILT until then. No counterpart in the
+ * reference: it serves a backward pass that rebuilds the topology instead of holding it. */
+sesdf_status sesdf_release( sesdf_handle * h );
 /* Same stage sequence on atoms already resident in device memory (float4, atom_nb of them). When
  * d_xyzr is the handle's own input buffer (sesdf_input_buffer) nothing is copied. */
 sesdf_status sesdf_build_device( sesdf_handle * h, const float * d_xyzr );

@@ -75,6 +75,7 @@ SIGNATURES = {
     "sesdf_create": (i32, [ctypes.POINTER(CreateInfo), ctypes.POINTER(vp)]),
     "sesdf_destroy": (i32, [vp]),
     "sesdf_build": (i32, [vp]),
+    "sesdf_release": (i32, [vp]),
     "sesdf_build_device": (i32, [vp, vp]),
     "sesdf_set_atoms": (i32, [vp, vp]),
     "sesdf_input_buffer": (i32, [vp, ctypes.POINTER(vp)]),

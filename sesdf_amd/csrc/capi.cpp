@@ -50,6 +50,19 @@ sesdf_status sesdf::report_error( sesdf_status st, const char * what ) { return 
             return hip_fail( _e, #expr );                                                                                        \
     } while ( 0 )
 
+// Buffers freed by sesdf_release are allocated again before the handle is used.
+#define BUFFERS_TRY( h )                                                                                                         \
+    do                                                                                                                           \
+    {                                                                                                                            \
+        if ( ( h ).released )                                                                                                    \
+        {                                                                                                                        \
+            sesdf_status _s = allocate( h );                                                                                     \
+            if ( _s != SESDF_OK )                                                                                                \
+                return _s;                                                                                                       \
+            ( h ).released = false;                                                                                              \
+        }                                                                                                                        \
+    } while ( 0 )
+
 // A bounding box the grid can be laid out on: finite, min <= max on each axis (an infinite or
 // inverted box would give a brick of no cells, or none that fits).
 static bool box_ok( const float * bx )
@@ -623,6 +636,7 @@ extern "C"
         if ( !okW( t->circle_waves ) || !okW( t->intersection_waves ) || t->cell_size < 0.f )
             return SESDF_ERR_INVALID;
         HIP_TRY( hipSetDevice( h.device ) );
+        BUFFERS_TRY( h );
         h.wavesCircles = t->circle_waves ? t->circle_waves : 4;
         h.wavesInter   = t->intersection_waves ? t->intersection_waves : 4;
         h.b.wavesCircles = h.wavesCircles;
@@ -670,7 +684,10 @@ extern "C"
     {
         if ( !hh || !d )
             return SESDF_ERR_INVALID;
-        *d = reinterpret_cast<float *>( reinterpret_cast<Handle *>( hh )->input );
+        Handle & h = *reinterpret_cast<Handle *>( hh );
+        HIP_TRY( hipSetDevice( h.device ) );
+        BUFFERS_TRY( h );
+        *d = reinterpret_cast<float *>( h.input );
         return SESDF_OK;
     }
 
@@ -682,12 +699,28 @@ extern "C"
         return SESDF_OK;
     }
 
+    sesdf_status sesdf_release( sesdf_handle * hh )
+    {
+        if ( !hh )
+            return SESDF_ERR_INVALID;
+        Handle & h = *reinterpret_cast<Handle *>( hh );
+        HIP_TRY( hipSetDevice( h.device ) );
+        HIP_TRY( hipStreamSynchronize( h.stream ) );
+        h.freeSdf();
+        h.freeAll();
+        h.built     = false;
+        h.released  = true;
+        h.stageNext = 0;
+        return SESDF_OK;
+    }
+
     sesdf_status sesdf_build( sesdf_handle * hh )
     {
         if ( !hh )
             return SESDF_ERR_INVALID;
         Handle & h = *reinterpret_cast<Handle *>( hh );
         HIP_TRY( hipSetDevice( h.device ) );
+        BUFFERS_TRY( h );
         if ( h.n )
         {
             if ( !h.hostAtoms )
@@ -704,6 +737,7 @@ extern "C"
             return SESDF_ERR_INVALID;
         Handle & h = *reinterpret_cast<Handle *>( hh );
         HIP_TRY( hipSetDevice( h.device ) );
+        BUFFERS_TRY( h );
         if ( h.n && reinterpret_cast<const float4 *>( d_xyzr ) != h.input )
             HIP_TRY( hipMemcpyAsync( h.input, d_xyzr, size_t( h.n ) * sizeof( float4 ), hipMemcpyDeviceToDevice, h.stream ) );
         return build_resident( h );
@@ -713,7 +747,10 @@ extern "C"
     {
         if ( !hh )
             return SESDF_ERR_INVALID;
-        return enqueue( *reinterpret_cast<Handle *>( hh ) );
+        Handle & h = *reinterpret_cast<Handle *>( hh );
+        HIP_TRY( hipSetDevice( h.device ) );
+        BUFFERS_TRY( h );
+        return enqueue( h );
     }
 
     sesdf_status sesdf_sync( sesdf_handle * hh )
@@ -738,6 +775,7 @@ extern "C"
             return SESDF_ERR_INVALID;
         Handle & h = *reinterpret_cast<Handle *>( hh );
         HIP_TRY( hipSetDevice( h.device ) );
+        BUFFERS_TRY( h );
         for ( auto & e : h.events )
             if ( !e )
                 HIP_TRY( hipEventCreate( &e ) );
@@ -972,6 +1010,7 @@ extern "C"
             return SESDF_ERR_INVALID;
         Handle & h = *reinterpret_cast<Handle *>( hh );
         HIP_TRY( hipSetDevice( h.device ) );
+        BUFFERS_TRY( h );
         if ( h.n )
         {
             if ( !h.hostAtoms )
@@ -996,6 +1035,7 @@ extern "C"
         if ( stage != 0 && h.stageNext == 0 )
             return set_error( SESDF_ERR_INVALID, "sesdf_find_circles must run first" );
         HIP_TRY( hipSetDevice( h.device ) );
+        BUFFERS_TRY( h );
         const Buffers & b = h.b;
         hipStream_t     s = h.stream;
         if ( h.n )

@@ -61,6 +61,7 @@ namespace sesdf
         bool        useGraph  = false;
         bool        allocated = false;
         bool        built     = false;
+        bool        released  = false;   // device buffers freed by sesdf_release (the next build allocates)
         bool        inputValid = false;
         uint32_t    rebuilds  = 0;
         uint32_t    relayouts = 0;           // grid lay-outs redone by the last build (moved atoms)

@@ -74,6 +74,11 @@ class _Surface:
         """Build from atoms already resident in device memory (pointer to float4 array)."""
         L.check(L.lib().sesdf_build_device(self._h, ctypes.c_void_p(d_xyzr)), "sesdf_build_device")
 
+    def release(self):
+        """Free every device buffer of the build (topology and sampling state); the next build
+        allocates them again."""
+        L.check(L.lib().sesdf_release(self._h), "sesdf_release")
+
     def get_max_neighbor_per_atom(self):
         return self.internal().max_neighbors
 

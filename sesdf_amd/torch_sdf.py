@@ -4,9 +4,12 @@
 ``atoms`` ([N, 4] float32 on the GPU: x, y, z, vdW radius, input order) with the handle ``surface``
 (a ``sesdf_amd.Sesdf`` created with the same atom count), and returns SDF(points) ([Q] float32).
 Its backward gives d(sum_q g_q SDF(q_q))/d(atoms) with the patch topology of the forward build held
-fixed (the hand-written HIP backward kernel behind the C ABI). ``store=True`` keeps the winning
-element of every query from the forward pass (8 bytes per query); ``store=False`` re-derives it in
-the backward pass.
+fixed (the hand-written HIP backward kernel behind the C ABI). ``store`` trades memory for time:
+  True       keeps the winning element of every query from the forward pass (8 bytes per query);
+  False      keeps the topology and re-derives the winners in the backward pass;
+  "release"  frees the whole build (neighbour lists, circles, patches, segments: the device
+             buffers of the handle) after the forward pass; the backward pass builds the same
+             atoms again (deterministic: the same topology) and re-derives the winners.
 
 The handle may be rebuilt between forward and backward (two forwards on one handle before one
 backward, several conformers in one loss): the backward then builds the forward's atoms again
@@ -29,16 +32,22 @@ class SurfaceSDF(torch.autograd.Function):
         pts = pts.contiguous()
         if atoms.shape[0] != surface.atom_nb:
             raise ValueError("surface handle was created for a different atom count")
+        if store not in (True, False, "release"):
+            raise ValueError('store must be True, False or "release"')
+        keep = store is True
         torch.cuda.current_stream().synchronize()
         surface.build_device(atoms.data_ptr())
         q = pts.shape[0]
         out = torch.empty(q, device=pts.device, dtype=torch.float32)
-        codes = torch.empty(q, device=pts.device, dtype=torch.int64) if store else None
+        codes = torch.empty(q, device=pts.device, dtype=torch.int64) if keep else None
         surface.sample_points_ptr(pts.data_ptr(), q, out.data_ptr(), band=band, exterior=exterior,
-                                  d_codes=codes.data_ptr() if store else None)
+                                  d_codes=codes.data_ptr() if keep else None)
         _sync(surface)
         ctx.surface, ctx.band, ctx.exterior, ctx.codes = surface, band, exterior, codes
         ctx.serial = surface.counts()["build_serial"]
+        ctx.released = store == "release"
+        if ctx.released:
+            surface.release()
         ctx.save_for_backward(pts, atoms)
         return out
 
@@ -46,7 +55,7 @@ class SurfaceSDF(torch.autograd.Function):
     def backward(ctx, grad_out):
         pts, atoms = ctx.saved_tensors
         surface = ctx.surface
-        if surface.counts()["build_serial"] != ctx.serial:
+        if ctx.released or surface.counts()["build_serial"] != ctx.serial:
             # the handle holds another build now: build the forward's atoms again
             torch.cuda.current_stream().synchronize()
             surface.build_device(atoms.data_ptr())

@@ -326,7 +326,7 @@ def test_differentiation_workload(gpu):
     """20,000-atom globule, 1M queries drawn uniformly within 2 A of the surface, w ~ N(0, 1): the
     SDF values, the backward with stored and with recomputed winners against the FP64 analytic
     gradient, and against FP64 central differences (h = 1e-4 A, topology held fixed) on 256 random
-    surface atoms away from winner changes (for the loss over a quarter of the queries)."""
+    atoms of the loss over 50,000 queries, per atom."""
     import torch
 
     atoms = synth.globule(20000, synth.SEED_BASE + 4)
@@ -378,21 +378,27 @@ def test_differentiation_workload(gpu):
         assert np.quantile(err, 0.999) <= 1e-3
         assert np.mean(err > 1e-3) < 1e-3
 
-    # and against central differences on 256 random surface atoms, for the loss over the first
-    # 250,000 queries (the differences cost 8 oracle evaluations per query near each atom)
-    sub = 250_000
+    # and against FP64 central differences (h = 1e-4 A, topology held fixed) on 256 random atoms
+    # of the loss over the first 50,000 queries. An atom is stable when no query near it changes
+    # its winning element under the +-h moves (a change puts a kink inside the difference). Per
+    # atom: ||g - fd|| / max(||fd||, floor), floor = 1% of the median ||fd|| of the 256 atoms.
+    sub = 50_000
     grad = torch.zeros(len(atoms), 4, device="cuda")
     scratch = torch.empty(len(atoms), 4, device="cuda")
     surf.sdf_backward_ptr(P.data_ptr(), W.data_ptr(), sub, grad.data_ptr(), scratch.data_ptr(), band=band,
                           d_codes=codes.data_ptr())
     torch.cuda.synchronize()
     g_sub = grad.cpu().numpy().astype(np.float64)
-    surface = np.nonzero(np.abs(ref).max(1) > 1e-3 * scale)[0]
-    ids = np.sort(rng.choice(surface, 256, replace=False)).astype(np.uint32)
+    ref_sub = orc.sdf_grad(p64[:sub], w[:sub].astype(np.float64), band=band, threads=16)
+    norm_ref = np.linalg.norm(ref_sub, axis=1)
+    touched = np.nonzero(norm_ref > 1e-3 * norm_ref.max())[0]
+    ids = np.sort(rng.choice(touched, 256, replace=False)).astype(np.uint32)
     fd, stable = orc.sdf_fd(p64[:sub], w[:sub].astype(np.float64), ids, h=1e-4, band=band, threads=16)
-    scale_sub = np.abs(g_sub).max()
-    rel = np.abs(fd - g_sub[ids]).max(1) / scale_sub
-    _say(f"finite differences: {int(stable.sum())} of 256 atoms stable, max rel err {rel[stable].max():.2e}")
-    assert stable.mean() > 0.5
+    nfd = np.linalg.norm(fd, axis=1)
+    floor = 1e-2 * np.median(nfd)
+    rel = np.linalg.norm(g_sub[ids] - fd, axis=1) / np.maximum(nfd, floor)
+    _say(f"finite differences: {int(stable.sum())} of 256 atoms stable; per-atom ||g - fd|| / max(||fd||, {floor:.3g}): "
+         f"max {rel[stable].max():.2e}, median {np.median(rel[stable]):.2e}; unstable atoms max {rel[~stable].max() if (~stable).any() else 0:.2e}")
+    assert stable.mean() >= 0.9
     assert np.all(rel[stable] <= 1e-3)
     surf.close()

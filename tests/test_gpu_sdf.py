@@ -174,6 +174,17 @@ def test_torch_autograd_function(gpu):
     g_rec = A.grad.detach().cpu().numpy()
     assert torch.equal(out, out2)
     np.testing.assert_allclose(g_store, g_rec, rtol=0, atol=1e-4 * np.abs(g_store).max())
+    # release: the forward frees the whole build, the backward builds it again
+    A.grad = None
+    held = surf.counts()["device_bytes"]
+    out3 = SurfaceSDF.apply(A, P, surf, 2.0, "release", False)
+    assert surf.counts()["device_bytes"] == 0 < held
+    (out3 * w).sum().backward()
+    g_rel = A.grad.detach().cpu().numpy()
+    assert torch.equal(out, out3)
+    assert surf.counts()["device_bytes"] > 0
+    # the rebuilt topology is the same, and the accumulation exact: the recompute gradient, bit for bit
+    assert np.array_equal(g_rel, g_rec)
     ref = orc.sdf_grad(pts.astype(np.float64), w.cpu().numpy().astype(np.float64), band=2.0)
     assert np.abs(g_store - ref).max() <= GRAD_TOL * np.abs(ref).max() * 10
     # finite differences of the FP64 oracle with the topology held fixed, on a few atoms
