@@ -1066,6 +1066,99 @@ namespace sesdf
             gf[ a ] = 0ull;
     }
 
+    // Gradient contributions of one query in FP32: the winner's nearest point p, e = (p - q) / |p - q|
+    // and nu from the normal equations of its 1..3 sphere constraints (Cramer). False, with nothing
+    // added, where FP32 is not enough: constraint vectors nearly dependent (the Gram determinant
+    // below 1e-3 of the product of its diagonal: squared sine or squared volume) or, for an arc, q
+    // near the circle's axis (the nearest point then turns fast with q). FP32 errors stay below 1e-4
+    // of the contribution scale elsewhere.
+    __device__ bool winner_gradient_fp32( const Buffers & b, const SdfArgs & s, f3 q, uint32_t type, uint32_t idx, float w, double scale )
+    {
+        const float rp = b.probeRadius;
+        uint32_t    at[ 3 ];
+        int         m = 0;
+        f3          p;
+        if ( type == SdfFace )
+        {
+            const float4 A = b.atoms[ idx ];
+            const f3     v = q - xyz( A );
+            at[ 0 ] = idx, m = 1;
+            p       = xyz( A ) + v * ( ( A.w + rp ) / sqrtf( len2( v ) ) );
+        }
+        else if ( type == SdfVertex )
+        {
+            const int4 ids = b.concaveIds[ idx ];
+            at[ 0 ] = uint32_t( ids.x ), at[ 1 ] = uint32_t( ids.y ), at[ 2 ] = uint32_t( ids.z ), m = 3;
+            p = xyz( b.concavePos[ idx ] );
+        }
+        else
+        {
+            const uint2  ij = type == SdfCircle ? b.fullCircles[ idx ] : make_uint2( b.segments[ idx ].x, b.segments[ idx ].y );
+            const float4 Ai = b.atoms[ ij.x ], Aj = b.atoms[ ij.y ];
+            at[ 0 ] = ij.x, at[ 1 ] = ij.y, m = 2;
+            const float Ri = Ai.w + rp, Rj = Aj.w + rp;
+            const f3    dj  = xyz( Aj ) - xyz( Ai );
+            const float d2  = len2( dj );
+            const f3    nrm = dj * ( 1.f / sqrtf( d2 ) );
+            const f3    ctr = xyz( Ai ) + dj * ( ( Ri * Ri - Rj * Rj + d2 ) / ( 2.f * d2 ) );
+            const float rad = sqrtf( fmaxf( Ri * Ri - len2( ctr - xyz( Ai ) ), 0.f ) );
+            const f3    v   = q - ctr;
+            const f3    wv  = v - nrm * dot( v, nrm );
+            const float lw  = sqrtf( len2( wv ) );
+            if ( !( lw > 1e-2f * rad ) )
+                return false;
+            p = ctr + wv * ( rad / lw );
+        }
+        f3          e = p - q;
+        const float D = sqrtf( len2( e ) );
+        if ( !( D > 0.f ) )
+            return false;
+        e = e * ( 1.f / D );
+        f3    a[ 3 ];
+        float G[ 3 ][ 3 ], r[ 3 ], nu[ 3 ] = { 0.f, 0.f, 0.f };
+        for ( int u = 0; u < m; u++ )
+        {
+            a[ u ] = p - xyz( b.atoms[ at[ u ] ] );
+            r[ u ] = dot( a[ u ], e );
+            for ( int v = 0; v <= u; v++ )
+                G[ u ][ v ] = G[ v ][ u ] = dot( a[ u ], a[ v ] );
+        }
+        if ( m == 1 )
+            nu[ 0 ] = r[ 0 ] / G[ 0 ][ 0 ];
+        else if ( m == 2 )
+        {
+            const float det = G[ 0 ][ 0 ] * G[ 1 ][ 1 ] - G[ 0 ][ 1 ] * G[ 1 ][ 0 ];
+            if ( !( det > 1e-3f * G[ 0 ][ 0 ] * G[ 1 ][ 1 ] ) )
+                return false;
+            nu[ 0 ] = ( r[ 0 ] * G[ 1 ][ 1 ] - G[ 0 ][ 1 ] * r[ 1 ] ) / det;
+            nu[ 1 ] = ( G[ 0 ][ 0 ] * r[ 1 ] - G[ 1 ][ 0 ] * r[ 0 ] ) / det;
+        }
+        else
+        {
+            auto det3 = []( const float A[ 3 ][ 3 ] ) {
+                return A[ 0 ][ 0 ] * ( A[ 1 ][ 1 ] * A[ 2 ][ 2 ] - A[ 1 ][ 2 ] * A[ 2 ][ 1 ] ) - A[ 0 ][ 1 ] * ( A[ 1 ][ 0 ] * A[ 2 ][ 2 ] - A[ 1 ][ 2 ] * A[ 2 ][ 0 ] ) +
+                       A[ 0 ][ 2 ] * ( A[ 1 ][ 0 ] * A[ 2 ][ 1 ] - A[ 1 ][ 1 ] * A[ 2 ][ 0 ] );
+            };
+            const float d = det3( G );
+            if ( !( d > 1e-3f * G[ 0 ][ 0 ] * G[ 1 ][ 1 ] * G[ 2 ][ 2 ] ) )
+                return false;
+            for ( int c = 0; c < 3; c++ )
+            {
+                float A[ 3 ][ 3 ];
+                for ( int u = 0; u < 3; u++ )
+                    for ( int v = 0; v < 3; v++ )
+                        A[ u ][ v ] = v == c ? r[ u ] : G[ u ][ v ];
+                nu[ c ] = det3( A ) / d;
+            }
+        }
+        for ( int u = 0; u < m; u++ )
+        {
+            const float Ru = b.atoms[ at[ u ] ].w + rp;
+            grad_add( s.gradFixed, at[ u ], mkd( a[ u ] * ( -nu[ u ] ) ), double( -nu[ u ] * Ru ), w, scale );
+        }
+        return true;
+    }
+
     template<bool EXT>
     __global__ void __launch_bounds__( 256 ) sdf_backward_kernel( Buffers b, SdfArgs s )
     {
@@ -1096,6 +1189,8 @@ namespace sesdf
                 grad_add( s.gradFixed, idx, v * ( -1.0 / dn ), -1.0, w, scale );
                 continue;
             }
+            if ( winner_gradient_fp32( b, s, qf, type, idx, w, scale ) )
+                continue; // well conditioned: done in FP32
             uint32_t at[ 3 ];
             int      m = 0;
             d3       p;
