@@ -117,6 +117,8 @@ typedef struct
     uint32_t grid_relayout_nb;         /* grid lay-outs redone because atoms moved outside the cell bricks */
     uint32_t component_overflow_atoms; /* exterior SDF: atoms whose arcs exceeded the per-atom face grouping
                                           capacity (their loops were joined) */
+    uint32_t max_neighbors;            /* neighbour capacity K of the last build */
+    uint32_t neighbor_growth_nb;       /* K growths by the last build (SESDF_NEIGHBORS_GROW) */
 } sesdf_counts;
 
 /* Device pointers of the internal arrays exposed by the reference's analysis getters
@@ -152,6 +154,14 @@ sesdf_status sesdf_build( sesdf_handle * h );
  * allocates them again, and getters return SESDF_ERR_NOT_BUILT until then. No counterpart in the
  * reference: it serves a backward pass that rebuilds the topology instead of holding it. */
 sesdf_status sesdf_release( sesdf_handle * h );
+/* What a build does with an atom that has more than K neighbours. SESDF_NEIGHBORS_TRUNCATE (the
+ * default): keep the K lowest sorted indices, as the reference does (operations.cu:86-121), and
+ * count the atom (neighbor_overflow_atoms). SESDF_NEIGHBORS_GROW: raise K to the longest list
+ * rounded up to a multiple of 32 (at most 320; beyond that lists truncate at 320) and build again.
+ * No counterpart in the reference, whose K is a compile-time 128 or 288. */
+#define SESDF_NEIGHBORS_TRUNCATE 0
+#define SESDF_NEIGHBORS_GROW     1
+sesdf_status sesdf_set_neighbor_policy( sesdf_handle * h, int policy );
 /* Same stage sequence on atoms already resident in device memory (float4, atom_nb of them). When
  * d_xyzr is the handle's own input buffer (sesdf_input_buffer) nothing is copied. */
 sesdf_status sesdf_build_device( sesdf_handle * h, const float * d_xyzr );

@@ -48,7 +48,8 @@ class Counts(ctypes.Structure):
                 ("neighbor_overflow_atoms", u32), ("probe_neighbor_overflow", u32),
                 ("circle_intersection_overflow", u32), ("jk_anomaly_nb", u32), ("fp64_resolve_nb", u32),
                 ("rebuild_nb", u32), ("max_atom_records", u32), ("device_bytes", u64), ("build_serial", u64),
-                ("grid_relayout_nb", u32), ("component_overflow_atoms", u32)]
+                ("grid_relayout_nb", u32), ("component_overflow_atoms", u32), ("max_neighbors", u32),
+                ("neighbor_growth_nb", u32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -76,6 +77,7 @@ SIGNATURES = {
     "sesdf_destroy": (i32, [vp]),
     "sesdf_build": (i32, [vp]),
     "sesdf_release": (i32, [vp]),
+    "sesdf_set_neighbor_policy": (i32, [vp, ctypes.c_int]),
     "sesdf_build_device": (i32, [vp, vp]),
     "sesdf_set_atoms": (i32, [vp, vp]),
     "sesdf_input_buffer": (i32, [vp, ctypes.POINTER(vp)]),

@@ -38,6 +38,8 @@ namespace
 
     // Sorted atom indices share a 32-bit word with two flags in the sampling kernels.
     constexpr uint64_t MaxAtoms = 1ull << 30;
+    // Largest neighbour capacity K (the kernels' widest list instance).
+    constexpr uint32_t MaxNeighbors = 320;
 } // namespace
 
 sesdf_status sesdf::report_error( sesdf_status st, const char * what ) { return set_error( st, what ); }
@@ -384,10 +386,32 @@ namespace sesdf
         return I > h.cap.records || c.intersectionWithNbNb > h.cap.probeNb;
     }
 
+    // Neighbour lists longer than K: with the grow policy, K becomes the longest list rounded up to
+    // a multiple of 32 (at most 320) and the build runs again on reallocated buffers.
+    static sesdf_status grow_neighbors( Handle & h, bool & grown )
+    {
+        grown = false;
+        const Control & c = *h.hctl;
+        if ( !h.growNeighbors || c.nbOverflowAtoms == 0 || h.K >= MaxNeighbors )
+            return SESDF_OK;
+        const uint32_t want = std::min<uint32_t>( MaxNeighbors, ( std::max( c.nbMaxCount, h.K + 1 ) + 31u ) & ~31u );
+        std::vector<float4> keep( h.n );
+        HIP_TRY( hipMemcpy( keep.data(), h.input, size_t( h.n ) * sizeof( float4 ), hipMemcpyDeviceToHost ) );
+        h.K = want;
+        h.freeSdf(); // sampling state reads the N x K lists
+        const sesdf_status st = allocate( h );
+        if ( st != SESDF_OK )
+            return st;
+        HIP_TRY( hipMemcpy( h.input, keep.data(), size_t( h.n ) * sizeof( float4 ), hipMemcpyHostToDevice ) );
+        h.neighborGrowths++;
+        grown = true;
+        return SESDF_OK;
+    }
+
     sesdf_status build_resident( Handle & h )
     {
-        h.rebuilds = 0, h.relayouts = 0;
-        for ( int attempt = 0; attempt < 4; attempt++ )
+        h.rebuilds = 0, h.relayouts = 0, h.neighborGrowths = 0;
+        for ( int attempt = 0; attempt < 6; attempt++ )
         {
             sesdf_status st = enqueue( h );
             if ( st != SESDF_OK )
@@ -395,6 +419,12 @@ namespace sesdf
             st = readback( h );
             if ( st != SESDF_OK )
                 return st;
+            bool grownK = false;
+            st          = grow_neighbors( h, grownK );
+            if ( st != SESDF_OK )
+                return st;
+            if ( grownK )
+                continue;
             if ( !overflowed( h ) )
             {
                 h.built = true;
@@ -438,7 +468,7 @@ static sesdf_status common_setup( Handle & h, int device, void * stream, uint32_
 {
     if ( K == 0 )
         K = 128;
-    if ( K % 32 != 0 || K > 320 || ( mode != 0 && mode != 1 ) || !( rp >= 0.f ) )
+    if ( K % 32 != 0 || K > MaxNeighbors || ( mode != 0 && mode != 1 ) || !( rp >= 0.f ) )
         return SESDF_ERR_INVALID;
     h.device     = device;
     h.K          = K;
@@ -714,6 +744,14 @@ extern "C"
         return SESDF_OK;
     }
 
+    sesdf_status sesdf_set_neighbor_policy( sesdf_handle * hh, int policy )
+    {
+        if ( !hh || ( policy != SESDF_NEIGHBORS_TRUNCATE && policy != SESDF_NEIGHBORS_GROW ) )
+            return SESDF_ERR_INVALID;
+        reinterpret_cast<Handle *>( hh )->growNeighbors = policy == SESDF_NEIGHBORS_GROW;
+        return SESDF_OK;
+    }
+
     sesdf_status sesdf_build( sesdf_handle * hh )
     {
         if ( !hh )
@@ -825,6 +863,8 @@ extern "C"
              hipMemcpy( &compOverflow, h.sdf.comp.extComp + 1, 4, hipMemcpyDeviceToHost ) != hipSuccess )
             compOverflow = 0;
         out->component_overflow_atoms = compOverflow;
+        out->max_neighbors            = h.K;
+        out->neighbor_growth_nb       = h.neighborGrowths;
         return SESDF_OK;
     }
 

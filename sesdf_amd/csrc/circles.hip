@@ -212,7 +212,10 @@ namespace sesdf
             wave_sync();
         }
         if ( count > K && lane == 0 )
+        {
             atomicAdd( &b.ctl->nbOverflowAtoms, 1u ); // rare: truncated list, as the reference
+            atomicMax( &b.ctl->nbMaxCount, count );
+        }
         const uint32_t ni = uint32_t( __builtin_amdgcn_readfirstlane( int( min( count, K ) ) ) ); // uniform
         wave_sync();
 

@@ -135,7 +135,8 @@ namespace sesdf
         uint32_t fp64ResolveNb;         // triplets re-solved in FP64
         uint32_t maxAtomRecords;        // largest per-atom concave record count
         uint32_t neighborNb;            // sum of neighbour counts (ordered circles)
-        uint32_t pad[ 4 ];
+        uint32_t nbMaxCount;            // largest neighbour count of a truncated list (before truncation)
+        uint32_t pad[ 3 ];
     };
 
     // ------------------------------------------------------------------------------------------

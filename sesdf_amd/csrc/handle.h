@@ -62,6 +62,8 @@ namespace sesdf
         bool        allocated = false;
         bool        built     = false;
         bool        released  = false;   // device buffers freed by sesdf_release (the next build allocates)
+        bool        growNeighbors = false; // grow K and rebuild when a list overflows (else truncate)
+        uint32_t    neighborGrowths = 0;   // K growths by the last build
         bool        inputValid = false;
         uint32_t    rebuilds  = 0;
         uint32_t    relayouts = 0;           // grid lay-outs redone by the last build (moved atoms)

@@ -37,7 +37,7 @@ class _Surface:
 
     def __init__(self, molecule, aabb=None, probe_radius=1.4, build_surface=True, graphics=False, *,
                  max_neighbors=128, max_intersection_neighbors=32, cell_size=0.0, device=0, stream=None,
-                 use_graph=True):
+                 use_graph=True, grow_neighbors=False):
         lib = L.lib()
         self._atoms = np.ascontiguousarray(np.asarray(molecule, np.float32).reshape(-1, 4))
         self.probe_radius = float(probe_radius)
@@ -62,6 +62,9 @@ class _Surface:
         h = ctypes.c_void_p()
         L.check(lib.sesdf_create(ctypes.byref(info), ctypes.byref(h)), "sesdf_create")
         self._h = h
+        if grow_neighbors:
+            # lists longer than max_neighbors: grow K and build again instead of truncating
+            L.check(lib.sesdf_set_neighbor_policy(h, 1), "sesdf_set_neighbor_policy")
         if build_surface:
             self.build()
 

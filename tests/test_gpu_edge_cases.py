@@ -187,6 +187,27 @@ def test_large_probe_radius_wide_lists(gpu, mode):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
+def test_neighbor_overflow_grows_the_capacity(gpu, mode):
+    """With the grow policy a build whose lists exceed K = 32 raises K to the longest list (a
+    multiple of 32) and builds again: no list is truncated, and the whole build equals the oracle's
+    with untruncated lists (K = 128)."""
+    atoms = synth.globule(3000, synth.SEED_BASE + 50)
+    surf = _surf(atoms, mode, max_neighbors=32, grow_neighbors=True)
+    c = surf.counts()
+    print("after growth:", {k: c[k] for k in ("max_neighbors", "neighbor_growth_nb", "neighbor_overflow_atoms")}, flush=True)
+    assert c["neighbor_growth_nb"] >= 1 and c["neighbor_overflow_atoms"] == 0
+    assert 32 < c["max_neighbors"] <= 128 and c["max_neighbors"] % 32 == 0
+    cnt, _ = surf.get_neighbors()
+    assert cnt.max() > 32 and c["max_neighbors"] - 32 < cnt.max() <= c["max_neighbors"]
+    orc = O.build(atoms, mode=mode, precision=64, track=True, threads=8, max_neighbors=128)
+    rep = parity.compare(parity.gpu_arrays(surf), orc, atoms, mode)
+    parity.assert_parity(rep)
+    # the default policy truncates, as the reference
+    plain = _surf(atoms, mode, max_neighbors=32)
+    assert plain.counts()["neighbor_overflow_atoms"] > 0 and plain.counts()["max_neighbors"] == 32
+
+
+@pytest.mark.parametrize("mode", [0, 1])
 def test_tangent_spheres_within_epsilon(gpu, mode):
     """Spheres farther apart than Ri + Rj by less than the predicate's slack are neighbours whose
     circle has radius^2 < 0. The reference's sector angle is then acos(clamp(NaN)) = acos(1) = 0
