@@ -363,6 +363,113 @@ sesdf_status sesdf_set_tuning( sesdf_handle * h, const sesdf_tuning * t );
  * segments, torus records, the molecule_nb + 1 prefix offsets (host memory). */
 sesdf_status sesdf_batch_offsets( sesdf_handle * h, uint64_t * out_offsets );
 
+/* -------- Contour-Buildup comparator: bcs::ContourBuildup (sesdf/include/bcs/contour_buildup/
+ * contour_buildup.hpp:24-43, sesdf/src/contour_buildup/contour_buildup.cu:58-703), the reference's
+ * second surface method, which its benchmark times beside the SES builder
+ * (app/benchmarks/main.cu:155-170). Same algorithm, grid (grid_dimensions^3 cells over the box,
+ * reference 16) and capacities (64 neighbours per atom, 32 per probe); the OpenGL buffers become
+ * device buffers owned by the handle, valid until the next build. -------- */
+typedef struct sesdf_cb_handle sesdf_cb_handle;
+
+/* ContourBuildup(molecule, aabb, probeRadius = 1.4, gridDimensions = 16, buildSurface = true)
+ * (contour_buildup.hpp:27-31) plus the launch environment. */
+typedef struct
+{
+    const float * xyzr;            /* host atoms, float4 xyz + vdW radius (non-owning, read at build) */
+    uint32_t      atom_nb;         /* below 2^24 */
+    float         probe_radius;
+    float         aabb[ 6 ];       /* min xyz, max xyz; all zero: centre -/+ radius of the atoms (getAabb) */
+    uint32_t      grid_dimensions; /* cells per axis, a power of two up to 128; 0: 16 (the reference's) */
+    uint32_t      tex_size;        /* row length of the singularity texture (GL_MAX_TEXTURE_SIZE in the
+                                      reference, contour_buildup.cu:182-186); 0: 32768 */
+    int32_t       device;
+    void *        stream;          /* hipStream_t; NULL: the handle creates its own */
+    int32_t       use_graph;       /* 1: replay the stage sequence as one hipGraph */
+    int32_t       build_surface;   /* 1: build inside sesdf_cb_create */
+} sesdf_cb_create_info;
+
+/* The buffers behind ContourBuildupGraphics (sesdf/include/bcs/contour_buildup/graphics.hpp:6-19;
+ * contour_buildup.cu:97-112 buffer roles). Device pointers, float4 unless noted. */
+typedef struct
+{
+    uint32_t atom_nb;
+    float *  atoms;                    /* grid-sorted atoms */
+    uint32_t probe_nb;                 /* spherical triangles */
+    float *  probe_positions;          /* PROBE_POS: xyz, probe radius */
+    float *  sphere_triangle_vec1;     /* SPHERE_TRIA_VEC_1: atom i - probe, 1 */
+    float *  sphere_triangle_vec2;     /* SPHERE_TRIA_VEC_2: atom j - probe, 1 */
+    float *  sphere_triangle_vec3;     /* SPHERE_TRIA_VEC_3: atom k - probe, probe radius^2 */
+    uint32_t torus_nb;
+    float *  torus_positions;          /* TORUS_POS: centre, radius R */
+    float *  torus_axes;               /* TORUS_AXIS: unit axis, probe radius */
+    float *  torus_visibility_spheres; /* TORUS_VS: centre (relative to the torus centre), radius */
+    float *  tex_coords;               /* TEX_COORD: float3 per probe (neighbour count, row, column) */
+    float *  singularity_texture;      /* SING_TEX: float3 * probe_neighbor_nb per sorted probe (first
+                                          count entries set) */
+    uint32_t probe_neighbor_nb;        /* 32 */
+    uint32_t tex_size;
+    float    probe_radius;
+} sesdf_cb_data;
+
+typedef struct
+{
+    uint32_t atom_nb;
+    uint32_t probe_nb;                 /* numProbes (contour_buildup.cu:467-481) */
+    uint32_t torus_nb;                 /* numSC (contour_buildup.cu:452-463) */
+    uint32_t neighbor_overflow_atoms;  /* atoms with more than 64 neighbours (lists truncated, as the reference) */
+    uint32_t no_contour_atoms;         /* atoms whose contour vanished in the covered-circle pass */
+    uint32_t arc_capacity;             /* arcs per circle held on chip (8, or 80 after an overflow) */
+    uint32_t build_attempts;           /* builds run by the last call (capacity growth repeats it) */
+    uint32_t grid_dimensions;
+    uint64_t device_bytes;
+    uint64_t build_serial;
+} sesdf_cb_counts;
+
+/* Internal arrays (device pointers), for census and parity checks. */
+typedef struct
+{
+    uint32_t * sorted_to_initial; /* N: sorted atom -> input atom */
+    uint32_t * neighbor_count;    /* N: neighbours after the covered-circle pass (0: no contour) */
+    uint32_t * neighbor_ids;      /* N * max_neighbors: sorted atom indices, scan order */
+    uint32_t   max_neighbors;     /* 64 */
+    uint32_t * probe_atoms;       /* uint4 per probe: (i, j, k, 0) sorted atom indices */
+    uint32_t * torus_atoms;       /* uint2 per torus: (i, j) */
+    uint32_t * probe_sorted;      /* per sorted probe: probe index (row order of the texture) */
+} sesdf_cb_internal;
+
+/* ContourBuildup::ContourBuildup (contour_buildup.cu:58-369). */
+sesdf_status sesdf_cb_create( const sesdf_cb_create_info * info, sesdf_cb_handle ** out );
+/* ContourBuildup::~ContourBuildup (contour_buildup.cu:370-398). */
+sesdf_status sesdf_cb_destroy( sesdf_cb_handle * h );
+/* ContourBuildup::build (contour_buildup.cu:400-660): uploads the host atoms, builds, reads the
+ * counts back (blocking). */
+sesdf_status sesdf_cb_build( sesdf_cb_handle * h );
+/* Same on atoms already in device memory (float4, atom_nb of them; null: the input buffer). */
+sesdf_status sesdf_cb_build_device( sesdf_cb_handle * h, const float * d_xyzr );
+/* Replace the host atom span read by sesdf_cb_build (same atom count). */
+sesdf_status sesdf_cb_set_atoms( sesdf_cb_handle * h, const float * xyzr );
+sesdf_status sesdf_cb_input_buffer( sesdf_cb_handle * h, float ** d_xyzr );
+/* Replay the last build's stage sequence without readback (timing); sesdf_cb_sync ends it and
+ * reports SESDF_ERR_CAPACITY if a replay outgrew a capacity. */
+sesdf_status sesdf_cb_enqueue( sesdf_cb_handle * h );
+sesdf_status sesdf_cb_sync( sesdf_cb_handle * h );
+sesdf_status sesdf_cb_get_stream( sesdf_cb_handle * h, void ** stream );
+/* ContourBuildup::getGraphics (contour_buildup.cu:662-665): the buffers and counts. */
+sesdf_status sesdf_cb_get_data( sesdf_cb_handle * h, sesdf_cb_data * out );
+sesdf_status sesdf_cb_get_counts( sesdf_cb_handle * h, sesdf_cb_counts * out );
+sesdf_status sesdf_cb_get_internal( sesdf_cb_handle * h, sesdf_cb_internal * out );
+/* Capacities of the next build (no reference counterpart: the reference sizes its buffers for
+ * 64 probes per atom). arc_capacity: arcs per circle held on chip, 8 (default; wider lists build
+ * again at 80) or 80; probe_capacity / torus_capacity: output slots (0 keeps the current ones; a
+ * build that needs more grows them and runs again). */
+typedef struct
+{
+    uint32_t arc_capacity;
+    uint32_t probe_capacity;
+    uint32_t torus_capacity;
+} sesdf_cb_tuning;
+sesdf_status sesdf_cb_set_tuning( sesdf_cb_handle * h, const sesdf_cb_tuning * t );
+
 const char * sesdf_status_string( sesdf_status s );
 const char * sesdf_last_error( void );
 

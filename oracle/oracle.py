@@ -2,7 +2,8 @@
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module.
 It runs the CPU restatement of the reference's stage sequence and the FP64 SDF / gradient oracle
-(both in oracle/sesdf_oracle.cpp) and returns numpy arrays.
+(both in oracle/sesdf_oracle.cpp), and the restatement of the reference's Contour-Buildup
+comparator (oracle/cb_oracle.cpp), and returns numpy arrays.
 """
 import ctypes
 import os
@@ -130,6 +131,108 @@ class OracleResult:
             except (TypeError, AttributeError):  # interpreter shutdown
                 pass
             self._h = None
+
+
+CB_ARRAYS = {
+    "atoms": (np.float32, 4), "sorted_to_initial": (np.uint32, 0), "nb_count0": (np.uint32, 0),
+    "nb_count": (np.uint32, 0), "nb_ids": (np.uint32, 64), "removed": (np.uint8, 64), "arc_count": (np.uint8, 64),
+    "visible": (np.uint8, 64), "probe_pos": (np.float32, 4), "probe_pos64": (np.float64, 3),
+    "probe_ids": (np.uint32, 4), "tria1": (np.float32, 4), "tria2": (np.float32, 4), "tria3": (np.float32, 4),
+    "torus_pos": (np.float32, 4), "torus_axis": (np.float32, 4), "torus_vs": (np.float32, 4),
+    "torus_ids": (np.uint32, 2), "probe_sorted": (np.uint32, 0), "tex_coord": (np.float32, 3),
+    "sing_tex": (np.float32, 96), "amb_atom": (np.uint8, 0), "amb_cover": (np.uint8, 64), "amb_circle": (np.uint8, 64),
+    "amb_probe": (np.uint8, 0), "stats": (np.uint64, 0),
+}
+
+
+def _cb_lib():
+    L = lib()
+    if not getattr(L, "_cb_ready", False):
+        L.oracle_cb_build.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_float, ctypes.c_void_p, ctypes.c_uint32,
+                                      ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int]
+        L.oracle_cb_build.restype = ctypes.c_void_p
+        L.oracle_cb_singularities.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                                              ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_cb_singularities.restype = ctypes.c_int
+        L.oracle_cb_free.argtypes = [ctypes.c_void_p]
+        L.oracle_cb_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint32)]
+        L.oracle_cb_get.restype = ctypes.c_uint64
+        L.oracle_cb_stat_names.restype = ctypes.c_char_p
+        L._cb_ready = True
+    return L
+
+
+class CbResult:
+    """Arrays of one Contour-Buildup oracle build (grid-sorted atom index space, see sorted_to_initial)."""
+
+    def __init__(self, handle, n):
+        self._h, self.n = handle, n
+        self._cache = {}
+
+    def __getitem__(self, name):
+        if name not in self._cache:
+            L = _cb_lib()
+            dt, width = CB_ARRAYS[name]
+            es = ctypes.c_uint32(0)
+            cnt = L.oracle_cb_get(self._h, name.encode(), None, ctypes.byref(es))
+            arr = np.zeros(cnt, dt)
+            assert cnt == 0 or arr.itemsize == es.value, (name, arr.itemsize, es.value)
+            if cnt:
+                L.oracle_cb_get(self._h, name.encode(), ctypes.c_void_p(arr.ctypes.data), None)
+            if width:
+                arr = arr.reshape(-1, width)
+            self._cache[name] = arr
+        return self._cache[name]
+
+    def stats(self):
+        names = _cb_lib().oracle_cb_stat_names().decode().split()
+        return dict(zip(names, (int(v) for v in self["stats"])))
+
+    def singularities(self, probes):
+        """FP32 singularity texture of the given probe array [P, 4] on this build's grid:
+        (tex_coord [P, 3] in probe order, sing_tex [P, 96] in sorted probe order, sorted probe -> probe)."""
+        p = np.ascontiguousarray(probes, np.float32).reshape(-1, 4)
+        P = p.shape[0]
+        tc = np.zeros((P, 3), np.float32)
+        st = np.zeros((P, 96), np.float32)
+        order = np.zeros(P, np.uint32)
+        _cb_lib().oracle_cb_singularities(self._h, ctypes.c_void_p(p.ctypes.data), P, ctypes.c_void_p(tc.ctypes.data),
+                                          ctypes.c_void_p(st.ctypes.data), ctypes.c_void_p(order.ctypes.data))
+        return tc, st, order
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                _cb_lib().oracle_cb_free(self._h)
+            except (TypeError, AttributeError):  # interpreter shutdown
+                pass
+            self._h = None
+
+
+def aabb_of(atoms):
+    """The reference's getAabb: min / max of centre -/+ vdW radius (FP32)."""
+    a = np.asarray(atoms, np.float32).reshape(-1, 4)
+    lo = (a[:, :3] - a[:, 3:4]).min(axis=0)
+    hi = (a[:, :3] + a[:, 3:4]).max(axis=0)
+    return np.concatenate([lo, hi]).astype(np.float32)
+
+
+def cb_build(atoms, probe_radius=1.4, aabb=None, grid_dim=16, tex_size=32768, precision=64, track=True, delta=5e-5,
+             delta_theta=1e-5, threads=1):
+    """Contour-Buildup oracle build (bcs::ContourBuildup::build()). precision 32 mirrors the reference's
+    FP32 arithmetic; 64 is the accuracy oracle (with the epsilon-band tracking when track).
+
+    delta (A) bands the distance-like predicates, about ten times the FP32 rounding of coordinates of
+    tens of A; angle comparisons on a circle of radius w are banded by delta_theta + delta / w."""
+    a = np.ascontiguousarray(atoms, np.float32).reshape(-1, 4)
+    box = np.ascontiguousarray(aabb_of(a) if aabb is None else aabb, np.float32)
+    h = _cb_lib().oracle_cb_build(ctypes.c_void_p(a.ctypes.data), a.shape[0], probe_radius,
+                                  ctypes.c_void_p(box.ctypes.data), grid_dim, tex_size, precision, int(track), delta,
+                                  delta_theta, threads)
+    if not h:
+        raise ValueError("grid_dim must be a power of two and tex_size positive")
+    return CbResult(h, a.shape[0])
 
 
 def build(atoms, probe_radius=1.4, mode=0, precision=64, max_neighbors=128, track=True, delta=1e-3,

@@ -71,8 +71,52 @@ class BatchInfo(ctypes.Structure):
                 ("stream", vp), ("use_graph", i32)]
 
 
+class CbCreateInfo(ctypes.Structure):
+    _fields_ = [("xyzr", vp), ("atom_nb", u32), ("probe_radius", f32), ("aabb", f32 * 6), ("grid_dimensions", u32),
+                ("tex_size", u32), ("device", i32), ("stream", vp), ("use_graph", i32), ("build_surface", i32)]
+
+
+class CbData(ctypes.Structure):
+    _fields_ = [("atom_nb", u32), ("atoms", vp), ("probe_nb", u32), ("probe_positions", vp),
+                ("sphere_triangle_vec1", vp), ("sphere_triangle_vec2", vp), ("sphere_triangle_vec3", vp),
+                ("torus_nb", u32), ("torus_positions", vp), ("torus_axes", vp), ("torus_visibility_spheres", vp),
+                ("tex_coords", vp), ("singularity_texture", vp), ("probe_neighbor_nb", u32), ("tex_size", u32),
+                ("probe_radius", f32)]
+
+
+class CbCounts(ctypes.Structure):
+    _fields_ = [("atom_nb", u32), ("probe_nb", u32), ("torus_nb", u32), ("neighbor_overflow_atoms", u32),
+                ("no_contour_atoms", u32), ("arc_capacity", u32), ("build_attempts", u32), ("grid_dimensions", u32),
+                ("device_bytes", u64), ("build_serial", u64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class CbInternal(ctypes.Structure):
+    _fields_ = [("sorted_to_initial", vp), ("neighbor_count", vp), ("neighbor_ids", vp), ("max_neighbors", u32),
+                ("probe_atoms", vp), ("torus_atoms", vp), ("probe_sorted", vp)]
+
+
+class CbTuning(ctypes.Structure):
+    _fields_ = [("arc_capacity", u32), ("probe_capacity", u32), ("torus_capacity", u32)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "sesdf_cb_create": (i32, [ctypes.POINTER(CbCreateInfo), ctypes.POINTER(vp)]),
+    "sesdf_cb_destroy": (i32, [vp]),
+    "sesdf_cb_build": (i32, [vp]),
+    "sesdf_cb_build_device": (i32, [vp, vp]),
+    "sesdf_cb_set_atoms": (i32, [vp, vp]),
+    "sesdf_cb_input_buffer": (i32, [vp, ctypes.POINTER(vp)]),
+    "sesdf_cb_enqueue": (i32, [vp]),
+    "sesdf_cb_sync": (i32, [vp]),
+    "sesdf_cb_get_stream": (i32, [vp, ctypes.POINTER(vp)]),
+    "sesdf_cb_get_data": (i32, [vp, ctypes.POINTER(CbData)]),
+    "sesdf_cb_get_counts": (i32, [vp, ctypes.POINTER(CbCounts)]),
+    "sesdf_cb_get_internal": (i32, [vp, ctypes.POINTER(CbInternal)]),
+    "sesdf_cb_set_tuning": (i32, [vp, ctypes.POINTER(CbTuning)]),
     "sesdf_create": (i32, [ctypes.POINTER(CreateInfo), ctypes.POINTER(vp)]),
     "sesdf_destroy": (i32, [vp]),
     "sesdf_build": (i32, [vp]),

@@ -21,6 +21,7 @@ HIP_SOURCES = [
     "sdf.hip",
     "components.hip",
     "sort.hip",
+    "contour_buildup.hip",
     "mmtf.cpp",
     "capi.cpp",
 ]

@@ -42,7 +42,8 @@ def test_struct_layouts_match_header(tmp_path):
 
     names = {"sesdf_create_info": L.CreateInfo, "sesdf_data": L.SesdfData, "ssesdf_data": L.SsesdfData,
              "sesdf_grid_desc": L.GridDesc, "sesdf_counts": L.Counts, "sesdf_internal": L.Internal,
-             "sesdf_batch_info": L.BatchInfo}
+             "sesdf_batch_info": L.BatchInfo, "sesdf_cb_create_info": L.CbCreateInfo, "sesdf_cb_data": L.CbData,
+             "sesdf_cb_counts": L.CbCounts, "sesdf_cb_internal": L.CbInternal, "sesdf_cb_tuning": L.CbTuning}
     prog = tmp_path / "sizes.c"
     prog.write_text('#include <stdio.h>\n#include "sesdf_c.h"\nint main(void) {\n' +
                     "".join(f'    printf("%zu\\n", sizeof({n}));\n' for n in names) + "    return 0;\n}\n")
