@@ -14,6 +14,8 @@ replayed as one hipGraph. Further legs (same JSON line):
   grad       20,000-atom globule, 1M queries within |SDF| <= 2 A, loss = sum w_i SDF(q_i), w ~ N(0, 1):
              forward (build + sampling) vs backward (stored / recomputed winners; or the build
              released after the forward pass and built again in the backward pass);
+  cb         the reference's Contour-Buildup comparator against both SES paths on the 12 sample
+             proteins and the 100k globule (the reference's benchmark comparison);
   cpu        host C++ build of the same stage sequence (the FP32 oracle restatement) on this machine's
              cores for the workloads above (oracle/cpu_baseline.py); its predicate counts give the
              compute side of the roofline.
@@ -34,7 +36,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 STAGES = ["grid", "circles_kernel", "circles_offsets_emit", "intersections_kernel", "intersections_offsets_emit",
           "circle_list_scan", "fill", "segments", "probe_neighbors", "torus"]
-ALL_LEGS = "main,exterior,sdf,protein,large,batch,grad,cpu"
+ALL_LEGS = "main,exterior,sdf,protein,large,batch,grad,cb,cpu"
 
 
 def parse():
@@ -300,6 +302,57 @@ def leg_protein(args, out):
         log(f"7CGO {name}: {ms:.3f} ms/build")
         surf.close()
     out["protein_7CGO"] = res
+
+
+CB_CASES = ["1AGA", "101M", "1VIS", "7SC0", "3EAM", "7DBB", "1A8R", "7O0U", "1AON", "7RGD", "3JC8", "7CGO"]
+
+
+def time_cb(cb, steps, warmup):
+    """Mean device time (ms) of `steps` graph replays of a Contour-Buildup build, hipEvents on its stream."""
+    import sesdf_amd._lib as L
+
+    cb.build()
+    for _ in range(warmup):
+        cb.enqueue()
+    cb.sync()
+    e0, e1 = L.Event(), L.Event()
+    stream = cb.stream()
+    e0.record(stream)
+    for _ in range(steps):
+        cb.enqueue()
+    e1.record(stream)
+    e1.sync()
+    cb.sync()
+    return e0.elapsed_ms(e1) / steps
+
+
+def leg_cb(args, out, globule):
+    """The reference's benchmark comparison (app/benchmarks/main.cu:155-202): Contour-Buildup (16^3 grid,
+    the reference's constant), the exterior path and the complete surface, on the sample proteins present
+    (the first twelve cases of the reference's list) and the 100k globule; default launch parameters for
+    all three (no autotuning), each timed as graph replays on atoms resident in HBM."""
+    from sesdf_amd import ContourBuildup, Sesdf, Ssesdf, load
+
+    cases = [(c, load(os.path.join(ROOT, "tests", "golden", "mmtf", c + ".mmtf"))) for c in CB_CASES]
+    cases.append(("globule_100k", globule))
+    res = []
+    steps = max(10, args.steps // 2)
+    for name, atoms in cases:
+        row = {"case": name, "atoms": int(atoms.shape[0])}
+        cb = ContourBuildup(atoms, probe_radius=1.4, grid_dimensions=16)
+        row["cb_ms"] = time_cb(cb, steps, 3)
+        c = cb.counts()
+        row["cb_probes"], row["cb_tori"] = int(c["probe_nb"]), int(c["torus_nb"])
+        cb.close()
+        for key, cls in (("ssesdf_ms", Ssesdf), ("sesdf_ms", Sesdf)):
+            surf = cls(atoms, probe_radius=1.4, use_graph=True)
+            row[key] = time_builds(surf, steps, 3)
+            surf.close()
+        row["cb_over_ssesdf"] = row["cb_ms"] / row["ssesdf_ms"]
+        row["cb_over_sesdf"] = row["cb_ms"] / row["sesdf_ms"]
+        log(f"cb {name}: CB {row['cb_ms']:.3f} ms, SSESDF {row['ssesdf_ms']:.3f} ms, SESDF {row['sesdf_ms']:.3f} ms")
+        res.append(row)
+    out["contour_buildup"] = {"grid": "16^3", "probe_radius": 1.4, "cases": res}
 
 
 def leg_large(args, out):
@@ -585,6 +638,8 @@ def main():
         leg_batch(args, out)
     if "grad" in legs:
         leg_grad(args, out)
+    if "cb" in legs:
+        leg_cb(args, out, atoms)
     out["cpu_baseline"] = leg_cpu() if "cpu" in legs else None
     if out["cpu_baseline"] and "roofline_kernels" in out:
         out["roofline_compute"] = compute_roofline(out["cpu_baseline"]["workloads"]["c1"]["predicate_counts"], out)

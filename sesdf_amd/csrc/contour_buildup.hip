@@ -51,6 +51,7 @@ namespace sesdf
             OverflowProbes = 1u << 0,
             OverflowTori   = 1u << 1,
             OverflowArcs   = 1u << 2,
+            OverflowStage  = 1u << 3,
         };
 
         struct Ctl
@@ -64,7 +65,8 @@ namespace sesdf
             uint32_t overflow;    // CbOverflow bits
             uint32_t nbOverflow;  // atoms with more than 64 neighbours (lists truncated, as the reference)
             uint32_t noContour;   // atoms whose contour vanished
-            uint32_t pad[ 7 ];
+            uint32_t stageNeed;   // largest per-atom probe count beyond the staging stride
+            uint32_t pad[ 6 ];
         };
 
         struct Params
@@ -88,6 +90,18 @@ namespace sesdf
             return ( uint32_t( gz & m ) * uint32_t( p.dim ) + uint32_t( gy & m ) ) * uint32_t( p.dim ) + uint32_t( gx & m );
         }
 
+        // sqrtf(d2) < r, deciding without the square root when d2 is farther than 1e-6 (relative)
+        // from r^2: the correctly rounded root then lies on the same side of r. NaN compares false.
+        __device__ __forceinline__ bool closer( float d2, float r )
+        {
+            const float r2 = r * r;
+            if ( d2 < r2 * ( 1.0f - 1e-6f ) )
+                return true;
+            if ( !( d2 <= r2 * ( 1.0f + 1e-6f ) ) )
+                return false;
+            return sqrtf( d2 ) < r;
+        }
+
         __global__ void begin_kernel( Ctl * c )
         {
             uint32_t * w = reinterpret_cast<uint32_t *>( c );
@@ -95,13 +109,6 @@ namespace sesdf
                 w[ 0 ] = w[ 0 ] + 1; // epoch: tags this build's look-back flags
             else if ( threadIdx.x < sizeof( Ctl ) / 4 )
                 w[ threadIdx.x ] = 0;
-        }
-
-        __global__ void fill_kernel( uint32_t * p, uint32_t v, uint32_t n )
-        {
-            const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-            if ( i < n )
-                p[ i ] = v;
         }
 
         // Cell hash and index of every point (calcHashD, particles_kernel.cu:86-107). Slots past the
@@ -126,83 +133,102 @@ namespace sesdf
             vals[ i ] = i;
         }
 
-        // Cell ranges and the gather into sorted order (reorderDataAndFindCellStartD, :110-176).
-        __global__ void cells_kernel( const uint32_t * keys, const uint32_t * vals, const float4 * src, uint32_t cap,
-                                      const uint32_t * countPtr, uint32_t * cellStart, uint32_t * cellEnd, float4 * sorted,
-                                      uint32_t * sortedToInitial )
+        // Gather into sorted order (reorderDataAndFindCellStartD, :110-176).
+        __global__ void gather_kernel( const uint32_t * vals, const float4 * src, uint32_t cap, const uint32_t * countPtr,
+                                       float4 * sorted, uint32_t * sortedToInitial )
         {
             const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
             const uint32_t n = countPtr ? *countPtr : cap;
             if ( s >= n )
                 return;
-            const uint32_t h = keys[ s ];
-            if ( s == 0 || h != keys[ s - 1 ] )
-            {
-                cellStart[ h ] = s;
-                if ( s > 0 )
-                    cellEnd[ keys[ s - 1 ] ] = s;
-            }
-            if ( s == n - 1 )
-                cellEnd[ h ] = n;
-            const uint32_t v = vals[ s ];
-            sorted[ s ]      = src[ v ];
+            const uint32_t v     = vals[ s ];
+            sorted[ s ]          = src[ v ];
             sortedToInitial[ s ] = v;
+        }
+
+        // First sorted point of every cell, cells + 1 entries (a cell's points are
+        // [first[h], first[h + 1])): the reference's cellStart / cellEnd, without empty markers, so a
+        // run of consecutive cells is read with two loads.
+        __global__ void cell_first_kernel( const uint32_t * keys, uint32_t cap, const uint32_t * countPtr, uint32_t cells,
+                                           uint32_t * first )
+        {
+            const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+            if ( h > cells )
+                return;
+            uint32_t lo = 0, hi = countPtr ? *countPtr : cap; // first key >= h
+            while ( lo < hi )
+            {
+                const uint32_t mid = ( lo + hi ) >> 1;
+                if ( keys[ mid ] < h )
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            first[ h ] = lo;
         }
 
         struct AtomArgs
         {
             Ctl *            ctl;
-            const float4 *   atoms;     // sorted
-            const uint32_t * cellStart;
-            const uint32_t * cellEnd;
-            uint32_t *       nbCount;   // N: neighbours after the covered-circle pass
-            uint32_t *       nbIds;     // N * 64
-            float4 *         probePos;
-            float4 *         tria1;
-            float4 *         tria2;
-            float4 *         tria3;
-            uint4 *          probeIds;  // (i, j, k, 0) sorted atom indices
-            float4 *         torusPos;
-            float4 *         torusAxis;
-            float4 *         torusVs;
-            uint2 *          torusIds;
-            uint64_t *       flags;     // look-back: 2 columns of flagStride words
-            uint32_t         flagStride;
+            const float4 *   atoms;      // sorted
+            const uint32_t * cellFirst;  // cells + 1: first sorted atom of each cell (of any later cell if empty)
+            uint32_t *       nbCount;    // N: neighbours after the covered-circle pass
+            uint32_t *       nbIds;      // N * 64
+            uint32_t *       atomProbes; // N: probes written from atom i
+            uint32_t *       atomTori;   // N: tori of atom i
+            uint64_t *       visible;    // N: neighbour slots whose circle is a torus
+            float4 *         stage;      // N * stageStride: probe xyz, k | slot << 24
+            uint32_t         stageStride;
             uint32_t         n;
-            uint32_t         capProbes;
-            uint32_t         capTori;
             Params           prm;
         };
 
         template<int CAP>
         struct AtomLds
         {
-            float    vx[ K ], vy[ K ], vz[ K ];        // small circle of neighbour slot s: centre vector
-            float    nx[ K ], ny[ K ], nz[ K ];        // unit vector from the neighbour to atom i
-            float    vv[ K ];                          // |v|^2 of the small circle's centre vector
-            uint32_t id[ K ];                          // neighbour (sorted atom index)
+            float4   v[ K ];                           // small circle of slot s: centre vector, |v|^2
+            float4   nrm[ K ];                         // unit vector from the neighbour to atom i; w: the
+                                                       // neighbour's sorted index (bits)
             float    s[ CAP ][ K ], e[ CAP ][ K ];     // arcs of the lane's circle: angles
             uint32_t key[ CAP ][ K ];                  // start slot | end slot << 8 | valid << 16
         };
 
         __device__ __forceinline__ float fmod2pi( float x ) { return fmodf( x, TwoPiCb ); }
 
-        // One wave per atom (ticket order), lane s = neighbour slot s.
+        // Point of the line where the planes of two small circles meet that is closest to atom i's
+        // centre (each plane holds its circle's centre vector v and is normal to it):
+        // h = (vj (vj.(vj - vk)) |vk|^2 + vk ((vk - vj).vk) |vj|^2) / (|vj|^2 |vk|^2 - (vj.vk)^2)
+        // (particles_kernel.cu:1731-1732), with one hardware reciprocal of the denominator. The
+        // reference compiles with fast math (approximate division); either way the decisions that
+        // follow differ from exact arithmetic only inside the oracle's epsilon band.
+        __device__ __forceinline__ f3 plane_point( f3 vj, f3 vk, float vjvj, float vkvk )
+        {
+            const float vjvk = dot( vj, vk );
+            const float inv  = frcp( vjvj * vkvk - vjvk * vjvk );
+            const float c1   = dot( vj, vj - vk ) * vkvk * inv;
+            const float c2   = dot( vk - vj, vk ) * vjvj * inv;
+            return vj * c1 + vk * c2;
+        }
+
+        // One wave per atom (cell-sorted order), lane s = neighbour slot s.
         template<int CAP, int WAVES>
         __global__ void __launch_bounds__( WaveSize * WAVES ) atoms_kernel( AtomArgs a )
         {
             __shared__ AtomLds<CAP> ldsAll[ WAVES ];
             AtomLds<CAP> &           L    = ldsAll[ wave_in_block() ];
             const int                lane = lane_id();
-            const uint32_t           t    = wave_ticket( &a.ctl->ticket );
-            if ( t >= a.n )
+            // consecutive (cell-sorted) atoms per XCD: their neighbours share that XCD's L2
+            const uint32_t           i    = xcd_block() * uint32_t( WAVES ) + uint32_t( wave_in_block() );
+            if ( i >= a.n )
                 return;
-            const uint32_t i  = t;
             const float4   ai = a.atoms[ i ];
             const Params & p  = a.prm;
             const f3       pi = xyz( ai );
 
             // ---- neighbours and small circles (findNeighborsCBCuda) ----
+            // The stencil's cells of one (y, z) row are consecutive in the hash, so the row's atoms
+            // are one contiguous run of sorted indices; the runs of up to 64 rows are numbered as one
+            // stream (row order, then index: the reference's scan order) and tested 64 at a time.
             int gx, gy, gz;
             grid_pos( p, ai.x, ai.y, ai.z, gx, gy, gz );
             const float range = ai.w + 3.0f + 2.0f * p.rp;
@@ -210,58 +236,65 @@ namespace sesdf
             const int   x0 = max( gx - cx, 0 ), x1 = min( gx + cx, p.dim );
             const int   y0 = max( gy - cy, 0 ), y1 = min( gy + cy, p.dim );
             const int   z0 = max( gz - cz, 0 ), z1 = min( gz + cz, p.dim );
-            const int   ex = max( x1 - x0, 0 ), ey = max( y1 - y0, 0 ), ez = max( z1 - z0, 0 );
-            const uint32_t cells = uint32_t( ex ) * uint32_t( ey ) * uint32_t( ez );
+            const int   ey = max( y1 - y0, 0 ), ez = max( z1 - z0, 0 );
+            const uint32_t rows  = x1 > x0 ? uint32_t( ey ) * uint32_t( ez ) : 0u;
             const float    Ri    = ai.w + p.rp;
             uint32_t       count = 0;
             bool           full  = false;
-            for ( uint32_t f0 = 0; f0 < cells && !full; f0 += WaveSize )
+            for ( uint32_t r0 = 0; r0 < rows && !full; r0 += WaveSize )
             {
-                const uint32_t f  = f0 + uint32_t( lane );
-                uint32_t       cs = EmptyCell, ce = 0;
-                if ( f < cells )
+                const uint32_t r  = r0 + uint32_t( lane );
+                uint32_t       rs = 0, rn = 0;
+                if ( r < rows )
                 {
-                    const int x = x0 + int( f % uint32_t( ex ) ), y = y0 + int( ( f / uint32_t( ex ) ) % uint32_t( ey ) ),
-                              z = z0 + int( f / ( uint32_t( ex ) * uint32_t( ey ) ) );
-                    const uint32_t h = grid_hash( p, x, y, z );
-                    cs               = a.cellStart[ h ];
-                    if ( cs != EmptyCell )
-                        ce = a.cellEnd[ h ];
+                    const int y = y0 + int( r % uint32_t( ey ) ), z = z0 + int( r / uint32_t( ey ) );
+                    rs          = a.cellFirst[ grid_hash( p, x0, y, z ) ];
+                    rn          = a.cellFirst[ grid_hash( p, x1 - 1, y, z ) + 1 ] - rs;
                 }
-                uint64_t live = ballot( cs != EmptyCell );
-                while ( live && !full )
+                const uint32_t incl  = wave_incl_scan( rn );
+                const uint32_t total = wave_bcast( incl, WaveSize - 1 );
+                // stream position u -> sorted atom: the run holding u is the number of runs ending at
+                // or before it (binary search over the lanes' inclusive ends)
+                const uint32_t nr    = min( rows - r0, uint32_t( WaveSize ) );
+                auto           atom_at = [ & ]( uint32_t u ) {
+                    int q = 0;
+                    for ( int step = WaveSize / 2; step > 0; step >>= 1 )
+                        if ( uint32_t( step ) < nr * 2 && __shfl( incl, q + step - 1, WaveSize ) <= u )
+                            q += step;
+                    return __shfl( rs, q, WaveSize ) + ( u - ( __shfl( incl, q, WaveSize ) - __shfl( rn, q, WaveSize ) ) );
+                };
+                // loads of the next 64 candidates are issued before the current ones are tested
+                uint32_t jn = atom_at( uint32_t( lane ) );
+                float4   an = uint32_t( lane ) < total ? a.atoms[ jn ] : make_float4( 0.f, 0.f, 0.f, 0.f );
+                for ( uint32_t u0 = 0; u0 < total && !full; u0 += WaveSize )
                 {
-                    const int l = __ffsll( (unsigned long long)live ) - 1;
-                    live &= live - 1;
-                    const uint32_t s = wave_bcast( cs, l ), e = wave_bcast( ce, l );
-                    for ( uint32_t b = s; b < e && !full; b += WaveSize )
+                    const uint32_t j  = jn;
+                    const float4   aj = an;
+                    const uint32_t u  = u0 + uint32_t( lane );
+                    if ( u0 + WaveSize < total )
                     {
-                        const uint32_t j    = b + uint32_t( lane );
-                        bool           pass = false;
-                        float4         aj   = make_float4( 0.f, 0.f, 0.f, 0.f );
-                        f3             rel  = mk( 0.f, 0.f, 0.f );
-                        float          dist = 0.f;
-                        if ( j < e && j != i )
-                        {
-                            aj                = a.atoms[ j ];
-                            rel               = xyz( aj ) - pi;
-                            dist              = sqrtf( dot( rel, rel ) );
-                            const float nd    = ai.w + aj.w + 2.0f * p.rp;
-                            pass              = dist < nd;
-                        }
-                        const uint64_t m    = ballot( pass );
-                        const uint32_t slot = count + rank_below( m );
-                        if ( pass && slot < K )
-                            L.id[ slot ] = j;
-                        const uint32_t add = uint32_t( __popcll( m ) );
-                        if ( count + add > K )
-                        {
-                            full  = true;
-                            count = K;
-                        }
-                        else
-                            count += add;
+                        jn = atom_at( u + WaveSize );
+                        if ( u + WaveSize < total )
+                            an = a.atoms[ jn ];
                     }
+                    bool pass = false;
+                    if ( u < total && j != i )
+                    {
+                        const f3 rel = xyz( aj ) - pi;
+                        pass         = closer( dot( rel, rel ), ai.w + aj.w + 2.0f * p.rp );
+                    }
+                    const uint64_t m    = ballot( pass );
+                    const uint32_t slot = count + rank_below( m );
+                    if ( pass && slot < K )
+                        L.nrm[ slot ].w = __uint_as_float( j );
+                    const uint32_t add = uint32_t( __popcll( m ) );
+                    if ( count + add > K )
+                    {
+                        full  = true;
+                        count = K;
+                    }
+                    else
+                        count += add;
                 }
             }
             if ( full && lane == 0 )
@@ -276,7 +309,7 @@ namespace sesdf
             if ( act )
             {
                 // the small circle of slot `lane`, from the same FP32 expressions the search tested
-                j               = L.id[ lane ];
+                j               = __float_as_uint( L.nrm[ lane ].w );
                 const float4 aj = a.atoms[ j ];
                 const f3     rel  = xyz( aj ) - pi;
                 const float  dist = sqrtf( dot( rel, rel ) );
@@ -286,7 +319,6 @@ namespace sesdf
                 vj                = rel * r;
                 wj                = sqrtf( Ri * Ri - dot( vj, vj ) );
                 pj                = xyz( aj );
-                L.vx[ lane ] = vj.x, L.vy[ lane ] = vj.y, L.vz[ lane ] = vj.z;
                 a.nbIds[ size_t( i ) * K + lane ] = j;
             }
 
@@ -298,7 +330,10 @@ namespace sesdf
             const f3    nj   = act ? unit( pi - pj ) : mk( 0.f, 0.f, 0.f );
             const float vjvj = dot( vj, vj );
             if ( act )
-                L.nx[ lane ] = nj.x, L.ny[ lane ] = nj.y, L.nz[ lane ] = nj.z, L.vv[ lane ] = vjvj;
+            {
+                L.v[ lane ]   = make_float4( vj.x, vj.y, vj.z, vjvj );
+                L.nrm[ lane ] = make_float4( nj.x, nj.y, nj.z, __uint_as_float( j ) );
+            }
             // slots whose circle centre lies on the far side of atom i from their atom (the arc
             // pass swaps the two cut points for them)
             const uint64_t flipped = ballot( act && dot( vj, pj - pi ) < 0.0f );
@@ -309,17 +344,14 @@ namespace sesdf
             {
                 if ( !act || kk == uint32_t( lane ) )
                     continue;
-                const f3    vk    = mk( L.vx[ kk ], L.vy[ kk ], L.vz[ kk ] );
-                const float vkvk  = L.vv[ kk ];
-                const float vjvk  = dot( vj, vk );
-                const float denom = vjvj * vkvk - vjvk * vjvk;
-                const float c1    = dot( vj, vj - vk ) * vkvk;
-                const float c2    = dot( vk - vj, vk ) * vjvj;
-                const f3    hv    = mk( vj.x * c1 / denom, vj.y * c1 / denom, vj.z * c1 / denom )
-                                + mk( vk.x * c2 / denom, vk.y * c2 / denom, vk.z * c2 / denom );
-                const bool  far   = sqrtf( dot( hv, hv ) ) > Ri;
+                const float4 vv4  = L.v[ kk ];
+                const f3    vk    = mk( vv4.x, vv4.y, vv4.z );
+                const float vkvk  = vv4.w;
+                const f3    hv    = plane_point( vj, vk, vjvj, vkvk );
+                const bool  far   = dot( hv, hv ) > Ri * Ri;
                 meet |= uint64_t( !far ) << kk;
-                const f3    nk   = mk( L.nx[ kk ], L.ny[ kk ], L.nz[ kk ] );
+                const float4 nk4 = L.nrm[ kk ];
+                const f3    nk   = mk( nk4.x, nk4.y, nk4.z );
                 const f3    q    = vk - vj;
                 const float njnk = dot( nj, nk );
                 if ( njnk == 1.0f )
@@ -371,14 +403,10 @@ namespace sesdf
             {
                 const uint32_t kk = uint32_t( __ffsll( (unsigned long long)cutters ) - 1 );
                 cutters &= cutters - 1;
-                const f3    vk    = mk( L.vx[ kk ], L.vy[ kk ], L.vz[ kk ] );
-                const float vkvk  = L.vv[ kk ];
-                const float vjvk  = dot( vj, vk );
-                const float denom = vjvj * vkvk - vjvk * vjvk;
-                const float c1    = dot( vj, vj - vk ) * vkvk;
-                const float c2    = dot( vk - vj, vk ) * vjvj;
-                const f3    h     = mk( vj.x * c1 / denom, vj.y * c1 / denom, vj.z * c1 / denom )
-                               + mk( vk.x * c2 / denom, vk.y * c2 / denom, vk.z * c2 / denom );
+                const float4 vv4  = L.v[ kk ];
+                const f3    vk    = mk( vv4.x, vv4.y, vv4.z );
+                const float vkvk  = vv4.w;
+                const f3    h     = plane_point( vj, vk, vjvj, vkvk );
                 const f3    cr   = cross( vk, vj );
                 const float root = sqrtf( ( Ri * Ri - dot( h, h ) ) / dot( cr, cr ) );
                 f3          q1 = h + cr * root, q2 = h - cr * root;
@@ -535,7 +563,7 @@ namespace sesdf
                 }
             }
             // atom index of an arc end (slot of N(i), or the initial arc's 0)
-            auto endAtom = [ & ]( uint32_t slot ) { return slot == Sentinel ? 0u : L.id[ slot ]; };
+            auto endAtom = [ & ]( uint32_t slot ) { return slot == Sentinel ? 0u : __float_as_uint( L.nrm[ slot ].w ); };
             const bool lower = mine && !overflow && i < j;
             if ( lower )
                 for ( uint32_t q = 0; q < arcCnt; q++ )
@@ -545,40 +573,32 @@ namespace sesdf
                 }
             const bool visible = lower && arcCnt > 0;
 
-            // ---- offsets: ordered look-back over atoms ----
-            const uint64_t visMask  = ballot( visible );
-            const uint32_t local[ 2 ] = { wave_sum( written ), uint32_t( __popcll( visMask ) ) };
-            uint32_t       excl[ 2 ];
-            wave_lookback_joint<2>( a.flags, a.flagStride, t, a.ctl->epoch * 8u + 1u, local, excl );
-            if ( t == a.n - 1 && lane == 0 )
+            // ---- counts per atom; probes staged in the atom's row ----
+            const uint64_t visMask = ballot( visible );
+            const uint32_t total   = wave_sum( written );
+            const uint32_t before  = wave_incl_scan( written ) - written; // every lane takes part
+            if ( lane == 0 )
             {
-                const uint32_t P = excl[ 0 ] + local[ 0 ], T = excl[ 1 ] + local[ 1 ];
-                a.ctl->probeNeed = P, a.ctl->torusNeed = T;
-                a.ctl->probeNb   = min( P, a.capProbes );
-                a.ctl->torusNb   = min( T, a.capTori );
-                if ( P > a.capProbes )
-                    atomicOr( &a.ctl->overflow, uint32_t( OverflowProbes ) );
-                if ( T > a.capTori )
-                    atomicOr( &a.ctl->overflow, uint32_t( OverflowTori ) );
+                a.atomProbes[ i ] = total;
+                a.atomTori[ i ]   = uint32_t( __popcll( visMask ) );
+                a.visible[ i ]    = visMask;
+                if ( total > a.stageStride )
+                {
+                    atomicOr( &a.ctl->overflow, uint32_t( OverflowStage ) );
+                    atomicMax( &a.ctl->stageNeed, total );
+                }
             }
-
-            // ---- probes and spherical triangles (writeProbePositionsCBCuda) ----
-            const uint32_t before = wave_incl_scan( written ) - written; // every lane takes part
             if ( written )
             {
-                uint32_t o = excl[ 0 ] + before;
+                uint32_t r    = before;
                 auto     emit = [ & ]( float ang, uint32_t k ) {
-                    if ( o < a.capProbes )
+                    if ( r < a.stageStride )
                     {
-                        const f3     x  = ( pi + vj ) + ( xAxis * cosf( ang ) + yAxis * sinf( ang ) ) * wj;
-                        const float4 ak = a.atoms[ k ];
-                        a.probePos[ o ]  = make_float4( x.x, x.y, x.z, p.rp );
-                        a.tria1[ o ]     = make_float4( ai.x - x.x, ai.y - x.y, ai.z - x.z, 1.0f );
-                        a.tria2[ o ]     = make_float4( pj.x - x.x, pj.y - x.y, pj.z - x.z, 1.0f );
-                        a.tria3[ o ]     = make_float4( ak.x - x.x, ak.y - x.y, ak.z - x.z, p.rp * p.rp );
-                        a.probeIds[ o ]  = make_uint4( i, j, k, 0u );
+                        const f3 x = ( pi + vj ) + ( xAxis * cosf( ang ) + yAxis * sinf( ang ) ) * wj;
+                        a.stage[ size_t( i ) * a.stageStride + r ] =
+                            make_float4( x.x, x.y, x.z, __uint_as_float( k | ( uint32_t( lane ) << 24 ) ) );
                     }
-                    o++;
+                    r++;
                 };
                 for ( uint32_t q = 0; q < arcCnt; q++ )
                 {
@@ -590,28 +610,105 @@ namespace sesdf
                         emit( L.e[ q ][ lane ], ke );
                 }
             }
-            // ---- torus of the lane's circle ----
-            if ( visible )
+        }
+
+        // Probe count and capacity checks after the scan of per-atom counts.
+        __global__ void finish_counts_kernel( Ctl * c, uint32_t capProbes, uint32_t capTori )
+        {
+            if ( threadIdx.x != 0 )
+                return;
+            c->probeNb = min( c->probeNeed, capProbes );
+            c->torusNb = min( c->torusNeed, capTori );
+            if ( c->probeNeed > capProbes )
+                c->overflow |= OverflowProbes;
+            if ( c->torusNeed > capTori )
+                c->overflow |= OverflowTori;
+        }
+
+        struct EmitArgs
+        {
+            const float4 *   atoms;
+            const uint32_t * nbIds;
+            const uint32_t * atomProbes;
+            const uint32_t * probeOff;
+            const uint32_t * torusOff;
+            const uint64_t * visible;
+            const float4 *   stage;
+            uint32_t         stageStride;
+            float4 *         probePos;
+            float4 *         tria1;
+            float4 *         tria2;
+            float4 *         tria3;
+            uint4 *          probeIds;  // (i, j, k, 0) sorted atom indices
+            float4 *         torusPos;
+            float4 *         torusAxis;
+            float4 *         torusVs;
+            uint2 *          torusIds;
+            uint32_t         n;
+            uint32_t         capProbes;
+            uint32_t         capTori;
+            float            rp;
+        };
+
+        // Probes with their spherical-triangle vectors, and torus records (writeProbePositionsCBCuda,
+        // particles_kernel.cu:2179-2257), at the scanned offsets: one wave per atom, a lane per
+        // staged probe and per torus slot.
+        __global__ void __launch_bounds__( 256 ) emit_kernel( EmitArgs a )
+        {
+            const uint32_t i    = ( blockIdx.x * blockDim.x + threadIdx.x ) / WaveSize;
+            const int      lane = lane_id();
+            if ( i >= a.n )
+                return;
+            const float4   ai  = a.atoms[ i ];
+            const f3       pi  = xyz( ai );
+            const uint32_t cnt = min( a.atomProbes[ i ], a.stageStride );
+            const uint32_t off = a.probeOff[ i ];
+            for ( uint32_t r = uint32_t( lane ); r < cnt; r += WaveSize )
             {
-                const uint32_t o = excl[ 1 ] + rank_below( visMask );
-                if ( o < a.capTori )
-                {
-                    const f3    ta    = unit( vj );
-                    const f3    tc    = vj + pi;
-                    const f3    ortho = unit( cross( ta, mk( 0.f, 0.f, 1.f ) ) );
-                    const f3    P     = tc + ortho * wj;
-                    const f3    X     = unit( P - pi ) * ai.w;
-                    const float li = sqrtf( dot( P - pi, P - pi ) ), lj = sqrtf( dot( P - pj, P - pj ) );
-                    f3          C    = ( pj - pi ) * ( li / ( lj + li ) );
-                    const f3    XC   = X - C;
-                    const float dist = sqrtf( dot( XC, XC ) );
-                    C                = ( C + pi ) - tc;
-                    a.torusPos[ o ]  = make_float4( tc.x, tc.y, tc.z, wj );
-                    a.torusAxis[ o ] = make_float4( ta.x, ta.y, ta.z, p.rp );
-                    a.torusVs[ o ]   = make_float4( C.x, C.y, C.z, dist );
-                    a.torusIds[ o ]  = make_uint2( i, j );
-                }
+                const uint32_t o = off + r;
+                if ( o >= a.capProbes )
+                    break;
+                const float4   sp   = a.stage[ size_t( i ) * a.stageStride + r ];
+                const uint32_t bits = __float_as_uint( sp.w );
+                const uint32_t k = bits & 0xffffffu, j = a.nbIds[ size_t( i ) * K + ( bits >> 24 ) ];
+                const float4   aj = a.atoms[ j ], ak = a.atoms[ k ];
+                a.probePos[ o ] = make_float4( sp.x, sp.y, sp.z, a.rp );
+                a.tria1[ o ]    = make_float4( ai.x - sp.x, ai.y - sp.y, ai.z - sp.z, 1.0f );
+                a.tria2[ o ]    = make_float4( aj.x - sp.x, aj.y - sp.y, aj.z - sp.z, 1.0f );
+                a.tria3[ o ]    = make_float4( ak.x - sp.x, ak.y - sp.y, ak.z - sp.z, a.rp * a.rp );
+                a.probeIds[ o ] = make_uint4( i, j, k, 0u );
             }
+            const uint64_t vis = a.visible[ i ];
+            if ( !( ( vis >> lane ) & 1ull ) )
+                return;
+            const uint32_t o = a.torusOff[ i ] + rank_below( vis );
+            if ( o >= a.capTori )
+                return;
+            // the small circle of slot `lane` again, by the atom kernel's expressions
+            const uint32_t j    = a.nbIds[ size_t( i ) * K + lane ];
+            const float4   aj   = a.atoms[ j ];
+            const f3       pj   = xyz( aj );
+            const f3       rel  = pj - pi;
+            const float    dist = sqrtf( dot( rel, rel ) );
+            const float    Ri = ai.w + a.rp, Rj = aj.w + a.rp;
+            float          r  = Ri * Ri + dist * dist - Rj * Rj;
+            r                 = r / ( 2.0f * dist * dist );
+            const f3    vj    = rel * r;
+            const float wj    = sqrtf( Ri * Ri - dot( vj, vj ) );
+            const f3    ta    = unit( vj );
+            const f3    tc    = vj + pi;
+            const f3    ortho = unit( cross( ta, mk( 0.f, 0.f, 1.f ) ) );
+            const f3    P     = tc + ortho * wj;
+            const f3    X     = unit( P - pi ) * ai.w;
+            const float li = sqrtf( dot( P - pi, P - pi ) ), lj = sqrtf( dot( P - pj, P - pj ) );
+            f3          C    = ( pj - pi ) * ( li / ( lj + li ) );
+            const f3    XC   = X - C;
+            const float dd   = sqrtf( dot( XC, XC ) );
+            C                = ( C + pi ) - tc;
+            a.torusPos[ o ]  = make_float4( tc.x, tc.y, tc.z, wj );
+            a.torusAxis[ o ] = make_float4( ta.x, ta.y, ta.z, a.rp );
+            a.torusVs[ o ]   = make_float4( C.x, C.y, C.z, dd );
+            a.torusIds[ o ]  = make_uint2( i, j );
         }
     } // namespace cb
 } // namespace sesdf
@@ -620,52 +717,115 @@ namespace sesdf
 {
     namespace cb
     {
-        // Singularity texture (findProbeNeighborsCBCuda, particles_kernel.cu:2318-2379): thread per
-        // sorted probe; the first 32 other probes closer than 2 rp in scan order, texture coordinates
-        // of the probe's row.
-        __global__ void __launch_bounds__( 256 ) singularity_kernel( const float4 * sorted, const uint32_t * probeSorted,
-                                                                   const uint32_t * cellStart, const uint32_t * cellEnd,
-                                                                   const uint32_t * countPtr, Params p, float * texCoord,
-                                                                   float * singTex )
+        // Singularity texture (findProbeNeighborsCBCuda, particles_kernel.cu:2318-2379): per sorted
+        // probe, the first 32 other probes closer than 2 rp in scan order, and the texture
+        // coordinates of its row. Probes are sorted by cell, so a wave's 64 probes usually share one
+        // cell and stencil; the stencil's (y, z) rows are contiguous runs of sorted probes. The wave
+        // stages 64 candidates of a run per step in LDS, one per lane, and every lane tests them all
+        // in the scan order (LDS broadcast reads, eight at a time); a wave whose probes sit in two
+        // cells makes two such passes, and probes beyond those go lane by lane (waves spread over
+        // many small cells, as on small molecules). The distance test decides from the squared
+        // distance unless it lies within 1e-5 (relative) of (2 rp)^2, where it takes the reference's
+        // sqrtf(dx*dx + dy*dy + dz*dz) < 2 rp.
+        constexpr int SingWaves = 4;
+        __global__ void __launch_bounds__( WaveSize * SingWaves ) singularity_kernel( const float4 * sorted, const uint32_t * probeSorted,
+                                                                                    const uint32_t * cellFirst, const uint32_t * countPtr,
+                                                                                    Params p, float * texCoord, float * singTex )
         {
-            const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-            const uint32_t n = *countPtr;
-            if ( s >= n )
+            __shared__ float4 chunkAll[ SingWaves ][ WaveSize ];
+            float4 *          chunk = chunkAll[ wave_in_block() ];
+            const uint32_t    s     = blockIdx.x * blockDim.x + threadIdx.x;
+            const uint32_t    n     = *countPtr;
+            const bool        valid = s < n;
+            if ( ballot( valid ) == 0 )
                 return;
-            const float4 a = sorted[ s ];
+            const float4 a = valid ? sorted[ s ] : make_float4( 0.f, 0.f, 0.f, 0.f );
             int          gx, gy, gz;
             grid_pos( p, a.x, a.y, a.z, gx, gy, gz );
             const float range = p.rp + p.rp;
+            const float lo2 = range * range * ( 1.0f - 1e-5f ), hi2 = range * range * ( 1.0f + 1e-5f );
             const int   cx = int( ceilf( range / p.cx ) ), cy = int( ceilf( range / p.cy ) ), cz = int( ceilf( range / p.cz ) );
             uint32_t    count = 0;
             bool        full  = false;
-            for ( int z = max( gz - cz, 0 ); z < min( gz + cz, p.dim ) && !full; z++ )
-                for ( int y = max( gy - cy, 0 ); y < min( gy + cy, p.dim ) && !full; y++ )
-                    for ( int x = max( gx - cx, 0 ); x < min( gx + cx, p.dim ) && !full; x++ )
+            auto        near  = [ & ]( float4 b ) {
+                const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+                const float d2 = __builtin_fmaf( dz, dz, __builtin_fmaf( dy, dy, dx * dx ) );
+                if ( d2 < lo2 )
+                    return true;
+                if ( d2 > hi2 )
+                    return false;
+                return sqrtf( dx * dx + dy * dy + dz * dz ) < range; // the reference's test, exactly
+            };
+            auto record = [ & ]( float4 b ) {
+                if ( count >= ProbeK )
+                {
+                    full = true;
+                    return;
+                }
+                float * o = singTex + ( size_t( s ) * ProbeK + count ) * 3;
+                o[ 0 ] = b.x, o[ 1 ] = b.y, o[ 2 ] = b.z;
+                count++;
+            };
+            uint64_t todo = ballot( valid );
+            // at most two cooperative passes (the cells of the first probes still open)
+            for ( int pass = 0; todo && pass < 2; pass++ )
+            {
+                const int  l    = __ffsll( (unsigned long long)todo ) - 1;
+                const int  gx0 = wave_bcast( gx, l ), gy0 = wave_bcast( gy, l ), gz0 = wave_bcast( gz, l );
+                const bool mine = valid && gx == gx0 && gy == gy0 && gz == gz0;
+                todo &= ~ballot( mine );
+                const int x0 = max( gx0 - cx, 0 ), x1 = min( gx0 + cx, p.dim );
+                if ( x1 <= x0 )
+                    continue;
+                for ( int z = max( gz0 - cz, 0 ); z < min( gz0 + cz, p.dim ); z++ )
+                    for ( int y = max( gy0 - cy, 0 ); y < min( gy0 + cy, p.dim ); y++ )
                     {
-                        const uint32_t h  = grid_hash( p, x, y, z );
-                        const uint32_t st = cellStart[ h ];
-                        if ( st == EmptyCell )
-                            continue;
-                        const uint32_t en = cellEnd[ h ];
-                        for ( uint32_t t = st; t < en; t++ )
+                        const uint32_t st = cellFirst[ grid_hash( p, x0, y, z ) ];
+                        const uint32_t en = cellFirst[ grid_hash( p, x1 - 1, y, z ) + 1 ];
+                        for ( uint32_t base = st; base < en; base += WaveSize )
                         {
-                            if ( t == s )
-                                continue;
-                            const float4 b  = sorted[ t ];
-                            const float  dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
-                            if ( !( sqrtf( dx * dx + dy * dy + dz * dz ) < range ) )
-                                continue;
-                            if ( count >= ProbeK )
+                            const uint32_t t = base + uint32_t( lane_id() );
+                            chunk[ lane_id() ] = t < en ? sorted[ t ] : make_float4( 0.f, 0.f, 0.f, 0.f );
+                            wave_sync();
+                            const uint32_t m = min( en - base, uint32_t( WaveSize ) );
+                            for ( uint32_t q0 = 0; q0 < m; q0 += 8 )
                             {
-                                full = true;
-                                break;
+                                uint32_t hits = 0;
+#pragma unroll
+                                for ( uint32_t u = 0; u < 8; u++ )
+                                    if ( q0 + u < m && near( chunk[ q0 + u ] ) )
+                                        hits |= 1u << u;
+                                if ( !mine || full )
+                                    continue;
+                                while ( hits )
+                                {
+                                    const uint32_t u = uint32_t( __ffs( hits ) - 1 );
+                                    hits &= hits - 1;
+                                    if ( base + q0 + u != s && !full )
+                                        record( chunk[ q0 + u ] );
+                                }
                             }
-                            float * o = singTex + ( size_t( s ) * ProbeK + count ) * 3;
-                            o[ 0 ] = b.x, o[ 1 ] = b.y, o[ 2 ] = b.z;
-                            count++;
+                            wave_sync();
                         }
                     }
+            }
+            if ( ( todo >> lane_id() ) & 1ull )
+            {
+                const int x0 = max( gx - cx, 0 ), x1 = min( gx + cx, p.dim );
+                for ( int z = max( gz - cz, 0 ); z < min( gz + cz, p.dim ) && !full && x1 > x0; z++ )
+                    for ( int y = max( gy - cy, 0 ); y < min( gy + cy, p.dim ) && !full; y++ )
+                    {
+                        const uint32_t en = cellFirst[ grid_hash( p, x1 - 1, y, z ) + 1 ];
+                        for ( uint32_t t = cellFirst[ grid_hash( p, x0, y, z ) ]; t < en && !full; t++ )
+                        {
+                            const float4 b = sorted[ t ];
+                            if ( t != s && near( b ) )
+                                record( b );
+                        }
+                    }
+            }
+            if ( !valid )
+                return;
             float * tc = texCoord + size_t( probeSorted[ s ] ) * 3;
             tc[ 0 ] = float( count ), tc[ 1 ] = float( s / p.texSize ), tc[ 2 ] = float( s % p.texSize );
         }
@@ -690,8 +850,14 @@ namespace sesdf
             float4 *        input = nullptr, *atoms = nullptr;
             uint32_t *      sortedToInitial = nullptr, *nbCount = nullptr, *nbIds = nullptr;
             uint32_t *      keys0 = nullptr, *vals0 = nullptr, *keys1 = nullptr, *vals1 = nullptr, *hist = nullptr;
-            uint32_t *      cellStart = nullptr, *cellEnd = nullptr, *pCellStart = nullptr, *pCellEnd = nullptr;
-            uint64_t *      flags = nullptr;
+            uint32_t *      cellFirst = nullptr, *pCellFirst = nullptr; // cells + 1 each
+            uint32_t *      atomProbes = nullptr, *atomTori = nullptr, *probeOff = nullptr, *torusOff = nullptr;
+            uint64_t *      visible = nullptr;
+            float4 *        stage   = nullptr;
+            uint32_t        stageStride = 32;    // probes staged per atom (largest seen: 22)
+            Control *       scanCtl = nullptr;   // tickets and epoch of the count scan
+            uint64_t *      scanFlags = nullptr; // its look-back flags
+            uint32_t        scanFlagStride = 0;
             float4 *        probePos = nullptr, *tria1 = nullptr, *tria2 = nullptr, *tria3 = nullptr, *sortedProbes = nullptr;
             uint4 *         probeIds = nullptr;
             float4 *        torusPos = nullptr, *torusAxis = nullptr, *torusVs = nullptr;
@@ -764,11 +930,16 @@ namespace sesdf
             CB_ALLOC( fixed, h.sortedToInitial, h.n );
             CB_ALLOC( fixed, h.nbCount, h.n );
             CB_ALLOC( fixed, h.nbIds, size_t( h.n ) * K );
-            CB_ALLOC( fixed, h.cellStart, h.cells );
-            CB_ALLOC( fixed, h.cellEnd, h.cells );
-            CB_ALLOC( fixed, h.pCellStart, h.cells );
-            CB_ALLOC( fixed, h.pCellEnd, h.cells );
-            CB_ALLOC( fixed, h.flags, size_t( h.n ) * 2, true );
+            CB_ALLOC( fixed, h.cellFirst, size_t( h.cells ) + 1 );
+            CB_ALLOC( fixed, h.pCellFirst, size_t( h.cells ) + 1 );
+            CB_ALLOC( fixed, h.atomProbes, h.n );
+            CB_ALLOC( fixed, h.atomTori, h.n );
+            CB_ALLOC( fixed, h.probeOff, size_t( h.n ) + 1 );
+            CB_ALLOC( fixed, h.torusOff, size_t( h.n ) + 1 );
+            CB_ALLOC( fixed, h.visible, h.n );
+            CB_ALLOC( fixed, h.scanCtl, 1, true );
+            h.scanFlagStride = h.n / ScanMinTile + 64;
+            CB_ALLOC( fixed, h.scanFlags, size_t( h.scanFlagStride ) * 4, true );
             return SESDF_OK;
         }
 
@@ -790,6 +961,7 @@ namespace sesdf
             CB_ALLOC( out, h.keys1, h.capSort );
             CB_ALLOC( out, h.vals1, h.capSort );
             CB_ALLOC( out, h.hist, sort_hist_words( h.capSort ) );
+            CB_ALLOC( out, h.stage, size_t( h.n ) * h.stageStride );
             CB_ALLOC( out, h.probePos, h.capP );
             CB_ALLOC( out, h.tria1, h.capP );
             CB_ALLOC( out, h.tria2, h.capP );
@@ -816,29 +988,45 @@ namespace sesdf
             // atom grid: hash, stable sort by cell, cell ranges and the sorted atoms
             hash_kernel<<<blocks( h.n, 256 ), 256, 0, s>>>( h.input, h.n, nullptr, p, h.keys0, h.vals0 );
             const int w = radix_sort_pairs( h.keys0, h.vals0, h.keys1, h.vals1, h.hist, h.n, h.bits, s );
-            fill_kernel<<<blocks( h.cells, 256 ), 256, 0, s>>>( h.cellStart, EmptyCell, h.cells );
-            cells_kernel<<<blocks( h.n, 256 ), 256, 0, s>>>( w ? h.keys1 : h.keys0, w ? h.vals1 : h.vals0, h.input, h.n, nullptr,
-                                                             h.cellStart, h.cellEnd, h.atoms, h.sortedToInitial );
-            // neighbours, covered circles, arcs, probes and tori
+            cell_first_kernel<<<blocks( uint64_t( h.cells ) + 1, 256 ), 256, 0, s>>>( w ? h.keys1 : h.keys0, h.n, nullptr, h.cells,
+                                                                                    h.cellFirst );
+            gather_kernel<<<blocks( h.n, 256 ), 256, 0, s>>>( w ? h.vals1 : h.vals0, h.input, h.n, nullptr, h.atoms, h.sortedToInitial );
+            // neighbours, covered circles, arcs: counts and staged probes per atom
             AtomArgs a {};
-            a.ctl = h.ctl, a.atoms = h.atoms, a.cellStart = h.cellStart, a.cellEnd = h.cellEnd, a.nbCount = h.nbCount;
-            a.nbIds = h.nbIds, a.probePos = h.probePos, a.tria1 = h.tria1, a.tria2 = h.tria2, a.tria3 = h.tria3;
-            a.probeIds = h.probeIds, a.torusPos = h.torusPos, a.torusAxis = h.torusAxis, a.torusVs = h.torusVs;
-            a.torusIds = h.torusIds, a.flags = h.flags, a.flagStride = h.n, a.n = h.n, a.capProbes = h.capP;
-            a.capTori = h.capT, a.prm = p;
+            a.ctl = h.ctl, a.atoms = h.atoms, a.cellFirst = h.cellFirst, a.nbCount = h.nbCount;
+            a.nbIds = h.nbIds, a.atomProbes = h.atomProbes, a.atomTori = h.atomTori, a.visible = h.visible;
+            a.stage = h.stage, a.stageStride = h.stageStride, a.n = h.n, a.prm = p;
             if ( h.arcCap == ArcCapSmall )
                 atoms_kernel<ArcCapSmall, Waves><<<blocks( h.n, Waves ), WaveSize * Waves, 0, s>>>( a );
             else
                 atoms_kernel<ArcCapFull, 1><<<h.n, WaveSize, 0, s>>>( a );
+            // offsets of every atom's probes and tori (the reference's two exclusive scans)
+            Buffers sb {};
+            sb.ctl = h.scanCtl, sb.flags = h.scanFlags, sb.flagStride = h.scanFlagStride;
+            launch_begin( sb, s );
+            ScanArgs sa {};
+            sa.in[ 0 ] = h.atomProbes, sa.in[ 1 ] = h.atomTori;
+            sa.out[ 0 ] = h.probeOff, sa.out[ 1 ] = h.torusOff;
+            sa.totals[ 0 ] = &h.ctl->probeNeed, sa.totals[ 1 ] = &h.ctl->torusNeed;
+            sa.n = h.n, sa.ticket = 0, sa.stage = 1;
+            launch_scan<2>( sb, sa, s );
+            finish_counts_kernel<<<1, 64, 0, s>>>( h.ctl, h.capP, h.capT );
+            EmitArgs e {};
+            e.atoms = h.atoms, e.nbIds = h.nbIds, e.atomProbes = h.atomProbes, e.probeOff = h.probeOff;
+            e.torusOff = h.torusOff, e.visible = h.visible, e.stage = h.stage, e.stageStride = h.stageStride;
+            e.probePos = h.probePos, e.tria1 = h.tria1, e.tria2 = h.tria2, e.tria3 = h.tria3, e.probeIds = h.probeIds;
+            e.torusPos = h.torusPos, e.torusAxis = h.torusAxis, e.torusVs = h.torusVs, e.torusIds = h.torusIds;
+            e.n = h.n, e.capProbes = h.capP, e.capTori = h.capT, e.rp = p.rp;
+            emit_kernel<<<blocks( uint64_t( h.n ) * WaveSize, 256 ), 256, 0, s>>>( e );
             // probe grid on the same cells, singularity texture
             hash_kernel<<<blocks( h.capP, 256 ), 256, 0, s>>>( h.probePos, h.capP, &h.ctl->probeNb, p, h.keys0, h.vals0 );
             const int w2 = radix_sort_pairs( h.keys0, h.vals0, h.keys1, h.vals1, h.hist, h.capP, h.bits, s );
-            fill_kernel<<<blocks( h.cells, 256 ), 256, 0, s>>>( h.pCellStart, EmptyCell, h.cells );
-            cells_kernel<<<blocks( h.capP, 256 ), 256, 0, s>>>( w2 ? h.keys1 : h.keys0, w2 ? h.vals1 : h.vals0, h.probePos, h.capP,
-                                                                &h.ctl->probeNb, h.pCellStart, h.pCellEnd, h.sortedProbes,
-                                                                h.probeSorted );
-            singularity_kernel<<<blocks( h.capP, 256 ), 256, 0, s>>>( h.sortedProbes, h.probeSorted, h.pCellStart, h.pCellEnd,
-                                                                     &h.ctl->probeNb, p, h.texCoord, h.singTex );
+            cell_first_kernel<<<blocks( uint64_t( h.cells ) + 1, 256 ), 256, 0, s>>>( w2 ? h.keys1 : h.keys0, h.capP, &h.ctl->probeNb,
+                                                                                    h.cells, h.pCellFirst );
+            gather_kernel<<<blocks( h.capP, 256 ), 256, 0, s>>>( w2 ? h.vals1 : h.vals0, h.probePos, h.capP, &h.ctl->probeNb,
+                                                                 h.sortedProbes, h.probeSorted );
+            singularity_kernel<<<blocks( h.capP, WaveSize * SingWaves ), WaveSize * SingWaves, 0, s>>>(
+                h.sortedProbes, h.probeSorted, h.pCellFirst, &h.ctl->probeNb, p, h.texCoord, h.singTex );
         }
 
         static sesdf_status enqueue( Handle & h )
@@ -906,7 +1094,14 @@ namespace sesdf
                         h.graph = nullptr;
                     }
                 }
-                if ( ov & ( OverflowProbes | OverflowTori ) )
+                if ( ov & OverflowStage )
+                {
+                    uint32_t st2 = h.stageStride;
+                    while ( st2 < h.hctl->stageNeed )
+                        st2 *= 2;
+                    h.stageStride = st2;
+                }
+                if ( ov & ( OverflowProbes | OverflowTori | OverflowStage ) )
                 {
                     if ( ov & OverflowProbes )
                         h.capP = uint32_t( std::min<uint64_t>( 0xffffffffull / 8, uint64_t( h.hctl->probeNeed ) * 5 / 4 + 1024 ) );
