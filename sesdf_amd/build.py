@@ -81,7 +81,7 @@ def build_hip(jobs=4):
     return out
 
 
-CPP_PROGRAMS = ["host_api", "circles_census", "detailed_stages", "load_molecule"]
+CPP_PROGRAMS = ["host_api", "circles_census", "detailed_stages", "load_molecule", "contour_buildup"]
 
 
 def build_host_api():

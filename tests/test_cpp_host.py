@@ -116,3 +116,28 @@ def test_stage_entry_points_give_the_build(gpu, tmp_path, mode):
     assert got["same_counts"] is True and min(got["ms"]) >= 0
     assert got["concave"] > 0 and (got["segments"] > 0 if mode == "complete" else got["torus"] > 0)
     assert got["out_of_order_status"] == (1 if mode == "complete" else 3)  # SESDF_ERR_INVALID / _UNSUPPORTED
+
+
+def test_contour_buildup_header_compiles_cleanly(tmp_path):
+    """bcs::ContourBuildup compiles with a host compiler, warnings as errors, and links."""
+    out = tmp_path / "contour_buildup"
+    src = os.path.join(ROOT, "tests", "cpp", "contour_buildup.cpp")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           "-I", os.path.join(ROOT, "include"), src, "-o", str(out), "-L", os.path.join(ROOT, "sesdf_amd"), "-lsesdf_hip"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_contour_buildup_class_matches_python_path(gpu, tmp_path):
+    """bcs::ContourBuildup (constructed as app/benchmarks/main.cu:168 does, built twice, moved) gives
+    the counts of the Python mirror on the same atoms."""
+    from sesdf_amd import ContourBuildup
+
+    atoms = synth.globule(2000, synth.SEED_BASE)
+    got = _run_cpp("contour_buildup", atoms, tmp_path)
+    c = ContourBuildup(atoms).counts()
+    assert got["atoms"] == atoms.shape[0] and got["status"] == 0 and got["vao"] == 0
+    assert got["probes"] == got["data_probes"] == got["again_probes"] == c["probe_nb"]
+    assert got["tori"] == got["data_tori"] == got["again_tori"] == c["torus_nb"]
+    assert got["probe_neighbors"] == 32
