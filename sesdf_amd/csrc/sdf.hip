@@ -128,30 +128,41 @@ namespace sesdf
         else
             return v;
     }
+    // Wave-wide min / max by DPP, within the VALU: pairs and quads (quad permutes), then rows
+    // (shifts by 4 and 8), then row 15 / lane 31 broadcasts; the result gathers in lane 63 and is
+    // returned from there (uniform). A lane whose DPP source lies outside the pattern keeps its
+    // own value. Every lane of the wave must be active.
+    template<int Ctrl>
+    __device__ __forceinline__ int dpp_move( int v )
+    {
+        return __builtin_amdgcn_update_dpp( v, v, Ctrl, 0xf, 0xf, false );
+    }
+    template<class Op>
+    __device__ __forceinline__ int wave_reduce( int v, Op op )
+    {
+        v = op( v, dpp_move<0xb1>( v ) );  // quad_perm [1, 0, 3, 2]
+        v = op( v, dpp_move<0x4e>( v ) );  // quad_perm [2, 3, 0, 1]
+        v = op( v, dpp_move<0x114>( v ) ); // row_shr:4
+        v = op( v, dpp_move<0x118>( v ) ); // row_shr:8
+        v = op( v, dpp_move<0x142>( v ) ); // row_bcast:15
+        v = op( v, dpp_move<0x143>( v ) ); // row_bcast:31
+        return __builtin_amdgcn_readlane( v, WaveSize - 1 );
+    }
     __device__ __forceinline__ int wave_min_int( int v )
     {
-        for ( int o = 32; o > 0; o >>= 1 )
-            v = min( v, __shfl_xor( v, o, WaveSize ) );
-        return v;
+        return wave_reduce( v, []( int a, int b ) { return min( a, b ); } );
     }
     __device__ __forceinline__ int wave_max_int( int v )
     {
-        for ( int o = 32; o > 0; o >>= 1 )
-            v = max( v, __shfl_xor( v, o, WaveSize ) );
-        return v;
+        return wave_reduce( v, []( int a, int b ) { return max( a, b ); } );
     }
-
     __device__ __forceinline__ float wave_min_float( float v )
     {
-        for ( int o = 32; o > 0; o >>= 1 )
-            v = fminf( v, __shfl_xor( v, o, WaveSize ) );
-        return v;
+        return __int_as_float( wave_reduce( __float_as_int( v ), []( int a, int b ) { return __float_as_int( fminf( __int_as_float( a ), __int_as_float( b ) ) ); } ) );
     }
     __device__ __forceinline__ float wave_max_float( float v )
     {
-        for ( int o = 32; o > 0; o >>= 1 )
-            v = fmaxf( v, __shfl_xor( v, o, WaveSize ) );
-        return v;
+        return __int_as_float( wave_reduce( __float_as_int( v ), []( int a, int b ) { return __float_as_int( fmaxf( __int_as_float( a ), __int_as_float( b ) ) ); } ) );
     }
 
     // Bounding box of the wave's lanes that search surface elements (pass B).
@@ -377,6 +388,22 @@ namespace sesdf
     __device__ __forceinline__ uint32_t rl( uint32_t v, int j ) { return uint32_t( __builtin_amdgcn_readlane( int( v ), j ) ); }
     __device__ __forceinline__ float4   rl( float4 v, int j ) { return make_float4( rl( v.x, j ), rl( v.y, j ), rl( v.z, j ), rl( v.w, j ) ); }
     __device__ __forceinline__ int4     rl( int4 v, int j ) { return make_int4( rl( v.x, j ), rl( v.y, j ), rl( v.z, j ), rl( v.w, j ) ); }
+    __device__ __forceinline__ float    uniform_float( float v ) { return __int_as_float( __builtin_amdgcn_readfirstlane( __float_as_int( v ) ) ); }
+    // Value at a wave-uniform index, read through the constant address space (nothing the field
+    // kernels read is written while they run): a scalar load into SGPRs. The chunk's records then
+    // need not stay in VGPRs, one per lane, across the element and face loops, nor do the cell
+    // ranges, the molecule and the build's control values. At a per-lane index it is an ordinary
+    // vector load.
+    template<class T>
+    __device__ __forceinline__ T cload( const T * p, uint32_t i )
+    {
+#if defined( __HIP_DEVICE_COMPILE__ )
+        using ConstT = const __attribute__( ( address_space( 4 ) ) ) T;
+        return ( (ConstT *)( p ) )[ i ];
+#else
+        return p[ i ]; // host pass: never executed
+#endif
+    }
 
     constexpr int StageSize = 384; // float4 per wave: 64 segments (4 each) + 64 + 64 vertices, or caps
     constexpr int MaskRows  = 32;  // near-block rows whose element-gate masks are kept (2 chunks each)
@@ -429,12 +456,11 @@ namespace sesdf
             const uint32_t myExt  = mySt != 0 ? 1u : 0u;
             const uint64_t chkMask = ballot( mySt == 2 ); // atoms whose elements are checked one by one
             const int      cnt    = int( min( uint32_t( WaveSize ), k1 - base ) );
-            const uint32_t last   = min( k, k1 - 1 );
             if ( elems )
             {
                 // Chunk prefilter, one atom per lane: the shell |d - R| < best of some searching lane
                 // must meet the box of those lanes (largest best, nearest / farthest box point).
-                const float bestMax = wave_max_float( need ? L.best : 0.f );
+                const float bestMax = uniform_float( wave_max_float( need ? L.best : 0.f ) );
                 const f3    c       = xyz( myAtom );
                 const float nx = fmaxf( fmaxf( box.lo.x - c.x, c.x - box.hi.x ), 0.f ), ny = fmaxf( fmaxf( box.lo.y - c.y, c.y - box.hi.y ), 0.f ),
                             nz = fmaxf( fmaxf( box.lo.z - c.z, c.z - box.hi.z ), 0.f );
@@ -443,19 +469,18 @@ namespace sesdf
                 const float dNear = fsqrt( nx * nx + ny * ny + nz * nz ), dFar = fsqrt( fx * fx + fy * fy + fz * fz );
                 // 1e-4 A of slack covers the rounding of the two bounds
                 uint64_t pre = ballot( ok && myExt && dNear - myAtom.w < bestMax + 1e-4f && myAtom.w - dFar < bestMax + 1e-4f );
-                const int4 myR0 = s.surfRanges[ 2 * last ], myR1 = s.surfRanges[ 2 * last + 1 ];
                 // the chunk's atoms are consecutive in sorted order, so the records they own are
                 // contiguous; staged in LDS (up to 64 of each kind) on the first atom that needs them
-                const int  s0 = rl( myR0.z, 0 ), s1 = rl( myR0.w, cnt - 1 );
-                const int  w0 = rl( myR1.x, 0 ), w1 = rl( myR1.y, cnt - 1 );
-                const int  n0 = int( Iw ) + rl( myR1.z, 0 ), n1 = int( Iw ) + rl( myR1.w, cnt - 1 );
+                const int4 f0 = cload( s.surfRanges, 2 * base ), f1 = cload( s.surfRanges, 2 * base + 1 );
+                const int4 l0 = cload( s.surfRanges, 2 * ( base + cnt - 1 ) ), l1 = cload( s.surfRanges, 2 * ( base + cnt - 1 ) + 1 );
+                const int  s0 = f0.z, s1 = l0.w, w0 = f1.x, w1 = l1.y, n0 = int( Iw ) + f1.z, n1 = int( Iw ) + l1.w;
                 const bool segIn = s1 - s0 <= WaveSize, wIn = w1 - w0 <= WaveSize, nIn = n1 - n0 <= WaveSize;
                 bool       staged = false;
                 while ( pre )
                 {
                     const int j = __ffsll( (unsigned long long)pre ) - 1;
                     pre &= pre - 1;
-                    const float4 aa = rl( myAtom, j );
+                    const float4 aa = cload( s.surfAtoms, base + j );
                     if ( !any_lane<true>( need && fabsf( fsqrt( len2( L.q - xyz( aa ) ) ) - aa.w ) < L.best ) )
                         continue;
                     passMask |= uint64_t( 1 ) << j;
@@ -471,7 +496,7 @@ namespace sesdf
                         wave_sync();
                         staged = true;
                     }
-                    const int4 r0 = rl( myR0, j ), r1 = rl( myR1, j );
+                    const int4 r0 = cload( s.surfRanges, 2 * ( base + j ) ), r1 = cload( s.surfRanges, 2 * ( base + j ) + 1 );
                     const bool chk = ( chkMask >> j ) & 1;
                     for ( int f = r0.x; f < r0.y; f++ )
                         if ( !chk || s.comp.label[ f ] == extC )
@@ -509,16 +534,15 @@ namespace sesdf
                            ( uint64_t( uint32_t( __builtin_amdgcn_readfirstlane( int( uint32_t( m >> 32 ) ) ) ) ) << 32 );
                 }
                 cand &= ballot( ok && ( myId & SurfOpen ) && myExt );
-                const int2 myCaps = s.surfCaps[ last ];
-                const int  c0 = rl( myCaps.x, 0 ), c1 = rl( myCaps.y, cnt - 1 );
+                const int  c0 = cload( s.surfCaps, base ).x, c1 = cload( s.surfCaps, base + cnt - 1 ).y;
                 const bool capIn  = c1 - c0 <= StageSize;
                 bool       staged = false;
                 while ( cand )
                 {
                     const int j = __ffsll( (unsigned long long)cand ) - 1;
                     cand &= cand - 1;
-                    const uint32_t id      = rl( myId, j );
-                    const float4   aa      = rl( myAtom, j );
+                    const uint32_t id      = cload( s.surfIds, base + j );
+                    const float4   aa      = cload( s.surfAtoms, base + j );
                     const f3       dq      = L.q - xyz( aa );
                     const float    dist    = fsqrt( len2( dq ) );
                     const float    sd      = dist - aa.w;
@@ -534,7 +558,8 @@ namespace sesdf
                         staged = true;
                     }
                     const f3  u  = dq * frcp( dist );
-                    const int a0 = rl( myCaps.x, j ), a1 = rl( myCaps.y, j );
+                    const int2 cr = cload( s.surfCaps, base + j );
+                    const int  a0 = cr.x, a1 = cr.y;
                     // caps in groups of four (largest first): one exit test per group
                     for ( int c = a0; c < a1 && any_lane<true>( exposed ); c += 4 )
                     {
@@ -570,11 +595,11 @@ namespace sesdf
                                      uint64_t * masks )
     {
         const float    rp    = b.probeRadius;
-        const float    Rmax  = __uint_as_float( b.ctl->radiusMaxBits ) + rp;
+        const float    Rmax  = __uint_as_float( cload( &b.ctl->radiusMaxBits, 0 ) ) + rp;
         const float    D     = rp + s.band;
-        const Molecule M     = b.mols[ mol ];
-        const uint32_t Iw    = b.ctl->intersectionWithNbNb;
-        const uint32_t extC  = EXT ? s.comp.extComp[ 0 ] : 0u;
+        const Molecule M     = cload( b.mols, mol );
+        const uint32_t Iw    = cload( &b.ctl->intersectionWithNbNb, 0 );
+        const uint32_t extC  = EXT ? cload( s.comp.extComp, 0 ) : 0u;
         const int      reach = max( int( ceilf( ( D + Rmax + 1e-3f ) / M.cell ) ), 1 );
         int            gx, gy, gz;
         cell_of( M, L.q.x, L.q.y, L.q.z, gx, gy, gz );
@@ -613,7 +638,7 @@ namespace sesdf
             if ( !any_lane<COOP>( L.active && L.g >= 0.f && lb <= gCut ) )
                 return;
             const uint32_t h0 = cell_index( M, xa, y, z );
-            const uint32_t a0 = b.cellStart[ h0 ], a1 = b.cellStart[ h0 + uint32_t( xb - xa ) + 1 ];
+            const uint32_t a0 = cload( b.cellStart, h0 ), a1 = cload( b.cellStart, h0 + uint32_t( xb - xa ) + 1 );
             if constexpr ( COOP )
                 coop_nearest<EXT>( s, L, a0, a1, extC );
             else
@@ -656,16 +681,16 @@ namespace sesdf
             LaneBox box {};
             if constexpr ( COOP )
             {
-                box.lo = mk( wave_min_float( needB ? L.q.x : 3.0e38f ), wave_min_float( needB ? L.q.y : 3.0e38f ),
-                             wave_min_float( needB ? L.q.z : 3.0e38f ) );
-                box.hi = mk( wave_max_float( needB ? L.q.x : -3.0e38f ), wave_max_float( needB ? L.q.y : -3.0e38f ),
-                             wave_max_float( needB ? L.q.z : -3.0e38f ) );
+                box.lo = mk( uniform_float( wave_min_float( needB ? L.q.x : 3.0e38f ) ), uniform_float( wave_min_float( needB ? L.q.y : 3.0e38f ) ),
+                             uniform_float( wave_min_float( needB ? L.q.z : 3.0e38f ) ) );
+                box.hi = mk( uniform_float( wave_max_float( needB ? L.q.x : -3.0e38f ) ), uniform_float( wave_max_float( needB ? L.q.y : -3.0e38f ) ),
+                             uniform_float( wave_max_float( needB ? L.q.z : -3.0e38f ) ) );
             }
             auto surface = [ & ]( int xa, int xb, int y, int z, int ord, float dyz2, bool elems, bool faces ) {
                 if ( !any_lane<COOP>( needB && lower( xa, xb, dyz2 ) < L.best ) )
                     return;
                 const uint32_t h0 = cell_index( M, xa, y, z );
-                const uint32_t k0 = s.surfCellStart[ h0 ], k1 = s.surfCellStart[ h0 + uint32_t( xb - xa ) + 1 ];
+                const uint32_t k0 = cload( s.surfCellStart, h0 ), k1 = cload( s.surfCellStart, h0 + uint32_t( xb - xa ) + 1 );
                 // faces of a near-block row whose atoms all failed the element gate: nothing to test
                 if constexpr ( COOP )
                     if ( !elems && ord >= 0 && ord < MaskRows && k1 - k0 <= 2 * WaveSize && ( masks[ 2 * ord ] | masks[ 2 * ord + 1 ] ) == 0 )
@@ -908,7 +933,7 @@ namespace sesdf
             L.active = q < s.n;
             L.q      = L.active ? xyz( s.points[ q ] ) : mk( 0.f, 0.f, 0.f );
             int gx = 0, gy = 0, gz = 0;
-            const Molecule M = b.mols[ 0 ];
+            const Molecule M = cload( b.mols, 0 );
             if ( L.active )
                 cell_of( M, L.q.x, L.q.y, L.q.z, gx, gy, gz );
             constexpr int Big = 1 << 30;
@@ -920,7 +945,7 @@ namespace sesdf
             if ( !any_lane<true>( L.active ) )
                 continue;
             if ( ex <= 2 && ey <= 2 && ez <= 2 )
-                v = sdf_eval_lanes<true, EXT>( b, s, 0, L, code, stage[ threadIdx.x / WaveSize ], masks[ threadIdx.x / WaveSize ] );
+                v = sdf_eval_lanes<true, EXT>( b, s, 0, L, code, stage[ wave_in_block() ], masks[ wave_in_block() ] );
             else if ( L.active )
                 v = sdf_eval_lanes<false, EXT>( b, s, 0, L, code, nullptr, nullptr );
             if ( L.active )
@@ -955,32 +980,42 @@ namespace sesdf
     }
 
     // Regular grids: one wave per 4 x 4 x 4 brick of grid vertices (batches: per molecule grid).
-    // Latency-bound: five waves per SIMD (96 VGPRs, a few values spilled to scratch outside the
-    // inner loops) run about 7% faster than four (128 VGPRs); six spill into the loops.
+    // Latency-bound, so occupancy decides: the complete field runs six waves per SIMD (the LDS
+    // stage allows no more), the exterior field five; neither spills. Uniform values (brick,
+    // molecule, stencil, cell ranges, the chunk's records) live in SGPRs to keep VGPRs down.
     template<bool EXT>
-    __global__ void __launch_bounds__( 256, 5 ) sdf_grid_kernel( Buffers b, SdfArgs s, uint32_t molNb )
+    __global__ void __launch_bounds__( 256, EXT ? 5 : 6 ) sdf_grid_kernel( Buffers b, SdfArgs s, uint32_t molNb )
     {
         __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
         __shared__ uint64_t masks[ 256 / WaveSize ][ 2 * MaskRows ];
+        // bricks of 4 x 4 x 4 vertices, one per wave (the launch has a wave for every brick); the
+        // host refuses grids of more than 2^31 bricks (their output alone would exceed device
+        // memory), so brick indices stay within 32 bits
         const uint32_t nbx = ( s.dx + 3 ) / 4, nby = ( s.dy + 3 ) / 4, nbz = ( s.dz + 3 ) / 4;
-        const uint64_t bricks = uint64_t( nbx ) * nby * nbz, total = bricks * molNb;
+        const uint32_t bricks = nbx * nby * nbz, total = bricks * molNb;
         const uint64_t per    = uint64_t( s.dx ) * s.dy * s.dz;
         const int      lane   = lane_id();
-        const uint32_t wpb    = blockDim.x / WaveSize;
-        for ( uint64_t w = uint64_t( blockIdx.x ) * wpb + threadIdx.x / WaveSize; w < total; w += uint64_t( gridDim.x ) * wpb )
+        // the wave index is uniform (readfirstlane), so the brick, the molecule and its stencil
+        // are scalar values
+        const uint32_t w = blockIdx.x * ( blockDim.x / WaveSize ) + uint32_t( wave_in_block() );
+        if ( w < total )
         {
-            const uint32_t mol = uint32_t( w / bricks );
-            const uint64_t br  = w - uint64_t( mol ) * bricks;
-            const uint32_t ix  = uint32_t( br % nbx ) * 4 + ( lane & 3 );
-            const uint32_t iy  = uint32_t( ( br / nbx ) % nby ) * 4 + ( ( lane >> 2 ) & 3 );
-            const uint32_t iz  = uint32_t( br / ( uint64_t( nbx ) * nby ) ) * 4 + ( lane >> 4 );
+            const uint32_t mol = w / bricks, br = w - mol * bricks;
+            const uint32_t bxy = br % ( nbx * nby ), bz = br / ( nbx * nby );
+            const uint32_t ix = ( bxy % nbx ) * 4 + ( lane & 3 ), iy = ( bxy / nbx ) * 4 + ( ( lane >> 2 ) & 3 ), iz = bz * 4 + ( lane >> 4 );
             SdfLane        L;
-            L.active         = ix < s.dx && iy < s.dy && iz < s.dz;
+            L.active = ix < s.dx && iy < s.dy && iz < s.dz;
             const uint64_t q = uint64_t( mol ) * per + ( uint64_t( iz ) * s.dy + iy ) * s.dx + ix;
-            uint32_t       m;
-            L.q = query_point( s, L.active ? q : uint64_t( mol ) * per, m );
+            float          ox = s.ox, oy = s.oy, oz = s.oz, sx = s.sx, sy = s.sy, sz = s.sz;
+            if ( s.molGrids )
+            {
+                const float4 o = cload( s.molGrids, 2 * mol ), sp = cload( s.molGrids, 2 * mol + 1 );
+                ox = o.x, oy = o.y, oz = o.z, sx = sp.x, sy = sp.y, sz = sp.z;
+            }
+            // lanes past the grid's edge stand at the molecule's first vertex
+            L.q = L.active ? mk( ox + sx * float( ix ), oy + sy * float( iy ), oz + sz * float( iz ) ) : mk( ox, oy, oz );
             uint64_t    code;
-            const float v = sdf_eval_lanes<true, EXT>( b, s, mol, L, code, stage[ threadIdx.x / WaveSize ], masks[ threadIdx.x / WaveSize ] );
+            const float v = sdf_eval_lanes<true, EXT>( b, s, mol, L, code, stage[ wave_in_block() ], masks[ wave_in_block() ] );
             if ( L.active )
             {
                 s.out[ q ] = v;
@@ -1393,11 +1428,14 @@ namespace
 
     // One wave per brick: brick costs differ by two orders of magnitude (outside vs surface), so
     // the hardware dispatcher, not a static stride, balances them. Grids beyond 2^31 blocks stride.
-    uint32_t brick_blocks( const SdfArgs & s, uint32_t molNb )
+    uint64_t grid_bricks( const SdfArgs & s, uint32_t molNb )
     {
-        const uint64_t waves = uint64_t( ( s.dx + 3 ) / 4 ) * ( ( s.dy + 3 ) / 4 ) * ( ( s.dz + 3 ) / 4 ) * molNb;
-        return uint32_t( std::max<uint64_t>( 1, std::min<uint64_t>( ( waves + 3 ) / 4, 0x7fffffffull ) ) );
+        return uint64_t( ( s.dx + 3 ) / 4 ) * ( ( s.dy + 3 ) / 4 ) * ( ( s.dz + 3 ) / 4 ) * molNb;
     }
+    uint32_t brick_blocks( const SdfArgs & s, uint32_t molNb ) { return uint32_t( std::max<uint64_t>( 1, ( grid_bricks( s, molNb ) + 3 ) / 4 ) ); }
+    // The grid kernel counts bricks in 32 bits: at most 2^31 (the output of 2^31 bricks, 549 GB,
+    // already exceeds device memory).
+    constexpr uint64_t MaxGridBricks = 1ull << 31;
 
     // One thread per item (costs vary per query): dynamic dispatch balances; beyond 2^31 blocks the
     // kernels stride.
@@ -1624,6 +1662,8 @@ extern "C"
         s.n    = uint64_t( s.dx ) * s.dy * s.dz;
         s.band = band;
         s.out  = d_out;
+        if ( grid_bricks( s, 1 ) > MaxGridBricks )
+            return report_error( SESDF_ERR_INVALID, "sesdf_sample_grid: more than 2^31 bricks of 4^3 vertices" );
         sesdf_status st = sdf_common( h, s, flags );
         if ( st != SESDF_OK || s.n == 0 )
             return st;
@@ -1674,6 +1714,8 @@ extern "C"
         s.band     = band;
         s.out      = d_out;
         s.molGrids = h.sdf.molGrids;
+        if ( grid_bricks( s, uint32_t( M ) ) > MaxGridBricks )
+            return report_error( SESDF_ERR_INVALID, "sesdf_batch_sample_grids: more than 2^31 bricks of 4^3 vertices" );
         sesdf_status st = sdf_common( h, s, 0 );
         if ( st != SESDF_OK )
             return st;
