@@ -405,7 +405,14 @@ namespace sesdf
 #endif
     }
 
-    constexpr int StageSize = 384; // float4 per wave: 64 segments (4 each) + 64 + 64 vertices, or caps
+    // LDS stage of a wave (float4): a chunk's segments (4 each), then its W and N vertices, or the
+    // caps of a chunk's faces. 4 KB per wave (with the masks, 16 KB per block of four) leaves room
+    // for eight waves per SIMD; larger stages cost occupancy and ran slower.
+    constexpr int StageSegs  = 40;
+    constexpr int StageVerts = 32; // at most one per lane
+    constexpr int StageW     = 4 * StageSegs, StageN = StageW + StageVerts;
+    constexpr int StageSize  = StageN + StageVerts;
+    static_assert( StageVerts <= WaveSize, "vertices are staged one per lane" );
     constexpr int MaskRows  = 32;  // near-block rows whose element-gate masks are kept (2 chunks each)
 
     template<bool EXT>
@@ -474,7 +481,7 @@ namespace sesdf
                 const int4 f0 = cload( s.surfRanges, 2 * base ), f1 = cload( s.surfRanges, 2 * base + 1 );
                 const int4 l0 = cload( s.surfRanges, 2 * ( base + cnt - 1 ) ), l1 = cload( s.surfRanges, 2 * ( base + cnt - 1 ) + 1 );
                 const int  s0 = f0.z, s1 = l0.w, w0 = f1.x, w1 = l1.y, n0 = int( Iw ) + f1.z, n1 = int( Iw ) + l1.w;
-                const bool segIn = s1 - s0 <= WaveSize, wIn = w1 - w0 <= WaveSize, nIn = n1 - n0 <= WaveSize;
+                const bool segIn = s1 - s0 <= StageSegs, wIn = w1 - w0 <= StageVerts, nIn = n1 - n0 <= StageVerts;
                 bool       staged = false;
                 while ( pre )
                 {
@@ -490,9 +497,9 @@ namespace sesdf
                         for ( int i = int( lane ); segIn && i < 4 * ( s1 - s0 ); i += WaveSize )
                             stage[ i ] = s.segGeo[ 4 * s0 + i ];
                         if ( wIn && int( lane ) < w1 - w0 )
-                            stage[ 256 + lane ] = b.concavePos[ w0 + int( lane ) ];
+                            stage[ StageW + lane ] = b.concavePos[ w0 + int( lane ) ];
                         if ( nIn && int( lane ) < n1 - n0 )
-                            stage[ 320 + lane ] = b.concavePos[ n0 + int( lane ) ];
+                            stage[ StageN + lane ] = b.concavePos[ n0 + int( lane ) ];
                         wave_sync();
                         staged = true;
                     }
@@ -510,10 +517,10 @@ namespace sesdf
                     }
                     for ( int v = r1.x; v < r1.y; v++ )
                         if ( !chk || s.comp.label[ vertex_node( s.comp, uint32_t( v ) ) ] == extC )
-                            vertex_test( L, wIn ? stage[ 256 + v - w0 ] : b.concavePos[ v ], uint32_t( v ) );
+                            vertex_test( L, wIn ? stage[ StageW + v - w0 ] : b.concavePos[ v ], uint32_t( v ) );
                     for ( int v = int( Iw ) + r1.z; v < int( Iw ) + r1.w; v++ )
                         if ( !chk || s.comp.label[ vertex_node( s.comp, uint32_t( v ) ) ] == extC )
-                            vertex_test( L, nIn ? stage[ 320 + v - n0 ] : b.concavePos[ v ], uint32_t( v ) );
+                            vertex_test( L, nIn ? stage[ StageN + v - n0 ] : b.concavePos[ v ], uint32_t( v ) );
                 }
             }
             if ( elems && !faces && keepMask )
@@ -919,9 +926,11 @@ namespace sesdf
     }
 
     // Scattered points: a wave whose points fall in a compact block of cells is evaluated
-    // cooperatively, any other wave lane by lane.
+    // cooperatively, any other wave lane by lane. Occupancy: the per-lane path sets the registers
+    // (complete 79 VGPRs, six waves per SIMD; exterior 115, four); neither spills, and higher
+    // bounds that spill ran no faster.
     template<bool EXT>
-    __global__ void __launch_bounds__( 256, 5 ) sdf_points_kernel( Buffers b, SdfArgs s ) // five waves per SIMD, as the grids
+    __global__ void __launch_bounds__( 256, EXT ? 4 : 6 ) sdf_points_kernel( Buffers b, SdfArgs s )
     {
         __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
         __shared__ uint64_t masks[ 256 / WaveSize ][ 2 * MaskRows ];
@@ -980,11 +989,11 @@ namespace sesdf
     }
 
     // Regular grids: one wave per 4 x 4 x 4 brick of grid vertices (batches: per molecule grid).
-    // Latency-bound, so occupancy decides: the complete field runs six waves per SIMD (the LDS
-    // stage allows no more), the exterior field five; neither spills. Uniform values (brick,
-    // molecule, stencil, cell ranges, the chunk's records) live in SGPRs to keep VGPRs down.
+    // Latency-bound, so occupancy decides: the complete field runs eight waves per SIMD (51
+    // VGPRs), the exterior field five (94); neither spills. Uniform values (brick, molecule,
+    // stencil, cell ranges, the chunk's records) live in SGPRs to keep VGPRs down.
     template<bool EXT>
-    __global__ void __launch_bounds__( 256, EXT ? 5 : 6 ) sdf_grid_kernel( Buffers b, SdfArgs s, uint32_t molNb )
+    __global__ void __launch_bounds__( 256, EXT ? 5 : 8 ) sdf_grid_kernel( Buffers b, SdfArgs s, uint32_t molNb )
     {
         __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
         __shared__ uint64_t masks[ 256 / WaveSize ][ 2 * MaskRows ];
