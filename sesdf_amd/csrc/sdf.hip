@@ -754,80 +754,94 @@ namespace sesdf
         return true;
     }
 
-    // The cap of hidden circle (a, slot h) is not contained in any single visible cap of a (FP64:
-    // angle between the poles + its half angle <= the visible cap's half angle, by the cosine and
-    // sine of the sum). Almost every hidden cap lies inside one visible cap.
-    __device__ bool hidden_cap_open( const Buffers & b, uint32_t a, uint32_t h, d3 ca, double Ra )
+    // Containment of a hidden circle's cap (pole ph, cos ch, sin sh) in a visible cap (pole pv,
+    // cos cv), FP64: the angle between the poles plus the hidden half angle is at most the visible
+    // half angle (cosine and sine of the sum).
+    __device__ __forceinline__ bool cap_in_cap( d3 ph, double ch, double sh, d3 pv, double cv )
     {
-        d3     ph;
-        double ch, sh;
-        if ( !cap64( b, ca, Ra, a, h, ph, ch, sh ) )
-            return false;
-        const uint32_t na = b.nbCount[ a ];
-        for ( uint32_t l = 0; l < na; l++ )
-        {
-            if ( b.status[ size_t( a ) * b.K + l ] == 0 )
-                continue;
-            d3     pv;
-            double cv, sv;
-            if ( !cap64( b, ca, Ra, a, l, pv, cv, sv ) )
-                continue;
-            const double c = dot( ph, pv );
-            const d3     x = cross( ph, pv );
-            const double s = sqrt( dot( x, x ) );
-            // angle(ph, pv) + half angle of h, as cos and sin of the sum
-            const double cSum = c * ch - s * sh, sSum = s * ch + c * sh;
-            if ( sSum >= 0.0 && cSum >= cv + 1e-12 )
-                return false;
-        }
-        return true;
+        const double c    = dot( ph, pv );
+        const d3     x    = cross( ph, pv );
+        const double sn   = sqrt( dot( x, x ) );
+        const double cSum = c * ch - sn * sh, sSum = sn * ch + c * sh;
+        return sSum >= 0.0 && cSum >= cv + 1e-12;
     }
 
     // Per sorted atom: centre and R; surface flag (owns an arc or vertex, or its face may be
     // exposed) for the compaction scan. A face can be exposed only if the atom has a visible
     // circle, or no neighbour cuts a cap off its sphere: otherwise the exposed part would be
-    // bounded by arcs of visible circles. info: face may be exposed (bit 0); a hidden cap lies in
-    // no single visible cap, so the face test also asks the hidden neighbours' balls (bit 1).
-    // One wave per atom, lanes over its neighbours.
+    // bounded by arcs of visible circles. info: face may be exposed (bit 0); some hidden circle's
+    // cap lies in no single visible cap, so the face test also asks the hidden neighbours' balls
+    // (bit 1; almost every hidden cap lies inside one visible cap).
+    // One wave per atom, lanes over its neighbours. The hidden-cap test runs only for atoms whose
+    // face may be exposed (bit 1 means nothing otherwise): hidden circles 64 at a time, one per
+    // lane, against the visible circles' caps, computed 64 at a time into LDS, so a (hidden,
+    // visible) pair costs a broadcast read and no memory access.
+    struct CapF64
+    {
+        double x, y, z, c;
+    };
     __global__ void __launch_bounds__( 256 ) sdf_pack_kernel( Buffers b, float4 * pa, uint32_t * surfFlag, uint32_t * info )
     {
-        // four atoms per wave, 16 lanes each (an atom has about 45 neighbours: three rounds)
-        const uint32_t lane = uint32_t( lane_id() ), sub = lane & 15u, grp = lane >> 4;
-        const uint32_t a    = ( xcd_block() * 4 + threadIdx.x / WaveSize ) * 4 + grp;
-        const bool     live = a < b.n;
+        __shared__ CapF64 visAll[ 256 / WaveSize ][ WaveSize ];
+        CapF64 *          vis  = visAll[ wave_in_block() ];
+        const uint32_t    lane = uint32_t( lane_id() );
+        const uint32_t    a    = xcd_block() * ( 256 / WaveSize ) + uint32_t( wave_in_block() );
+        if ( a >= b.n )
+            return;
         // inside: the sphere lies in a neighbour's ball; cap: some neighbour robustly cuts a
         // non-empty cap (the caps of visible circles then bound any exposed part)
-        bool           inside = false, cap = false, hidden = false;
-        float4         at     = make_float4( 0.f, 0.f, 0.f, 0.f );
-        float          Ra     = 0.f;
-        if ( live )
+        const float4   at     = b.atoms[ a ];
+        const float    Ra     = at.w + b.probeRadius;
+        const uint32_t na     = b.nbCount[ a ];
+        bool           inside = false, cap = false;
+        for ( uint32_t l = lane; l < na; l += WaveSize )
         {
-            at                = b.atoms[ a ];
-            Ra                = at.w + b.probeRadius;
-            const uint32_t na = b.nbCount[ a ];
-            const d3       ca = mkd( xyz( at ) );
-            const double   Rd = double( at.w ) + double( b.probeRadius );
-            for ( uint32_t l = sub; l < na; l += 16 )
+            const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
+            const float  Rl = al.w + b.probeRadius;
+            const float  d  = sqrtf( len2( xyz( al ) - xyz( at ) ) );
+            inside          = inside || d + Ra <= Rl;
+            cap             = cap || ( d < Ra + Rl - 1e-3f && d + Rl > Ra + 1e-3f );
+        }
+        inside           = any_lane<true>( inside );
+        cap              = any_lane<true>( cap );
+        const bool open  = !inside && ( b.vStart[ a + 1 ] > b.vStart[ a ] || !cap );
+        bool       hidden = false;
+        if ( open )
+        {
+            const d3     ca = mkd( xyz( at ) );
+            const double Rd = double( at.w ) + double( b.probeRadius );
+            for ( uint32_t hb = 0; hb < na && !hidden; hb += WaveSize )
             {
-                const float4 al = b.atoms[ b.nbIds[ size_t( a ) * b.K + l ] ];
-                const float  Rl = al.w + b.probeRadius;
-                const float  d  = sqrtf( len2( xyz( al ) - xyz( at ) ) );
-                inside          = inside || d + Ra <= Rl;
-                cap             = cap || ( d < Ra + Rl - 1e-3f && d + Rl > Ra + 1e-3f );
-                if ( !hidden && b.status[ size_t( a ) * b.K + l ] == 0 )
-                    hidden = hidden_cap_open( b, a, l, ca, Rd );
+                // this lane's hidden circle stays "open" until a visible cap contains its cap
+                const uint32_t h = hb + lane;
+                d3             ph {};
+                double         ch = 0.0, sh = 0.0;
+                bool           openCap = h < na && b.status[ size_t( a ) * b.K + h ] == 0 && cap64( b, ca, Rd, a, h, ph, ch, sh );
+                for ( uint32_t vb = 0; vb < na && any_lane<true>( openCap ); vb += WaveSize )
+                {
+                    const uint32_t l = vb + lane;
+                    d3             pv {};
+                    double         cv = 0.0, sv = 0.0;
+                    const bool     ok = l < na && b.status[ size_t( a ) * b.K + l ] != 0 && cap64( b, ca, Rd, a, l, pv, cv, sv );
+                    wave_sync();
+                    if ( ok )
+                        vis[ lane ] = { pv.x, pv.y, pv.z, cv };
+                    wave_sync();
+                    for ( uint64_t vm = ballot( ok ); vm && any_lane<true>( openCap ); vm &= vm - 1 )
+                    {
+                        const CapF64 v = vis[ __ffsll( (unsigned long long)vm ) - 1 ];
+                        if ( openCap && cap_in_cap( ph, ch, sh, { v.x, v.y, v.z }, v.c ) )
+                            openCap = false;
+                    }
+                }
+                hidden = any_lane<true>( openCap );
             }
         }
-        const uint64_t mine = uint64_t( 0xffff ) << ( 16 * grp );
-        inside              = ( ballot( inside ) & mine ) != 0;
-        cap                 = ( ballot( cap ) & mine ) != 0;
-        hidden              = ( ballot( hidden ) & mine ) != 0;
-        if ( !live || sub != 0 )
+        if ( lane != 0 )
             return;
         pa[ a ]          = make_float4( at.x, at.y, at.z, Ra );
         const bool elems = b.fullBase[ a + 1 ] > b.fullBase[ a ] || b.segStart[ b.trimBase[ a + 1 ] ] > b.segStart[ b.trimBase[ a ] ] ||
                            b.wStart[ a + 1 ] > b.wStart[ a ] || b.nStart[ a + 1 ] > b.nStart[ a ];
-        const bool open  = !inside && ( b.vStart[ a + 1 ] > b.vStart[ a ] || !cap );
         surfFlag[ a ]    = ( elems || open ) ? 1u : 0u;
         info[ a ]        = ( open ? 1u : 0u ) | ( open && hidden ? 2u : 0u );
     }
@@ -1365,7 +1379,7 @@ namespace
             sdf_geometry_kernel<<<std::min<uint32_t>( ( geoItems + 255 ) / 256, uint32_t( device_cu_count() ) * 8 ), 256, 0, s>>>(
                 h.b, d.geo, d.geo + d.geoSectors );
         if ( n )
-            sdf_pack_kernel<<<( n + 15 ) / 16, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo );
+            sdf_pack_kernel<<<( n + 3 ) / 4, 256, 0, s>>>( h.b, d.packAtoms, d.surfFlag, d.surfInfo );
         ScanArgs a {};
         a.in[ 0 ]     = d.surfFlag;
         a.out[ 0 ]    = d.surfPrefix;
