@@ -421,6 +421,7 @@ typedef struct
     uint32_t arc_capacity;             /* arcs per circle held on chip (8, or 80 after an overflow) */
     uint32_t build_attempts;           /* builds run by the last call (capacity growth repeats it) */
     uint32_t grid_dimensions;
+    uint32_t singularity_grid;         /* 1: the singularity texture came from the 2 rp search grid */
     uint64_t device_bytes;
     uint64_t build_serial;
 } sesdf_cb_counts;
@@ -458,15 +459,18 @@ sesdf_status sesdf_cb_get_stream( sesdf_cb_handle * h, void ** stream );
 sesdf_status sesdf_cb_get_data( sesdf_cb_handle * h, sesdf_cb_data * out );
 sesdf_status sesdf_cb_get_counts( sesdf_cb_handle * h, sesdf_cb_counts * out );
 sesdf_status sesdf_cb_get_internal( sesdf_cb_handle * h, sesdf_cb_internal * out );
-/* Capacities of the next build (no reference counterpart: the reference sizes its buffers for
- * 64 probes per atom). arc_capacity: arcs per circle held on chip, 8 (default; wider lists build
- * again at 80) or 80; probe_capacity / torus_capacity: output slots (0 keeps the current ones; a
- * build that needs more grows them and runs again). */
+/* Capacities and search of the next build (no reference counterpart: the reference sizes its
+ * buffers for 64 probes per atom). arc_capacity: arcs per circle held on chip, 8 (default; wider
+ * lists build again at 80) or 80; probe_capacity / torus_capacity: output slots (0 keeps the
+ * current ones; a build that needs more grows them and runs again); singularity_search: how the
+ * singularity texture finds probe neighbours, 0 = by cell size (default), 1 = the reference's
+ * stencil scan, 2 = a grid of 2 rp cells (same texture). */
 typedef struct
 {
     uint32_t arc_capacity;
     uint32_t probe_capacity;
     uint32_t torus_capacity;
+    uint32_t singularity_search;
 } sesdf_cb_tuning;
 sesdf_status sesdf_cb_set_tuning( sesdf_cb_handle * h, const sesdf_cb_tuning * t );
 

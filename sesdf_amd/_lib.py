@@ -87,7 +87,7 @@ class CbData(ctypes.Structure):
 class CbCounts(ctypes.Structure):
     _fields_ = [("atom_nb", u32), ("probe_nb", u32), ("torus_nb", u32), ("neighbor_overflow_atoms", u32),
                 ("no_contour_atoms", u32), ("arc_capacity", u32), ("build_attempts", u32), ("grid_dimensions", u32),
-                ("device_bytes", u64), ("build_serial", u64)]
+                ("singularity_grid", u32), ("device_bytes", u64), ("build_serial", u64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -99,7 +99,7 @@ class CbInternal(ctypes.Structure):
 
 
 class CbTuning(ctypes.Structure):
-    _fields_ = [("arc_capacity", u32), ("probe_capacity", u32), ("torus_capacity", u32)]
+    _fields_ = [("arc_capacity", u32), ("probe_capacity", u32), ("torus_capacity", u32), ("singularity_search", u32)]
 
 
 # name -> (restype, argtypes)

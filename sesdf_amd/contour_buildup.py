@@ -79,9 +79,11 @@ class ContourBuildup:
         L.check(L.lib().sesdf_cb_get_stream(self._h, ctypes.byref(s)), "sesdf_cb_get_stream")
         return s.value
 
-    def set_tuning(self, arc_capacity=0, probe_capacity=0, torus_capacity=0):
-        """Capacities of the next build (arcs per circle on chip: 8 or 80; output slots)."""
-        t = L.CbTuning(arc_capacity, probe_capacity, torus_capacity)
+    def set_tuning(self, arc_capacity=0, probe_capacity=0, torus_capacity=0, singularity_search=0):
+        """Capacities of the next build (arcs per circle on chip: 8 or 80; output slots) and the
+        singularity texture's neighbour search (0 by cell size, 1 the reference's stencil scan,
+        2 a grid of 2 rp cells; the texture is the same)."""
+        t = L.CbTuning(arc_capacity, probe_capacity, torus_capacity, singularity_search)
         L.check(L.lib().sesdf_cb_set_tuning(self._h, ctypes.byref(t)), "sesdf_cb_set_tuning")
 
     def counts(self):

@@ -830,6 +830,106 @@ namespace sesdf
             tc[ 0 ] = float( count ), tc[ 1 ] = float( s / p.texSize ), tc[ 2 ] = float( s % p.texSize );
         }
 
+        // Search grid of the singularity texture when the reference's cells are large: cells of at
+        // least 2 rp over the aabb grown by 10 A (coordinates clamped into it), linear index.
+        struct FineGrid
+        {
+            float ox, oy, oz, cell;
+            int   dx, dy, dz;
+        };
+        __device__ __forceinline__ int fine_axis( float v, float o, float cell, int d )
+        {
+            return int( fminf( fmaxf( floorf( ( v - o ) / cell ), 0.f ), float( d - 1 ) ) );
+        }
+        __global__ void fine_hash_kernel( const float4 * pts, uint32_t cap, const uint32_t * countPtr, FineGrid g, uint32_t * keys,
+                                          uint32_t * vals )
+        {
+            const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+            if ( i >= cap )
+                return;
+            uint32_t h = 0xffffffffu; // past the live count: sorted after every live point
+            if ( i < *countPtr )
+            {
+                const float4 a = pts[ i ];
+                h = ( uint32_t( fine_axis( a.z, g.oz, g.cell, g.dz ) ) * uint32_t( g.dy ) + uint32_t( fine_axis( a.y, g.oy, g.cell, g.dy ) ) ) *
+                        uint32_t( g.dx ) +
+                    uint32_t( fine_axis( a.x, g.ox, g.cell, g.dx ) );
+            }
+            keys[ i ] = h;
+            vals[ i ] = i;
+        }
+
+        // Singularity texture through the search grid, row for row what singularity_kernel writes.
+        // The reference scans its stencil's cells in hash order, which is the order of the sorted
+        // probes, so it records the first 32 (by sorted index) of the probes within 2 rp that lie in
+        // its stencil's cells. The 27 search cells around a probe hold every probe within 2 rp (cells
+        // of at least 2 rp; clamping moves no two points further apart in cells); of those, the ones
+        // whose reference cell (the same wrapped hash) is in the stencil are kept, and an insertion
+        // list in LDS (transposed, one column per lane) holds the 32 smallest sorted indices in order.
+        // The distance test is singularity_kernel's.
+        constexpr int FineWaves = 4;
+        __global__ void __launch_bounds__( WaveSize * FineWaves ) singularity_fine_kernel( const float4 * sorted, const uint32_t * probeSorted,
+                                                                                         const float4 * finePts, const uint32_t * fineT,
+                                                                                         const uint32_t * fineFirst, const uint32_t * countPtr,
+                                                                                         Params p, FineGrid g, float * texCoord, float * singTex )
+        {
+            __shared__ uint32_t lists[ FineWaves ][ ProbeK ][ WaveSize ];
+            uint32_t( *lst )[ WaveSize ] = lists[ wave_in_block() ];
+            const uint32_t s             = blockIdx.x * blockDim.x + threadIdx.x;
+            if ( s >= *countPtr )
+                return;
+            const int    lane = lane_id();
+            const float4 a    = sorted[ s ];
+            int          gx, gy, gz;
+            grid_pos( p, a.x, a.y, a.z, gx, gy, gz );
+            const float range = p.rp + p.rp;
+            const float lo2 = range * range * ( 1.0f - 1e-5f ), hi2 = range * range * ( 1.0f + 1e-5f );
+            const int   cx = int( ceilf( range / p.cx ) ), cy = int( ceilf( range / p.cy ) ), cz = int( ceilf( range / p.cz ) );
+            const int   x0 = max( gx - cx, 0 ), x1 = min( gx + cx, p.dim ), y0 = max( gy - cy, 0 ), y1 = min( gy + cy, p.dim );
+            const int   z0 = max( gz - cz, 0 ), z1 = min( gz + cz, p.dim ), m = p.dim - 1;
+            uint32_t    count = 0;
+            if ( x1 > x0 && y1 > y0 && z1 > z0 )
+            {
+                const int fx = fine_axis( a.x, g.ox, g.cell, g.dx ), fy = fine_axis( a.y, g.oy, g.cell, g.dy ), fz = fine_axis( a.z, g.oz, g.cell, g.dz );
+                const int xa = max( fx - 1, 0 ), xb = min( fx + 1, g.dx - 1 );
+                for ( int z = max( fz - 1, 0 ); z <= min( fz + 1, g.dz - 1 ); z++ )
+                    for ( int y = max( fy - 1, 0 ); y <= min( fy + 1, g.dy - 1 ); y++ )
+                    {
+                        const uint32_t row = ( uint32_t( z ) * uint32_t( g.dy ) + uint32_t( y ) ) * uint32_t( g.dx );
+                        const uint32_t en  = fineFirst[ row + uint32_t( xb ) + 1 ];
+                        for ( uint32_t u = fineFirst[ row + uint32_t( xa ) ]; u < en; u++ )
+                        {
+                            const uint32_t t = fineT[ u ];
+                            if ( t == s || ( count == ProbeK && t > lst[ ProbeK - 1 ][ lane ] ) )
+                                continue;
+                            const float4 b  = finePts[ u ];
+                            const float  dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+                            const float  d2 = __builtin_fmaf( dz, dz, __builtin_fmaf( dy, dy, dx * dx ) );
+                            if ( !( d2 < lo2 ) && ( d2 > hi2 || !( sqrtf( dx * dx + dy * dy + dz * dz ) < range ) ) )
+                                continue;
+                            int bx, by, bz;
+                            grid_pos( p, b.x, b.y, b.z, bx, by, bz );
+                            bx &= m, by &= m, bz &= m;
+                            if ( bx < x0 || bx >= x1 || by < y0 || by >= y1 || bz < z0 || bz >= z1 )
+                                continue;
+                            int j = int( min( count, ProbeK - 1 ) );
+                            for ( ; j > 0 && lst[ j - 1 ][ lane ] > t; j-- )
+                                lst[ j ][ lane ] = lst[ j - 1 ][ lane ];
+                            lst[ j ][ lane ] = t;
+                            count            = min( count + 1, ProbeK );
+                        }
+                    }
+                for ( uint32_t j = 0; j < count; j++ )
+                {
+                    const float4 b = sorted[ lst[ j ][ lane ] ];
+                    float *      o = singTex + ( size_t( s ) * ProbeK + j ) * 3;
+                    o[ 0 ] = b.x, o[ 1 ] = b.y, o[ 2 ] = b.z;
+                }
+            }
+            float * tc = texCoord + size_t( probeSorted[ s ] ) * 3;
+            tc[ 0 ] = float( count ), tc[ 1 ] = float( s / p.texSize ), tc[ 2 ] = float( s % p.texSize );
+        }
+
         constexpr int Waves = 4; // waves per workgroup of the atom kernel (small arc lists)
 
         struct Handle
@@ -864,6 +964,14 @@ namespace sesdf
             uint2 *         torusIds = nullptr;
             uint32_t *      probeSorted = nullptr;
             float *         texCoord = nullptr, *singTex = nullptr;
+            // singularity search: 0 by cell size, 1 the reference's stencil scan, 2 the search grid
+            int             singSearch = 0;
+            bool            fineOn     = false, fineBig = false;
+            FineGrid        fine {};
+            uint32_t        fineCells = 0;
+            int             fineBits  = 0;
+            uint32_t *      fineFirst = nullptr, *fineT = nullptr; // fineCells + 1; capP
+            float4 *        finePts   = nullptr;                   // capP
             std::vector<void *> fixedAllocs, outAllocs;
             uint64_t        fixedBytes = 0, outBytes = 0;
             hipGraphExec_t  graph     = nullptr;
@@ -932,6 +1040,7 @@ namespace sesdf
             CB_ALLOC( fixed, h.nbIds, size_t( h.n ) * K );
             CB_ALLOC( fixed, h.cellFirst, size_t( h.cells ) + 1 );
             CB_ALLOC( fixed, h.pCellFirst, size_t( h.cells ) + 1 );
+            CB_ALLOC( fixed, h.fineFirst, size_t( h.fineCells ) + 1 );
             CB_ALLOC( fixed, h.atomProbes, h.n );
             CB_ALLOC( fixed, h.atomTori, h.n );
             CB_ALLOC( fixed, h.probeOff, size_t( h.n ) + 1 );
@@ -971,11 +1080,48 @@ namespace sesdf
             CB_ALLOC( out, h.probeSorted, h.capP );
             CB_ALLOC( out, h.texCoord, size_t( h.capP ) * 3 );
             CB_ALLOC( out, h.singTex, size_t( h.capP ) * ProbeK * 3 );
+            CB_ALLOC( out, h.finePts, h.capP );
+            CB_ALLOC( out, h.fineT, h.capP );
             CB_ALLOC( out, h.torusPos, h.capT );
             CB_ALLOC( out, h.torusAxis, h.capT );
             CB_ALLOC( out, h.torusVs, h.capT );
             CB_ALLOC( out, h.torusIds, h.capT );
             return SESDF_OK;
+        }
+
+        // Search grid of the singularity texture: cells of 2 rp (0.1% over, and 1e-3 A, so that float
+        // rounding never puts two probes within 2 rp two cells apart) over the aabb grown by 10 A, at
+        // most 2^24 cells (larger cells past that). It replaces the reference's stencil scan when
+        // that stencil holds over four times the volume of the 27 search cells.
+        static void plan_fine_grid( Handle & h, const float * box )
+        {
+            const Params & p   = h.prm;
+            const float    pad = 10.f;
+            float          ext[ 3 ];
+            for ( int q = 0; q < 3; q++ )
+                ext[ q ] = box[ 3 + q ] - box[ q ] + 2.f * pad;
+            float cell = std::max( 2.f * p.rp * 1.001f + 1e-3f, 0.1f );
+            int   d[ 3 ];
+            for ( ;; )
+            {
+                uint64_t cells = 1;
+                for ( int q = 0; q < 3; q++ )
+                    d[ q ] = std::max( 1, int( std::ceil( ext[ q ] / cell ) ) ), cells *= uint64_t( d[ q ] );
+                if ( cells <= ( 1ull << 24 ) )
+                    break;
+                cell *= 1.25f;
+            }
+            h.fine      = { box[ 0 ] - pad, box[ 1 ] - pad, box[ 2 ] - pad, cell, d[ 0 ], d[ 1 ], d[ 2 ] };
+            h.fineCells = uint32_t( d[ 0 ] ) * uint32_t( d[ 1 ] ) * uint32_t( d[ 2 ] );
+            h.fineBits  = 0;
+            while ( ( 1ull << h.fineBits ) < h.fineCells )
+                h.fineBits++;
+            const float range = 2.f * p.rp, cs[ 3 ] = { p.cx, p.cy, p.cz };
+            double      stencil = 1.0;
+            for ( int q = 0; q < 3; q++ )
+                stencil *= cs[ q ] > 0.f ? 2.0 * std::ceil( range / cs[ q ] ) * cs[ q ] : 0.0;
+            h.fineBig = stencil > 4.0 * 27.0 * double( cell ) * cell * cell;
+            h.fineOn  = h.singSearch == 2 || ( h.singSearch == 0 && h.fineBig );
         }
 
         static uint32_t blocks( uint64_t n, uint32_t per ) { return uint32_t( std::max<uint64_t>( 1, ( n + per - 1 ) / per ) ); }
@@ -1025,8 +1171,21 @@ namespace sesdf
                                                                                     h.cells, h.pCellFirst );
             gather_kernel<<<blocks( h.capP, 256 ), 256, 0, s>>>( w2 ? h.vals1 : h.vals0, h.probePos, h.capP, &h.ctl->probeNb,
                                                                  h.sortedProbes, h.probeSorted );
-            singularity_kernel<<<blocks( h.capP, WaveSize * SingWaves ), WaveSize * SingWaves, 0, s>>>(
-                h.sortedProbes, h.probeSorted, h.pCellFirst, &h.ctl->probeNb, p, h.texCoord, h.singTex );
+            if ( h.fineOn )
+            {
+                // sorted probes into the search grid (values: sorted indices), then the texture
+                fine_hash_kernel<<<blocks( h.capP, 256 ), 256, 0, s>>>( h.sortedProbes, h.capP, &h.ctl->probeNb, h.fine, h.keys0, h.vals0 );
+                const int w3 = radix_sort_pairs( h.keys0, h.vals0, h.keys1, h.vals1, h.hist, h.capP, h.fineBits, s );
+                cell_first_kernel<<<blocks( uint64_t( h.fineCells ) + 1, 256 ), 256, 0, s>>>( w3 ? h.keys1 : h.keys0, h.capP, &h.ctl->probeNb,
+                                                                                            h.fineCells, h.fineFirst );
+                gather_kernel<<<blocks( h.capP, 256 ), 256, 0, s>>>( w3 ? h.vals1 : h.vals0, h.sortedProbes, h.capP, &h.ctl->probeNb,
+                                                                     h.finePts, h.fineT );
+                singularity_fine_kernel<<<blocks( h.capP, WaveSize * FineWaves ), WaveSize * FineWaves, 0, s>>>(
+                    h.sortedProbes, h.probeSorted, h.finePts, h.fineT, h.fineFirst, &h.ctl->probeNb, p, h.fine, h.texCoord, h.singTex );
+            }
+            else
+                singularity_kernel<<<blocks( h.capP, WaveSize * SingWaves ), WaveSize * SingWaves, 0, s>>>(
+                    h.sortedProbes, h.probeSorted, h.pCellFirst, &h.ctl->probeNb, p, h.texCoord, h.singTex );
         }
 
         static sesdf_status enqueue( Handle & h )
@@ -1195,6 +1354,7 @@ extern "C"
         h.bits    = 0;
         while ( ( 1u << h.bits ) < h.cells )
             h.bits++;
+        cb::plan_fine_grid( h, box );
         hipError_t e = hipSetDevice( h.device );
         if ( e != hipSuccess )
             return fail( cb::hip_err( e, "hipSetDevice" ) );
@@ -1346,6 +1506,7 @@ extern "C"
         out->arc_capacity            = h.arcCap;
         out->build_attempts          = h.attempts;
         out->grid_dimensions         = uint32_t( h.prm.dim );
+        out->singularity_grid        = h.fineOn ? 1u : 0u;
         out->device_bytes            = h.fixedBytes + h.outBytes;
         out->build_serial            = h.serial;
         return SESDF_OK;
@@ -1357,9 +1518,22 @@ extern "C"
             return report_error( SESDF_ERR_INVALID, "null argument" );
         if ( t->arc_capacity != 0 && t->arc_capacity != cb::ArcCapSmall && t->arc_capacity != cb::ArcCapFull )
             return report_error( SESDF_ERR_INVALID, "arc_capacity must be 0, 8 or 80" );
+        if ( t->singularity_search > 2 )
+            return report_error( SESDF_ERR_INVALID, "singularity_search must be 0, 1 or 2" );
         cb::Handle & h = hh->h;
         CB_TRY( hipSetDevice( h.device ) );
         CB_TRY( hipStreamSynchronize( h.stream ) );
+        h.singSearch      = int( t->singularity_search );
+        const bool fineOn = h.singSearch == 2 || ( h.singSearch == 0 && h.fineBig );
+        if ( fineOn != h.fineOn )
+        {
+            h.fineOn = fineOn;
+            if ( h.graph )
+            {
+                CB_TRY( hipGraphExecDestroy( h.graph ) );
+                h.graph = nullptr;
+            }
+        }
         if ( t->arc_capacity && t->arc_capacity != h.arcCap )
         {
             h.arcCap = t->arc_capacity;

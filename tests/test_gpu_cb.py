@@ -160,3 +160,33 @@ def test_replay_matches_build(gpu):
     for k in ("probe_atoms", "probe_positions", "torus_atoms", "torus_positions", "neighbor_ids"):
         assert np.array_equal(first[k], again[k], equal_nan=True), k
     assert np.array_equal(first["data"]["tex_coords"], again["data"]["tex_coords"])
+
+
+@pytest.mark.parametrize("case,auto", [("globule_100k", 1), ("3JC8", 1), ("7CGO", 1), ("1AGA", 0), ("globule_2000", 0)])
+def test_singularity_search_grid_matches_stencil_scan(gpu, case, auto):
+    """The singularity texture from the 2 rp search grid is the reference's stencil scan, row for row:
+    the same build with either search gives identical texture coordinates and rows. The search grid
+    is chosen by itself where the reference's cells are large."""
+    if case == "globule_100k":
+        atoms, _ = synth.globule_voids(100_000)
+    elif case == "globule_2000":
+        atoms = synth.globule(2000, synth.SEED_BASE)
+    else:
+        atoms = load(os.path.join(MMTF, case + ".mmtf"))
+    cb = ContourBuildup(atoms)
+    assert cb.counts()["singularity_grid"] == auto
+    out = {}
+    for mode in (1, 2):
+        cb.set_tuning(singularity_search=mode)
+        cb.build()
+        assert cb.counts()["singularity_grid"] == (1 if mode == 2 else 0)
+        d = cb.get_data()
+        out[mode] = (d["tex_coords"].copy(), d["singularity_texture"].copy())
+    tc, order = out[1][0], cb.get_internal()["probe_sorted"]
+    assert np.array_equal(out[2][0], tc)
+    cnt = tc[order, 0].astype(np.int64)
+    rows1, rows2 = out[1][1], out[2][1]
+    for s in np.nonzero(cnt)[0]:
+        assert np.array_equal(rows1[s, :3 * cnt[s]], rows2[s, :3 * cnt[s]]), s
+    _say(case, "probes", len(order), "neighbour pairs", int(cnt.sum()), "full rows", int((cnt == 32).sum()))
+    assert cnt.sum() > 0
