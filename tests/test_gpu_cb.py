@@ -190,3 +190,14 @@ def test_singularity_search_grid_matches_stencil_scan(gpu, case, auto):
         assert np.array_equal(rows1[s, :3 * cnt[s]], rows2[s, :3 * cnt[s]]), s
     _say(case, "probes", len(order), "neighbour pairs", int(cnt.sum()), "full rows", int((cnt == 32).sum()))
     assert cnt.sum() > 0
+
+
+def test_singularity_search_value_is_checked(gpu):
+    from sesdf_amd import _lib as L
+
+    cb = ContourBuildup(synth.globule(300, synth.SEED_BASE))
+    with pytest.raises(L.SesdfError, match="singularity_search"):
+        cb.set_tuning(singularity_search=3)
+    cb.set_tuning(singularity_search=2)
+    cb.build()
+    assert cb.counts()["singularity_grid"] == 1

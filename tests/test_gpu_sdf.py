@@ -225,3 +225,21 @@ def test_sampling_after_enqueue_uses_the_new_build(gpu):
     torch.cuda.synchronize()
     # same surface; the two handles differ only in grid layout and sorted order (rounding)
     assert (out - ref).abs().max().item() <= 1e-5
+
+
+def test_grid_of_more_than_2_31_bricks_is_refused(gpu):
+    """A grid whose 4^3-vertex bricks number more than 2^31 (its output alone would need over 500 GB)
+    is refused on the host before any launch; the output pointer is never touched."""
+    from sesdf_amd import _lib as L
+
+    atoms = synth.globule(500, synth.SEED_BASE)
+    surf = Sesdf(atoms, probe_radius=1.4)
+    dims = (65536, 65536, 4096)  # 16384 x 16384 x 1024 bricks = 2^38
+    with pytest.raises(L.SesdfError, match="bricks"):
+        surf.sample_grid_ptr((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), dims, 16, band=2.0)
+    # a grid under the limit still samples
+    torch = _torch()
+    out = torch.empty(8 ** 3, device="cuda")
+    surf.sample_grid_ptr(atoms[:, :3].min(0), (1.0, 1.0, 1.0), (8, 8, 8), out.data_ptr(), band=2.0)
+    torch.cuda.synchronize()
+    assert np.isfinite(out.cpu().numpy()).all()
