@@ -5,7 +5,8 @@ with 10 empty voids of radius U(4, 8) A, probe 1.4 A, complete surface. A step i
 (grid, circles, intersections, segments, probe neighbours) on atoms already resident in HBM,
 replayed as one hipGraph. Further legs (same JSON line):
   exterior   the exterior path on the same atoms;
-  sdf        SDF of the complete surface on a 256^3 grid over the bounding box + 3 A;
+  sdf        SDF of the complete surface on a 256^3 grid over the bounding box + 3 A, then the
+             exterior field (voids excluded) on the same grid;
   protein    both surfaces of 7CGO (335,722 atoms, the reference's largest sample present), from MMTF;
   large      exterior path on 2.5M atoms (500 globules of 5,000 on a 360 A Fibonacci sphere) + memory;
   batch      4,096 molecules (N ~ logU[20, 5000]) in one launch sequence + 32^3 SDF grid each, vs
@@ -279,10 +280,20 @@ def leg_sdf(args, atoms, surf, out):
     e1.sync()
     ms = e0.elapsed_ms(e1) / reps
     r = res.cpu().numpy()
+    # the exterior field (voids excluded) on the same build and grid
+    surf.sample_grid_ptr(lo, sp, (dim,) * 3, res.data_ptr(), band=2.0, exterior=True)  # warm-up
+    L.device_synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        surf.sample_grid_ptr(lo, sp, (dim,) * 3, res.data_ptr(), band=2.0, exterior=True)
+    e1.record(stream)
+    e1.sync()
+    ms_ext = e0.elapsed_ms(e1) / reps
     out["sdf"] = {"grid": "256^3", "band_A": 2.0, "ms": ms, "samples_per_s": dim ** 3 / (ms * 1e-3),
                   "inside_fraction": float((r < 0).mean()), "surface_band_fraction": float((np.abs(r) < 2).mean()),
+                  "exterior_ms": ms_ext, "exterior_inside_fraction": float((res.cpu().numpy() < 0).mean()),
                   "tuning": tune}
-    log(f"sdf 256^3: {ms:.3f} ms")
+    log(f"sdf 256^3: {ms:.3f} ms, exterior field {ms_ext:.3f} ms")
 
 
 def leg_protein(args, out):
