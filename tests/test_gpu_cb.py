@@ -162,7 +162,7 @@ def test_replay_matches_build(gpu):
     assert np.array_equal(first["data"]["tex_coords"], again["data"]["tex_coords"])
 
 
-@pytest.mark.parametrize("case,auto", [("globule_100k", 1), ("3JC8", 1), ("7CGO", 1), ("1AGA", 0), ("globule_2000", 0)])
+@pytest.mark.parametrize("case,auto", [("globule_100k", 0), ("3JC8", 1), ("7CGO", 1), ("1A8R", 1), ("1AGA", 0), ("globule_2000", 0)])
 def test_singularity_search_grid_matches_stencil_scan(gpu, case, auto):
     """The singularity texture from the 2 rp search grid is the reference's stencil scan, row for row:
     the same build with either search gives identical texture coordinates and rows. The search grid
