@@ -981,6 +981,32 @@ namespace sesdf
         }
     }
 
+    // Few points in caller order (under the sort threshold, scattered as the caller gave them): one
+    // wave per point, evaluated cooperatively (the lanes share the point's stencil and test records
+    // 64 at a time), so a small query spreads over the CUs instead of one wave walking 64 stencils
+    // lane by lane. Lane 0 carries the point; the others only help.
+    template<bool EXT>
+    __global__ void __launch_bounds__( 256, EXT ? 5 : 8 ) sdf_point_waves_kernel( Buffers b, SdfArgs s )
+    {
+        __shared__ float4   stage[ 256 / WaveSize ][ StageSize ];
+        __shared__ uint64_t masks[ 256 / WaveSize ][ 2 * MaskRows ];
+        const uint64_t      q = uint64_t( blockIdx.x ) * ( blockDim.x / WaveSize ) + uint32_t( wave_in_block() );
+        if ( q >= s.n )
+            return;
+        SdfLane L;
+        L.active = lane_id() == 0;
+        L.q      = xyz( s.points[ q ] );
+        uint64_t    code;
+        const float v = sdf_eval_lanes<true, EXT>( b, s, 0, L, code, stage[ wave_in_block() ], masks[ wave_in_block() ] );
+        if ( L.active )
+        {
+            const uint64_t dst = s.perm ? uint64_t( s.perm[ q ] ) : q;
+            s.out[ dst ]       = v;
+            if ( s.codes )
+                s.codes[ dst ] = code;
+        }
+    }
+
     // Cell key (molecule 0's brick) and index of every point, for the sort into cell order.
     __global__ void __launch_bounds__( 256 ) sdf_point_keys_kernel( Buffers b, const float4 * pts, uint32_t n, uint32_t * keys, uint32_t * idx )
     {
@@ -1603,7 +1629,15 @@ extern "C"
                     return st;
                 sort_points_in( h, s, point_sort_space( p, SESDF_NOALLOC_MAX_POINTS ) );
             }
-            if ( s.exterior )
+            if ( npoints < PointSortMin ) // a wave per point
+            {
+                const uint32_t blocks = uint32_t( ( npoints + 3 ) / 4 );
+                if ( s.exterior )
+                    sdf_point_waves_kernel<true><<<blocks, 256, 0, h.stream>>>( h.b, s );
+                else
+                    sdf_point_waves_kernel<false><<<blocks, 256, 0, h.stream>>>( h.b, s );
+            }
+            else if ( s.exterior )
                 sdf_points_kernel<true><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
             else
                 sdf_points_kernel<false><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
@@ -1636,7 +1670,15 @@ extern "C"
                 if ( ps != SESDF_OK )
                     return ps;
             }
-            if ( s.exterior )
+            if ( npoints < PointSortMin ) // a wave per point
+            {
+                const uint32_t blocks = uint32_t( ( npoints + 3 ) / 4 );
+                if ( s.exterior )
+                    sdf_point_waves_kernel<true><<<blocks, 256, 0, h.stream>>>( h.b, s );
+                else
+                    sdf_point_waves_kernel<false><<<blocks, 256, 0, h.stream>>>( h.b, s );
+            }
+            else if ( s.exterior )
                 sdf_points_kernel<true><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
             else
                 sdf_points_kernel<false><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, s );
