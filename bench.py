@@ -6,7 +6,8 @@ with 10 empty voids of radius U(4, 8) A, probe 1.4 A, complete surface. A step i
 replayed as one hipGraph. Further legs (same JSON line):
   exterior   the exterior path on the same atoms;
   sdf        SDF of the complete surface on a 256^3 grid over the bounding box + 3 A, then the
-             exterior field (voids excluded) on the same grid;
+             exterior field (voids excluded) on the same grid, then small queries at 64 and 4,096
+             scattered points;
   protein    both surfaces of 7CGO (335,722 atoms, the reference's largest sample present), from MMTF;
   large      exterior path on 2.5M atoms (500 globules of 5,000 on a 360 A Fibonacci sphere) + memory;
   batch      4,096 molecules (N ~ logU[20, 5000]) in one launch sequence + 32^3 SDF grid each, vs
@@ -289,11 +290,27 @@ def leg_sdf(args, atoms, surf, out):
     e1.record(stream)
     e1.sync()
     ms_ext = e0.elapsed_ms(e1) / reps
+    # small queries at scattered points (caller order, under the point-sort threshold): latency
+    g = torch.Generator(device="cuda").manual_seed(7)
+    small = {}
+    for n in (64, 4096):
+        P = torch.zeros(n, 4, device="cuda")
+        P[:, :3] = torch.tensor(lo, device="cuda") + torch.tensor(hi - lo, device="cuda") * torch.rand(n, 3, device="cuda", generator=g)
+        v = torch.empty(n, device="cuda")
+        surf.sample_points_ptr(P.data_ptr(), n, v.data_ptr(), band=2.0)  # warm-up
+        L.device_synchronize()
+        e0.record(stream)
+        for _ in range(20):
+            surf.sample_points_ptr(P.data_ptr(), n, v.data_ptr(), band=2.0)
+        e1.record(stream)
+        e1.sync()
+        small[f"points_{n}_ms"] = e0.elapsed_ms(e1) / 20
     out["sdf"] = {"grid": "256^3", "band_A": 2.0, "ms": ms, "samples_per_s": dim ** 3 / (ms * 1e-3),
                   "inside_fraction": float((r < 0).mean()), "surface_band_fraction": float((np.abs(r) < 2).mean()),
                   "exterior_ms": ms_ext, "exterior_inside_fraction": float((res.cpu().numpy() < 0).mean()),
-                  "tuning": tune}
-    log(f"sdf 256^3: {ms:.3f} ms, exterior field {ms_ext:.3f} ms")
+                  "scattered": small, "tuning": tune}
+    log(f"sdf 256^3: {ms:.3f} ms, exterior field {ms_ext:.3f} ms; scattered points: "
+        + ", ".join(f"{k} {v:.3f}" for k, v in small.items()))
 
 
 def leg_protein(args, out):
