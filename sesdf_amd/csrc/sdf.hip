@@ -1816,10 +1816,11 @@ extern "C"
         sesdf_status st = sdf_common( h, s, flags );
         if ( st != SESDF_OK || s.empty )
             return st; // no atoms: no gradient
-        if ( s.recompute && npoints >= PointSortMin && npoints <= PointSortMax )
+        if ( s.recompute && npoints && npoints <= PointSortMax )
         {
-            // winners re-derived by a forward pass in cell order (cooperative waves) into scratch
-            // codes, instead of one thread per query: the backward then reads them as stored ones
+            // winners re-derived by a cooperative forward pass into scratch codes (in cell order
+            // from PointSortMin points, one wave per point below), instead of one thread per query:
+            // the backward then reads them as stored ones
             SdfState & d = h.sdf;
             if ( npoints > d.ptEvalCap )
             {
@@ -1837,13 +1838,24 @@ extern "C"
             f.codes     = d.ptCodes;
             f.weights   = nullptr;
             f.recompute = 0;
-            sesdf_status ps = sort_points( h, f );
-            if ( ps != SESDF_OK )
-                return ps;
-            if ( f.exterior )
-                sdf_points_kernel<true><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, f );
+            if ( npoints >= PointSortMin )
+            {
+                sesdf_status ps = sort_points( h, f );
+                if ( ps != SESDF_OK )
+                    return ps;
+                if ( f.exterior )
+                    sdf_points_kernel<true><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, f );
+                else
+                    sdf_points_kernel<false><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, f );
+            }
             else
-                sdf_points_kernel<false><<<grid_blocks( npoints ), 256, 0, h.stream>>>( h.b, f );
+            {
+                const uint32_t blocks = uint32_t( ( npoints + 3 ) / 4 );
+                if ( f.exterior )
+                    sdf_point_waves_kernel<true><<<blocks, 256, 0, h.stream>>>( h.b, f );
+                else
+                    sdf_point_waves_kernel<false><<<blocks, 256, 0, h.stream>>>( h.b, f );
+            }
             s.codes     = d.ptCodes;
             s.recompute = 0;
         }

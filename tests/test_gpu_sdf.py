@@ -124,8 +124,8 @@ def test_sdf_grid_64_config0(gpu, exterior):
 
 @pytest.mark.parametrize("store,count", [(True, 60000), (False, 60000), (False, 5000)])
 def test_backward_matches_fp64_gradient(gpu, store, count):
-    """Stored winners; recomputed ones (from 8,192 queries on by a forward pass in cell order into
-    scratch codes, below that one thread per query)."""
+    """Stored winners; recomputed ones (a cooperative forward pass into scratch codes: in cell order
+    from 8,192 queries on, one wave per query below)."""
     torch = _torch()
     atoms = synth.globule(3000, synth.SEED_BASE + 61)
     surf = Sesdf(atoms)
