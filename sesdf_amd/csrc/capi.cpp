@@ -107,6 +107,7 @@ namespace sesdf
         if ( graphExec )
             (void)hipGraphExecDestroy( graphExec );
         graphExec = nullptr;
+        graphWarm = false;
         if ( g2t )
             (void)hipFree( g2t );
         g2t       = nullptr;
@@ -309,7 +310,9 @@ namespace sesdf
         h.stageNext = 0;
         if ( h.n == 0 )
             return SESDF_OK;
-        if ( h.useGraph )
+        // the first build on new buffers runs its stages directly: a handle built once (or rebuilt
+        // once after sesdf_release) does not pay a graph capture and instantiation it never replays
+        if ( h.useGraph && h.graphWarm )
         {
             if ( !h.graphExec )
             {
@@ -323,7 +326,10 @@ namespace sesdf
             HIP_TRY( hipGraphLaunch( h.graphExec, h.stream ) );
         }
         else
+        {
             enqueue_stages( h, h.stream, nullptr );
+            h.graphWarm = true;
+        }
         HIP_TRY( hipGetLastError() );
         return SESDF_OK;
     }

@@ -89,6 +89,7 @@ namespace sesdf
         uint64_t g2tSerial = ~0ull;
 
         hipGraphExec_t graphExec = nullptr;
+        bool           graphWarm = false; // a build has run on the current buffers: capture from the next one
         hipEvent_t     events[ SESDF_STAGE_COUNT + 1 ] = {};
 
         SdfState            sdf;
