@@ -65,6 +65,26 @@ namespace sesdf
     // passes no test: status 2, a full circle with NaN geometry.)
     __device__ __forceinline__ bool no_circle( const CircleTest & c ) { return !( c.radius2 > 0.f ); }
 
+    // Inclusive prefix maximum over the wave by DPP (row shifts by 1, 2, 4, 8, then the row 15 and
+    // lane 31 broadcasts), inside the VALU. A lane without a DPP source keeps its own value, and
+    // a maximum may take in any earlier lane more than once, so full row masks are exact. Every
+    // lane must be active.
+    template<int Ctrl>
+    __device__ __forceinline__ int dpp_from( int v )
+    {
+        return __builtin_amdgcn_update_dpp( v, v, Ctrl, 0xf, 0xf, false );
+    }
+    __device__ __forceinline__ int wave_incl_max( int v )
+    {
+        v = max( v, dpp_from<0x111>( v ) ); // row_shr:1
+        v = max( v, dpp_from<0x112>( v ) ); // row_shr:2
+        v = max( v, dpp_from<0x114>( v ) ); // row_shr:4
+        v = max( v, dpp_from<0x118>( v ) ); // row_shr:8
+        v = max( v, dpp_from<0x142>( v ) ); // row_bcast:15
+        v = max( v, dpp_from<0x143>( v ) ); // row_bcast:31
+        return v;
+    }
+
     // orthogonalVector (sesdf/include/bcs/cuda/math.cuh:32-38)
     __device__ __forceinline__ f3 orthogonal_vector( f3 n )
     {
@@ -114,7 +134,8 @@ namespace sesdf
         __shared__ float4   sAtom[ WAVES ][ KMAX ];   // (c_k - c_i, Rk^2)
         __shared__ float2   sAux[ WAVES ][ KMAX ];    // (|c_k - c_i|^2, Rk^2 + eps - |c_k - c_i|^2 - Ri^2)
         __shared__ uint32_t sRowStart[ WAVES ][ WaveSize ];
-        __shared__ uint32_t sRowPrefix[ WAVES ][ WaveSize + 1 ];
+        __shared__ uint32_t sRowPrefix[ WAVES ][ WaveSize ];     // first element of each row
+        __shared__ uint32_t sRowOf[ WAVES ][ WaveSize ];         // row marks of one 64-element window
 
         const int      w    = wave_in_block();
         const int      lane = lane_id();
@@ -167,32 +188,29 @@ namespace sesdf
                 }
             }
             const uint32_t incl  = wave_incl_scan( len );
+            const uint32_t excl  = incl - len;
             const uint32_t total = wave_bcast( incl, WaveSize - 1 );
-            sRowStart[ w ][ lane ]      = st;
-            sRowPrefix[ w ][ lane + 1 ] = incl;
-            if ( lane == 0 )
-                sRowPrefix[ w ][ 0 ] = 0;
-            wave_sync();
-            const int rowsHere = min( rows - rowBase, WaveSize );
+            sRowStart[ w ][ lane ]  = st;
+            sRowPrefix[ w ][ lane ] = excl;
             for ( uint32_t base = 0; base < total; base += WaveSize )
             {
+                // Row of element e = base + lane: each non-empty row of the window [base, base + 64)
+                // marks its first element there with its index (the row holding element `base`
+                // marks slot 0), and a prefix maximum carries the marks forward, since rows
+                // ascend with their elements.
+                sRowOf[ w ][ lane ] = 0u;
+                wave_sync();
+                if ( len > 0u && excl < base + WaveSize && incl > base )
+                    sRowOf[ w ][ max( excl, base ) - base ] = uint32_t( lane );
+                wave_sync();
+                const int      row  = wave_incl_max( int( sRowOf[ w ][ lane ] ) );
                 const uint32_t e    = base + lane;
                 bool           pass = false;
                 uint32_t       j    = 0;
                 float4         aj   = make_float4( 0.f, 0.f, 0.f, 0.f );
                 if ( e < total )
                 {
-                    // row r: largest r with prefix[r] <= e
-                    int lo = 0, hi = rowsHere;
-                    while ( hi - lo > 1 )
-                    {
-                        const int mid = ( lo + hi ) >> 1;
-                        if ( sRowPrefix[ w ][ mid ] <= e )
-                            lo = mid;
-                        else
-                            hi = mid;
-                    }
-                    j                = sRowStart[ w ][ lo ] + ( e - sRowPrefix[ w ][ lo ] );
+                    j                = sRowStart[ w ][ row ] + ( e - sRowPrefix[ w ][ row ] );
                     aj               = b.atoms[ j ];
                     const f3    d    = xyz( aj ) - ci;
                     const float Rs   = Ri + ( aj.w + rp );
