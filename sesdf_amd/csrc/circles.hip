@@ -65,26 +65,6 @@ namespace sesdf
     // passes no test: status 2, a full circle with NaN geometry.)
     __device__ __forceinline__ bool no_circle( const CircleTest & c ) { return !( c.radius2 > 0.f ); }
 
-    // Inclusive prefix maximum over the wave by DPP (row shifts by 1, 2, 4, 8, then the row 15 and
-    // lane 31 broadcasts), inside the VALU. A lane without a DPP source keeps its own value, and
-    // a maximum may take in any earlier lane more than once, so full row masks are exact. Every
-    // lane must be active.
-    template<int Ctrl>
-    __device__ __forceinline__ int dpp_from( int v )
-    {
-        return __builtin_amdgcn_update_dpp( v, v, Ctrl, 0xf, 0xf, false );
-    }
-    __device__ __forceinline__ int wave_incl_max( int v )
-    {
-        v = max( v, dpp_from<0x111>( v ) ); // row_shr:1
-        v = max( v, dpp_from<0x112>( v ) ); // row_shr:2
-        v = max( v, dpp_from<0x114>( v ) ); // row_shr:4
-        v = max( v, dpp_from<0x118>( v ) ); // row_shr:8
-        v = max( v, dpp_from<0x142>( v ) ); // row_bcast:15
-        v = max( v, dpp_from<0x143>( v ) ); // row_bcast:31
-        return v;
-    }
-
     // orthogonalVector (sesdf/include/bcs/cuda/math.cuh:32-38)
     __device__ __forceinline__ f3 orthogonal_vector( f3 n )
     {

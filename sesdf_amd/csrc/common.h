@@ -84,6 +84,25 @@ namespace sesdf
         }
         return v;
     }
+    // Inclusive prefix maximum over the wave by DPP (row shifts by 1, 2, 4, 8, then the row 15 and
+    // lane 31 broadcasts), inside the VALU. A lane without a DPP source keeps its own value, and
+    // a maximum may take in any earlier lane more than once, so full row masks are exact. Every
+    // lane must be active.
+    template<int Ctrl>
+    __device__ __forceinline__ int dpp_from( int v )
+    {
+        return __builtin_amdgcn_update_dpp( v, v, Ctrl, 0xf, 0xf, false );
+    }
+    __device__ __forceinline__ int wave_incl_max( int v )
+    {
+        v = max( v, dpp_from<0x111>( v ) ); // row_shr:1
+        v = max( v, dpp_from<0x112>( v ) ); // row_shr:2
+        v = max( v, dpp_from<0x114>( v ) ); // row_shr:4
+        v = max( v, dpp_from<0x118>( v ) ); // row_shr:8
+        v = max( v, dpp_from<0x142>( v ) ); // row_bcast:15
+        v = max( v, dpp_from<0x143>( v ) ); // row_bcast:31
+        return v;
+    }
 
     // ------------------------------------------------------------------------------------------
     // Device control block: tickets, counts and overflow flags of one build. Reset by the first

@@ -708,7 +708,8 @@ namespace sesdf
     __global__ void __launch_bounds__( 256 ) probe_neighbors_kernel( Buffers b )
     {
         __shared__ uint32_t sRowStart[ 4 ][ WaveSize ];
-        __shared__ uint32_t sRowPrefix[ 4 ][ WaveSize + 1 ];
+        __shared__ uint32_t sRowPrefix[ 4 ][ WaveSize ]; // first record of each row
+        __shared__ uint32_t sRowOf[ 4 ][ WaveSize ];     // row marks of one 64-record window
         const uint32_t      Iw = b.ctl->intersectionWithNbNb;
         if ( Iw + b.ctl->intersectionNoNbNb > b.capRecords || Iw > b.capProbeNb || ( b.ctl->overflow & OverflowIntersections ) )
             return;
@@ -747,30 +748,27 @@ namespace sesdf
                 len               = b.wStart[ a1 ] - st;
             }
             const uint32_t incl  = wave_incl_scan( len );
+            const uint32_t excl  = incl - len;
             const uint32_t total = wave_bcast( incl, WaveSize - 1 );
-            sRowStart[ w ][ lane ]      = st;
-            sRowPrefix[ w ][ lane + 1 ] = incl;
-            if ( lane == 0 )
-                sRowPrefix[ w ][ 0 ] = 0;
-            wave_sync();
-            const int rowsHere = min( rows - rowBase, WaveSize );
+            sRowStart[ w ][ lane ]  = st;
+            sRowPrefix[ w ][ lane ] = excl;
             for ( uint32_t base = 0; base < total; base += WaveSize )
             {
+                // row of record e = base + lane: each non-empty row marks its first record of the
+                // window (the row holding record `base` marks slot 0); a prefix maximum carries
+                // the marks forward
+                sRowOf[ w ][ lane ] = 0u;
+                wave_sync();
+                if ( len > 0u && excl < base + WaveSize && incl > base )
+                    sRowOf[ w ][ max( excl, base ) - base ] = uint32_t( lane );
+                wave_sync();
+                const int      row = wave_incl_max( int( sRowOf[ w ][ lane ] ) );
                 const uint32_t e   = base + lane;
                 bool           hit = false;
                 float4         pr  = make_float4( 0.f, 0.f, 0.f, 0.f );
                 if ( e < total )
                 {
-                    int lo = 0, hi = rowsHere; // row: largest with prefix <= e
-                    while ( hi - lo > 1 )
-                    {
-                        const int mid = ( lo + hi ) >> 1;
-                        if ( sRowPrefix[ w ][ mid ] <= e )
-                            lo = mid;
-                        else
-                            hi = mid;
-                    }
-                    const uint32_t rec = sRowStart[ w ][ lo ] + ( e - sRowPrefix[ w ][ lo ] );
+                    const uint32_t rec = sRowStart[ w ][ row ] + ( e - sRowPrefix[ w ][ row ] );
                     if ( rec != q )
                     {
                         pr  = b.concavePos[ rec ];
